@@ -1,0 +1,157 @@
+"""Draw-path throughput on one MI355X.
+
+A step is one frame: the recorded command buffer (attachment clears + all
+draws of the workload) submitted once through vkQueueSubmit. Geometry,
+indices and textures are resident in HBM before the timed region; the timed
+region is K back-to-back submissions bracketed by vkQueueWaitIdle.
+
+Prints one JSON line: value = fragments/s (covered (triangle, sample) pairs
+before the depth test, the count the CPU oracle reproduces), ms_per_step =
+frame time, a roofline object for the dominant kernel (HIP-event timed on the
+queue's stream) and a cpu_baseline object (the C++ CPU restatement, run on
+this host's cores on the same input buffers).
+
+  python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload grid_1080p]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fragments/sec + frame ms at 1080p, learned/batched vs plain GPU pipeline"
+
+
+def algorithmic_bytes(scene, fragments):
+    """Geometry and texture read once + per fragment D32 read+write and RGBA8
+    write (+ RGBA8 read when blending)."""
+    geo = scene.vertices.nbytes + (scene.indices.nbytes if scene.indices is not None else 0)
+    tex = scene.texture.nbytes if scene.texture is not None else 0
+    per_frag = 16 if scene.blend[0] else 12
+    return geo + tex + fragments * per_frag, per_frag
+
+
+def cpu_baseline(scene, fragments, seconds):
+    from oracle import bridge  # the CPU restatement: measured beside the GPU, never the product
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(times) < 2:
+        t0 = time.perf_counter()
+        _, _, _, st = bridge.render(scene, threads=threads)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 200:
+            break
+    assert st["fragments"] == fragments, "CPU and GPU fragment counts differ"
+    med = statistics.median(times)
+    return {"value": fragments / med, "unit": "fragments/s", "cores": threads, "kind": "port",
+            "frame_ms": med * 1e3,
+            "sample": "full %s frame (%d fragments), %d repetitions, median" % (scene.name, fragments, len(times))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="grid_1080p")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the draw path runs on a single MI355X (multi-GPU sharding is a "
+              "design note, not a measured configuration); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from kazan_amd.scenes import generators as g
+    from kazan_amd.scenes.renderer import Context, SceneRenderer
+
+    scene = g.WORKLOADS[args.workload]()
+    ctx = Context()
+    r = SceneRenderer(ctx, scene)
+    print("bench: %s on %s, %d triangles" % (scene.name, ctx.device_name, scene.num_triangles), file=sys.stderr,
+          flush=True)
+    if args.warmup:
+        r.render(args.warmup)
+
+    # timed region: K frames back to back
+    ctx.wait_idle()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.submit(r.cb, wait=False)
+    ctx.wait_idle()
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    wall = t1 - t0
+    frags_per_frame = st["fragments"] // args.steps
+    print("bench: %d frames in %.3f s, gpu %.3f ms total" % (args.steps, wall, st["gpu_ms"]), file=sys.stderr,
+          flush=True)
+
+    # per-kernel times (HIP events on the queue's stream), separate pass
+    ctx.set_kernel_timing(True)
+    n_prof = max(5, min(args.steps, 20))
+    r.render(n_prof)
+    ktimes = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    dom_ms, dom_n = ktimes[dom]
+    avg_launch_s = dom_ms / max(dom_n, 1) / 1e3
+    alg_bytes, per_frag = algorithmic_bytes(scene, frags_per_frame)
+    achieved = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("raster_hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": frags_per_frame * args.steps / wall,
+        "unit": "fragments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene generator)",
+        "config": {
+            "workload": scene.name,
+            "width": scene.width,
+            "height": scene.height,
+            "samples": scene.samples,
+            "triangles": scene.num_triangles,
+            "fragments_per_frame": frags_per_frame,
+            "gpu_ms_per_frame": st["gpu_ms"] / args.steps,
+            "batches_per_frame": st["batches"] / args.steps,
+            "kernel_ms_per_frame": {k: v[0] / n_prof for k, v in ktimes.items()},
+        },
+        "roofline": {
+            "kernel": "kz_raster" if dom == "raster" else "kz_" + dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes": alg_bytes,
+            "bytes_per_fragment": per_frag,
+            "avg_launch_ms": avg_launch_s * 1e3,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, frags_per_frame, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

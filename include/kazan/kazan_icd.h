@@ -1,0 +1,103 @@
+/*
+ * kazan_icd.h — the C ABI libkazan_driver.so exports.
+ *
+ * These are the three symbols the reference's cdylib exports
+ * (vulkan-driver/Cargo.toml:10-12 crate-type cdylib, vulkan-driver/src/lib.rs)
+ * and through which the Vulkan loader reaches every vk* entry point:
+ *
+ *   vk_icdGetInstanceProcAddr
+ *       replaces vulkan-driver/src/lib.rs:37-43 (forwards to
+ *       vkGetInstanceProcAddr, api_impl.rs:3067-3086, which resolves names
+ *       through get_proc_address, api_impl.rs:432-860).
+ *   vk_icdNegotiateLoaderICDInterfaceVersion
+ *       replaces lib.rs:53-61: clamps *pSupportedVersion to ICD_VERSION = 5
+ *       (lib.rs:51) and returns VK_SUCCESS.
+ *   vk_icdGetPhysicalDeviceProcAddr
+ *       replaces lib.rs:67-128: returns the entry point for physical-device
+ *       level names, NULL for any other name.
+ *
+ * Dispatchable handles (VkInstance, VkPhysicalDevice, VkDevice, VkQueue,
+ * VkCommandBuffer) point at objects whose first word holds ICD_LOADER_MAGIC
+ * (handle.rs:23-36); non-dispatchable handles are object pointers
+ * (handle.rs:121-159). pAllocator is accepted and ignored, as in the
+ * reference (e.g. api_impl.rs:3437).
+ *
+ * The draw-path entry points the reference leaves unimplemented and this
+ * driver implements, all reached through vk_icdGetInstanceProcAddr /
+ * vkGetDeviceProcAddr:
+ *   vkCmdDraw               api_impl.rs:4632-4640
+ *   vkCmdDrawIndexed        api_impl.rs:4643-4652
+ *   vkCreateGraphicsPipelines api_impl.rs:4018-4032 -> pipeline.rs:165-183
+ *   vkQueueSubmit           api_impl.rs:3414-3421
+ *   vkCmdBeginRenderPass / vkCmdEndRenderPass   :4944 / :4961
+ *   vkCmdBindPipeline :4506, vkCmdSetViewport :4515, vkCmdSetScissor :4525,
+ *   vkCmdBindDescriptorSets :4597, vkCmdBindIndexBuffer :4611,
+ *   vkCmdBindVertexBuffers :4621, vkCmdPushConstants :4932,
+ *   vkCmdCopyBuffer, vkCmdCopyBufferToImage, vkCmdCopyImageToBuffer :4746,
+ *   vkCreateCommandPool :4436, vkAllocateCommandBuffers :4464,
+ *   vkBegin/EndCommandBuffer :4483/:4491, vkCreateFramebuffer :4383-4390,
+ *   vkCreateRenderPass :4402-4415, vkCreateFence :3653, vkWaitForFences :3689,
+ *   vkQueueWaitIdle :3424.
+ *
+ * One device-level extension entry point, resolved by name through
+ * vkGetDeviceProcAddr, has no counterpart in the reference: it reports the
+ * draw statistics of the submissions completed since the last call.
+ */
+#ifndef KAZAN_ICD_H
+#define KAZAN_ICD_H
+
+#include "vk_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KAZAN_ICD_VERSION 5u
+
+VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetInstanceProcAddr(VkInstance instance, const char *pName);
+VKAPI_ATTR VkResult VKAPI_CALL vk_icdNegotiateLoaderICDInterfaceVersion(uint32_t *pSupportedVersion);
+VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetPhysicalDeviceProcAddr(VkInstance instance, const char *pName);
+
+/* Counters of vkGetDrawStatsKAZAN, in order. */
+enum KazanDrawStat {
+    KAZAN_STAT_TRIANGLES_IN = 0,          /* primitives assembled */
+    KAZAN_STAT_TRIANGLES_SETUP = 1,       /* survived clip-reject, cull, zero area */
+    KAZAN_STAT_TRIANGLES_UNSUPPORTED = 2, /* needed z or guard-band clipping: dropped */
+    KAZAN_STAT_FRAGMENTS = 3,             /* covered (triangle, sample) pairs */
+    KAZAN_STAT_DEPTH_PASSED = 4,          /* samples that passed the depth test */
+    KAZAN_STAT_LIST_ENTRIES = 5,          /* binned (triangle, tile) pairs */
+    KAZAN_STAT_LIST_OVERFLOW = 6,         /* batches whose tile lists overflowed */
+    KAZAN_STAT_COUNT = 8
+};
+
+/* vkGetDrawStatsKAZAN: waits for the device's queue, then writes the
+ * KAZAN_STAT_COUNT counters accumulated over the submissions since the
+ * previous call, the GPU time from the first of them to the end of the last
+ * (pGpuMs), and the number of batches and kernel launches. */
+typedef VkResult(VKAPI_PTR *PFN_vkGetDrawStatsKAZAN)(VkDevice device, uint64_t *pCounters, float *pGpuMs,
+                                                     uint32_t *pBatches, uint32_t *pLaunches);
+
+/* Kernels of one draw batch, in launch order, for per-kernel timing. */
+enum KazanKernel {
+    KAZAN_KERNEL_CLEAR_COUNTS = 0,
+    KAZAN_KERNEL_BIN_COUNT = 1,
+    KAZAN_KERNEL_BIN_SCAN = 2,
+    KAZAN_KERNEL_BIN_SCATTER = 3,
+    KAZAN_KERNEL_BIN_SORT = 4,
+    KAZAN_KERNEL_RASTER = 5,
+    KAZAN_KERNEL_COUNT = 6
+};
+
+/* vkSetKernelTimingKAZAN: records a HIP event pair around every batch kernel
+ * on the queue's stream from the next submission on. vkGetKernelTimesKAZAN
+ * waits for the queue and writes, per KazanKernel, the summed milliseconds
+ * and the number of launches of the submissions since the previous
+ * statistics read. Neither has a counterpart in the reference. */
+typedef void(VKAPI_PTR *PFN_vkSetKernelTimingKAZAN)(VkDevice device, VkBool32 enable);
+typedef VkResult(VKAPI_PTR *PFN_vkGetKernelTimesKAZAN)(VkDevice device, float *pMs, uint32_t *pLaunches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAZAN_ICD_H */

@@ -1,0 +1,66 @@
+/*
+ * kazan_tools.h — shader-compiler entry points exported beside the ICD for
+ * tests and tooling. They need no GPU and no Vulkan objects.
+ *
+ *   kazan_spirv_disassemble      counterpart of the reference's dump of a
+ *                                parsed module (spirv-parser/src/lib.rs:45-216
+ *                                golden tests)
+ *   kazan_translate_to_ir_text   counterpart of TranslatedSPIRVShader::new
+ *                                (shader-compiler-translate-spirv-to-ir/src/
+ *                                lib.rs:191-215) followed by the IR's text form
+ *                                (assert_ir_matches_file!, shader-compiler-ir/
+ *                                src/macros.rs:39-48)
+ *   kazan_compile_pipeline_spirv runs the whole pipeline compile (SPIR-V -> IR
+ *                                -> HIP source -> gfx950 code object) without
+ *                                a device, as vkCreateGraphicsPipelines does
+ *
+ * All return 0 on success and a negative value on failure, with a message in
+ * `out`. Output longer than `cap` is truncated; *out_len receives the full
+ * length.
+ */
+#ifndef KAZAN_TOOLS_H
+#define KAZAN_TOOLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stage: 0 = vertex, 4 = fragment, 5 = compute (SPIR-V execution models) */
+int kazan_spirv_disassemble(const uint32_t *words, size_t nwords, char *out, size_t cap, size_t *out_len);
+int kazan_translate_to_ir_text(const uint32_t *words, size_t nwords, uint32_t stage, const char *entry, char *out,
+                               size_t cap, size_t *out_len);
+
+typedef struct KazanPipelineDesc {
+    const uint32_t *vs_words;
+    size_t vs_nwords;
+    const uint32_t *fs_words; /* may be NULL */
+    size_t fs_nwords;
+    /* vertex input: up to 8 bindings and 16 attributes */
+    uint32_t num_bindings;
+    uint32_t binding_stride[8];
+    uint32_t binding_rate[8];
+    uint32_t num_attributes;
+    uint32_t attr_location[16], attr_binding[16], attr_format[16], attr_offset[16];
+    /* fixed function */
+    uint32_t tile_shift, samples, cull_mode, front_face;
+    uint32_t depth_test, depth_write, depth_op;
+    uint32_t blend, src_color, dst_color, color_op, src_alpha, dst_alpha, alpha_op, write_mask;
+    /* descriptor slots: (set << 16 | binding) of up to 4 uniform buffers and
+     * 4 combined image samplers, in slot order */
+    uint32_t num_ubos, ubo_key[4];
+    uint32_t num_textures, texture_key[4];
+} KazanPipelineDesc;
+
+/* what: 0 = generated shader source, 1 = full kernel source, 2 = compile to a
+ * code object and report its size and whether the cache held it */
+int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,
+                                 size_t *out_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAZAN_TOOLS_H */

@@ -1,0 +1,96 @@
+// device_common.h — data shared by the host engine, the prebuilt binning
+// kernels (kernels.hip) and the per-pipeline kernels that pipeline creation
+// compiles from raster_template.hip. Plain structs only: they are passed by
+// value as kernel arguments and copied into device memory as-is, so the
+// layout must be identical in the host compiler and in device code.
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <stdint.h>
+#else
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::int64_t int64_t;
+#endif
+
+#define KZ_MAX_BINDINGS 8
+#define KZ_MAX_TEXTURES 4
+#define KZ_MAX_UBOS 4
+#define KZ_PUSH_WORDS 32
+#define KZ_MAX_VARYINGS 16
+
+// Per-draw record. One record per (vkCmdDraw*, instance).
+struct KzDraw {
+    const uint8_t *vb[KZ_MAX_BINDINGS];  // binding base = buffer address + bind offset
+    const uint8_t *ib;                   // index buffer base (+ bind offset)
+    uint32_t index_type;                 // 0 = u16, 1 = u32, 2 = non-indexed
+    uint32_t first;                      // firstIndex (indexed) or firstVertex
+    int32_t vertex_offset;               // vertexOffset (indexed only)
+    uint32_t instance;                   // gl_InstanceIndex of this record
+    uint32_t tri_count;                  // triangles in this record
+    uint32_t tri_begin;                  // index of its first triangle in the batch
+    uint32_t pad[2];
+    uint32_t push[KZ_PUSH_WORDS];        // push-constant bytes at draw time
+};
+
+// Combined image sampler as the fragment kernel sees it. Images are linear
+// RGBA8, row pitch = w * 4 bytes.
+struct KzTexture {
+    const uint8_t *data;
+    uint32_t w, h;
+    uint32_t filter;  // VkFilter
+    uint32_t addr_u, addr_v;
+    uint32_t pad;
+};
+
+// One batch of draws that share a pipeline, dynamic state and descriptors.
+struct KzFrame {
+    const KzDraw *draws;
+    uint32_t ndraws;
+    uint32_t total_tris;
+
+    // attachments: colour RGBA8 and depth D32, W*H*samples, samples
+    // interleaved per pixel
+    uint8_t *color;
+    float *depth;
+    uint32_t width, height;
+    uint32_t tiles_x, tiles_y;
+
+    // viewport transform constants, precomputed on the host in the same
+    // order the oracle uses: hw = w*0.5f, ox = x + hw, ...
+    float vp_hw, vp_hh, vp_ox, vp_oy, vp_dz, vp_zmin;
+    float z_lo, z_hi;  // depth clamp range
+    // scissor ∩ framebuffer, inclusive pixel rectangle
+    int32_t sc_x0, sc_y0, sc_x1, sc_y1;
+
+    KzTexture tex[KZ_MAX_TEXTURES];
+    const uint8_t *ubo[KZ_MAX_UBOS];
+
+    // binning scratch (device memory owned by the engine)
+    uint32_t *tile_counts;   // [tiles]
+    uint32_t *tile_offsets;  // [tiles] exclusive prefix of counts
+    uint32_t *tile_cursor;   // [tiles] scatter cursors
+    uint32_t *tile_lists;    // [list_capacity] triangle ids, per tile, sorted
+    uint32_t *sort_tmp;      // [list_capacity] merge scratch for long lists
+    uint32_t list_capacity;
+    uint32_t pad0;
+    uint64_t *tri_rect;      // [total_tris] packed tile rectangle or ~0
+    unsigned long long *stats;  // KZ_STAT_* counters
+};
+
+enum {
+    KZ_STAT_TRIANGLES_IN = 0,
+    KZ_STAT_TRIANGLES_SETUP = 1,
+    KZ_STAT_TRIANGLES_UNSUPPORTED = 2,
+    KZ_STAT_FRAGMENTS = 3,
+    KZ_STAT_DEPTH_PASSED = 4,
+    KZ_STAT_LIST_ENTRIES = 5,
+    KZ_STAT_LIST_OVERFLOW = 6,
+    KZ_STAT_COUNT = 8
+};
+
+// Tile rectangle packing: tx0 | ty0 << 16 | tx1 << 32 | ty1 << 48.
+#define KZ_RECT_EMPTY (~0ULL)

@@ -1,0 +1,161 @@
+// pipeline_compiler.cpp — per-pipeline kernel source assembly, hiprtc
+// compilation for gfx950 and the on-disk code-object cache.
+//
+// The cache is the counterpart of the pipeline cache the reference leaves as
+// a stub (vkCreatePipelineCache, vulkan-driver/src/api_impl.rs:3979; nil
+// pipelineCacheUUID :1489-1492): code objects are keyed by a 64-bit FNV-1a
+// hash of the complete kernel source and the compile options, stored as
+// <dir>/<hash>.hsaco. The directory is $KAZAN_PIPELINE_CACHE, else
+// _pipeline_cache/ beside the driver library.
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+#include "_embedded_sources.h"
+#include "engine.h"
+
+namespace kz {
+
+namespace {
+
+const char *const kCompileOptions[] = {
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+};
+constexpr int kNumOptions = sizeof(kCompileOptions) / sizeof(kCompileOptions[0]);
+
+uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull) {
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string cache_dir() {
+    if (const char *e = std::getenv("KAZAN_PIPELINE_CACHE")) return e;
+    Dl_info info;
+    if (dladdr((void *)&cache_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        auto slash = p.rfind('/');
+        if (slash != std::string::npos) return p.substr(0, slash) + "/_pipeline_cache";
+    }
+    return "_pipeline_cache";
+}
+
+std::mutex g_compile_mutex;
+
+}  // namespace
+
+std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source) {
+    std::ostringstream os;
+    os << "// generated pipeline kernel source\n";
+    os << "#define KZ_TILE_SHIFT " << fs.tile_shift << "\n";
+    os << "#define KZ_SAMPLES " << fs.samples << "\n";
+    os << "#define KZ_NV " << fs.num_varyings << "\n";
+    os << "#define KZ_CULL_MODE " << fs.cull_mode << "\n";
+    os << "#define KZ_FRONT_FACE " << fs.front_face << "\n";
+    os << "#define KZ_DEPTH_TEST " << fs.depth_test << "\n";
+    os << "#define KZ_DEPTH_WRITE " << fs.depth_write << "\n";
+    os << "#define KZ_DEPTH_OP " << fs.depth_op << "\n";
+    os << "#define KZ_BLEND " << fs.blend << "\n";
+    os << "#define KZ_SRC_COLOR " << fs.src_color << "\n";
+    os << "#define KZ_DST_COLOR " << fs.dst_color << "\n";
+    os << "#define KZ_COLOR_OP " << fs.color_op << "\n";
+    os << "#define KZ_SRC_ALPHA " << fs.src_alpha << "\n";
+    os << "#define KZ_DST_ALPHA " << fs.dst_alpha << "\n";
+    os << "#define KZ_ALPHA_OP " << fs.alpha_op << "\n";
+    os << "#define KZ_WRITE_MASK " << fs.write_mask << "\n";
+    os << kz_src_device_common << "\n";
+    os << "__device__ void kz_sample(const KzTexture &t, float u, float v, float out[4]);\n";
+    os << shader_source << "\n";
+    os << kz_src_raster_template << "\n";
+    return os.str();
+}
+
+bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
+                           std::string &log, bool *cache_hit) {
+    const std::string src = pipeline_full_source(fs, shader_source);
+    std::string key_text = src;
+    for (int i = 0; i < kNumOptions; ++i) key_text += std::string("\n") + kCompileOptions[i];
+    int rtc_major = 0, rtc_minor = 0;
+    hiprtcVersion(&rtc_major, &rtc_minor);
+    key_text += "\nhiprtc " + std::to_string(rtc_major) + "." + std::to_string(rtc_minor);
+    char name[32];
+    std::snprintf(name, sizeof name, "%016llx", (unsigned long long)fnv1a(key_text));
+    const std::string dir = cache_dir();
+    const std::string path = dir + "/" + name + ".hsaco";
+    if (cache_hit) *cache_hit = false;
+    {
+        std::ifstream in(path, std::ios::binary);
+        if (in) {
+            code.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+            if (!code.empty()) {
+                if (cache_hit) *cache_hit = true;
+                return true;
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lock(g_compile_mutex);
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "kazan_pipeline.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const hiprtcResult rc = hiprtcCompileProgram(prog, kNumOptions, kCompileOptions);
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    if (n > 1) {
+        log.resize(n);
+        hiprtcGetProgramLog(prog, &log[0]);
+    }
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    // publish atomically: write a temporary file, then rename
+    mkdir(dir.c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream out(tmp, std::ios::binary);
+        if (out) out.write(code.data(), (std::streamsize)code.size());
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+    return true;
+}
+
+bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, std::string &err) {
+    hipError_t e = hipModuleLoadData(&out.module, code.data());
+    if (e != hipSuccess) {
+        err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
+        return false;
+    }
+    if ((e = hipModuleGetFunction(&out.bin_count, out.module, "kz_bin_count")) != hipSuccess ||
+        (e = hipModuleGetFunction(&out.raster, out.module, "kz_raster")) != hipSuccess) {
+        err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace kz

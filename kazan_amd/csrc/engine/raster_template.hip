@@ -1,0 +1,754 @@
+// raster_template.hip — per-pipeline kernels of the draw path for gfx950.
+//
+// vkCreateGraphicsPipelines compiles this file (through hiprtc) after a
+// generated prelude that defines the pipeline's fixed-function state as
+// macros and its shaders as device functions produced by the HIP shader
+// backend from the shader IR:
+//
+//   KZ_TILE_SHIFT, KZ_SAMPLES, KZ_NV (packed varying floats),
+//   KZ_CULL_MODE, KZ_FRONT_FACE, KZ_DEPTH_TEST, KZ_DEPTH_WRITE, KZ_DEPTH_OP,
+//   KZ_BLEND, KZ_SRC_COLOR, KZ_DST_COLOR, KZ_COLOR_OP, KZ_SRC_ALPHA,
+//   KZ_DST_ALPHA, KZ_ALPHA_OP, KZ_WRITE_MASK
+//   void kz_vertex(const KzFrame&, const KzDraw&, uint32_t vertex_index,
+//                  float pos[4], float vary[KZ_NV]);
+//   void kz_fragment(const KzFrame&, const float vary[KZ_NV], float out[4]);
+//
+// The rasterisation rules restated here are the Vulkan 1.1 polygon rules at
+// Kazan's advertised precision (vulkan-driver/src/api_impl.rs:1559-1568,
+// :1584, :1609): 1/16-px snapping with round-half-even, int64 edge
+// functions, top-left tie-break with y pointing down, standard 4x sample
+// positions, f32 barycentrics with a fixed operation order. The code is
+// compiled with -ffp-contract=off and IEEE division/square root.
+//
+// Kernels:
+//   kz_bin_count  one thread per triangle: vertex fetch + position part of
+//                 the vertex shader + setup; writes the triangle's tile
+//                 rectangle and bumps the per-tile counters.
+//   kz_raster     one wave64 per screen tile: walks the tile's triangle list
+//                 (sorted in primitive order by kz_bin_sort) in batches of
+//                 64, one triangle per lane; coverage is compacted with
+//                 ballot + prefix sums into an LDS fragment queue; depth test
+//                 and blending are applied in primitive order per pixel
+//                 against the tile's colour and depth held in LDS; the tile
+//                 is written back once.
+
+#define KZ_TS (1 << KZ_TILE_SHIFT)
+#define KZ_TP (KZ_TS * KZ_TS)
+#define KZ_FRAG_CAP (KZ_TP * 4)
+#define KZ_SMALL_CAND 16
+#define KZ_NO_CLAIM 0xFFFFFFFFu
+
+#if KZ_NV > 0
+#define KZ_NVA KZ_NV
+#else
+#define KZ_NVA 1
+#endif
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+
+__device__ __forceinline__ uint32_t kz_lane() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint64_t kz_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t kz_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ int32_t kz_bperm_i(int32_t v, uint32_t src_lane) {
+    return __builtin_amdgcn_ds_bpermute((int)(src_lane << 2), v);
+}
+__device__ __forceinline__ float kz_bperm_f(float v, uint32_t src_lane) {
+    return __int_as_float(kz_bperm_i(__float_as_int(v), src_lane));
+}
+__device__ __forceinline__ int32_t kz_readlane_i(int32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float kz_readlane_f(float v, uint32_t lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ uint32_t kz_wave_incl_scan(uint32_t v, uint32_t lane) {
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t o = (uint32_t)kz_bperm_i((int32_t)v, lane >= d ? lane - d : lane);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int32_t kz_wave_max(int32_t v, uint32_t lane) {
+    for (uint32_t d = 32; d > 0; d >>= 1) {
+        int32_t o = kz_bperm_i(v, lane ^ d);
+        v = v > o ? v : o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long kz_wave_sum_u64(unsigned long long v, uint32_t lane) {
+    for (uint32_t d = 32; d > 0; d >>= 1) {
+        uint32_t lo = (uint32_t)kz_bperm_i((int32_t)(uint32_t)v, lane ^ d);
+        uint32_t hi = (uint32_t)kz_bperm_i((int32_t)(uint32_t)(v >> 32), lane ^ d);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// primitive assembly
+
+__device__ __forceinline__ uint32_t kz_find_draw(const KzFrame &f, uint32_t g) {
+    uint32_t lo = 0, hi = f.ndraws - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (f.draws[mid].tri_begin <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t kz_vertex_index(const KzDraw &d, uint32_t t, uint32_t j) {
+    const uint32_t k = d.first + 3u * t + j;
+    if (d.index_type == 2u) return k;
+    const uint32_t idx = d.index_type == 1u ? ((const uint32_t *)d.ib)[k] : (uint32_t)((const uint16_t *)d.ib)[k];
+    return (uint32_t)((int32_t)idx + d.vertex_offset);
+}
+
+// ---------------------------------------------------------------------------
+// triangle setup (identical decisions in kz_bin_count and kz_raster)
+
+struct KzTri {
+    int32_t x0, y0, d1x, d1y, d2x, d2y;
+    int64_t S;
+    float invS, z0, dz1, dz2, iw0, iw1, iw2;
+    int32_t px0, py0, px1, py1;
+};
+
+// 0 = dropped (clip-rejected, degenerate or culled), 1 = set up,
+// 2 = unsupported (needs z or guard-band clipping; dropped and counted).
+__device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], KzTri &t) {
+#pragma unroll
+    for (int plane = 0; plane < 6; ++plane) {
+        int out = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bool o;
+            switch (plane) {
+            case 0: o = p[k][0] < -p[k][3]; break;
+            case 1: o = p[k][0] > p[k][3]; break;
+            case 2: o = p[k][1] < -p[k][3]; break;
+            case 3: o = p[k][1] > p[k][3]; break;
+            case 4: o = p[k][2] < 0.0f; break;
+            default: o = p[k][2] > p[k][3]; break;
+            }
+            out += o ? 1 : 0;
+        }
+        if (out == 3) return 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (!(p[k][3] > 0.0f) || !(p[k][2] >= 0.0f) || !(p[k][2] <= p[k][3])) return 2;
+    int32_t sx[3], sy[3];
+    float zf[3], iw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float xd = p[k][0] / p[k][3], yd = p[k][1] / p[k][3], zd = p[k][2] / p[k][3];
+        const float xf = xd * f.vp_hw + f.vp_ox;
+        const float yf = yd * f.vp_hh + f.vp_oy;
+        zf[k] = zd * f.vp_dz + f.vp_zmin;
+        if (!(xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f)) return 2;
+        sx[k] = (int32_t)__builtin_rintf(xf * 16.0f);
+        sy[k] = (int32_t)__builtin_rintf(yf * 16.0f);
+        iw[k] = 1.0f / p[k][3];
+    }
+    t.x0 = sx[0];
+    t.y0 = sy[0];
+    t.d1x = sx[1] - sx[0];
+    t.d1y = sy[1] - sy[0];
+    t.d2x = sx[2] - sx[0];
+    t.d2y = sy[2] - sy[0];
+    const int64_t S = (int64_t)t.d1x * t.d2y - (int64_t)t.d2x * t.d1y;
+    if (S == 0) return 0;
+    const bool front = (KZ_FRONT_FACE == 0) ? (S < 0) : (S > 0);
+    if ((KZ_CULL_MODE & 1) && front) return 0;
+    if ((KZ_CULL_MODE & 2) && !front) return 0;
+    t.S = S;
+    t.invS = 1.0f / (float)S;
+    t.z0 = zf[0];
+    t.dz1 = zf[1] - zf[0];
+    t.dz2 = zf[2] - zf[0];
+    t.iw0 = iw[0];
+    t.iw1 = iw[1];
+    t.iw2 = iw[2];
+    const int32_t xmin = min(sx[0], min(sx[1], sx[2])), xmax = max(sx[0], max(sx[1], sx[2]));
+    const int32_t ymin = min(sy[0], min(sy[1], sy[2])), ymax = max(sy[0], max(sy[1], sy[2]));
+    t.px0 = max(xmin >> 4, f.sc_x0);
+    t.px1 = min(xmax >> 4, f.sc_x1);
+    t.py0 = max(ymin >> 4, f.sc_y0);
+    t.py1 = min(ymax >> 4, f.sc_y1);
+    return 1;
+}
+
+// Coverage constants: the interior-positive orientation and the top-left
+// thresholds (0 on a top-left edge, 1 otherwise) of E01, E12, E20.
+struct KzCov {
+    int32_t flip, th01, th12, th20;
+};
+
+__device__ __forceinline__ int32_t kz_tl_threshold(int64_t A, int64_t B) { return (A > 0 || (A == 0 && B > 0)) ? 0 : 1; }
+
+__device__ __forceinline__ KzCov kz_cov_setup(const KzTri &t) {
+    KzCov c;
+    const int64_t sg = t.S < 0 ? -1 : 1;
+    c.flip = t.S < 0 ? 1 : 0;
+    c.th01 = kz_tl_threshold(-(int64_t)t.d1y * sg, (int64_t)t.d1x * sg);
+    c.th20 = kz_tl_threshold((int64_t)t.d2y * sg, -(int64_t)t.d2x * sg);
+    c.th12 = kz_tl_threshold(-((int64_t)t.d2y - t.d1y) * sg, ((int64_t)t.d2x - t.d1x) * sg);
+    return c;
+}
+
+// Signed (unflipped) edge values at absolute sample position (sx, sy) in
+// 1/16 px: E01, E20 and E12 = S - E01 - E20 (the three sum to S).
+__device__ __forceinline__ void kz_edges(int32_t x0, int32_t y0, int32_t d1x, int32_t d1y, int32_t d2x, int32_t d2y,
+                                         int64_t S, int32_t sx, int32_t sy, int64_t &e01, int64_t &e20, int64_t &e12) {
+    const int32_t rx = sx - x0, ry = sy - y0;
+    e01 = (int64_t)d1x * ry - (int64_t)d1y * rx;
+    e20 = (int64_t)d2y * rx - (int64_t)d2x * ry;
+    e12 = S - e01 - e20;
+}
+
+__device__ __forceinline__ bool kz_inside(int64_t e01, int64_t e20, int64_t e12, int32_t flip, int32_t th01,
+                                          int32_t th12, int32_t th20) {
+    if (flip) {
+        e01 = -e01;
+        e20 = -e20;
+        e12 = -e12;
+    }
+    return e01 >= th01 && e12 >= th12 && e20 >= th20;
+}
+
+__constant__ const int32_t kz_sample_x[4] = {KZ_SAMPLES == 4 ? 6 : 8, 14, 2, 10};
+__constant__ const int32_t kz_sample_y[4] = {KZ_SAMPLES == 4 ? 2 : 8, 6, 10, 14};
+
+__device__ __forceinline__ uint32_t kz_cover_mask(const KzTri &t, const KzCov &c, int32_t px, int32_t py) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < KZ_SAMPLES; ++s) {
+        int64_t e01, e20, e12;
+        kz_edges(t.x0, t.y0, t.d1x, t.d1y, t.d2x, t.d2y, t.S, px * 16 + kz_sample_x[s], py * 16 + kz_sample_y[s], e01,
+                 e20, e12);
+        if (kz_inside(e01, e20, e12, c.flip, c.th01, c.th12, c.th20)) m |= 1u << s;
+    }
+    return m;
+}
+
+__device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, float p[3][4],
+                                                  float vary[3][KZ_NVA]) {
+    const KzDraw &d = f.draws[kz_find_draw(f, g)];
+    const uint32_t t = g - d.tri_begin;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) kz_vertex(f, d, kz_vertex_index(d, t, j), p[j], vary[j]);
+}
+
+// ---------------------------------------------------------------------------
+// binning, pass 1
+
+extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = kz_lane();
+    uint32_t n_setup = 0, n_unsup = 0;
+    if (g < f.total_tris) {
+        float p[3][4], vary[3][KZ_NVA];
+        kz_fetch_triangle(f, g, p, vary);
+        KzTri t;
+        const int st = kz_setup(f, p, t);
+        uint64_t rect = KZ_RECT_EMPTY;
+        if (st == 1) {
+            n_setup = 1;
+            if (t.px0 <= t.px1 && t.py0 <= t.py1) {
+                const uint32_t tx0 = (uint32_t)t.px0 >> KZ_TILE_SHIFT, tx1 = (uint32_t)t.px1 >> KZ_TILE_SHIFT;
+                const uint32_t ty0 = (uint32_t)t.py0 >> KZ_TILE_SHIFT, ty1 = (uint32_t)t.py1 >> KZ_TILE_SHIFT;
+                rect = (uint64_t)tx0 | ((uint64_t)ty0 << 16) | ((uint64_t)tx1 << 32) | ((uint64_t)ty1 << 48);
+                for (uint32_t ty = ty0; ty <= ty1; ++ty)
+                    for (uint32_t tx = tx0; tx <= tx1; ++tx) atomicAdd(&f.tile_counts[ty * f.tiles_x + tx], 1u);
+            }
+        } else if (st == 2) {
+            n_unsup = 1;
+        }
+        f.tri_rect[g] = rect;
+    }
+    const uint64_t ms = kz_ballot(n_setup != 0), mu = kz_ballot(n_unsup != 0), mi = kz_ballot(g < f.total_tris);
+    if (lane == 0) {
+        if (mi) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
+        if (ms) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
+        if (mu) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_UNSUPPORTED], (unsigned long long)__popcll(mu));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fixed-function fragment operations
+
+__device__ __forceinline__ uint32_t kz_wrap(int32_t i, uint32_t n, uint32_t mode) {
+    if (mode == 0) {  // REPEAT
+        const int32_t r = i % (int32_t)n;
+        return (uint32_t)(r < 0 ? r + (int32_t)n : r);
+    }
+    return (uint32_t)(i < 0 ? 0 : (i >= (int32_t)n ? (int32_t)n - 1 : i));  // CLAMP_TO_EDGE
+}
+
+__device__ __forceinline__ float kz_clamp_coord(float v) {
+    const float lim = 1.0e8f;
+    v = fminf(fmaxf(v, -lim), lim);
+    return v != v ? 0.0f : v;
+}
+
+// Combined image sampler lookup on a linear RGBA8 image. LINEAR uses 4-bit
+// weights (subTexelPrecisionBits = 4, vulkan-driver/src/api_impl.rs:1560):
+// f = u*(W*16) - 8, i = floor(f), texel = i >> 4, weight = i & 15, integer
+// lerp over 16ths, result = sum * (1/65280). NEAREST takes floor(u*W).
+__device__ __forceinline__ void kz_sample(const KzTexture &t, float u, float v, float out[4]) {
+    if (t.filter == 0) {
+        const int32_t x = (int32_t)floorf(kz_clamp_coord(u * (float)t.w));
+        const int32_t y = (int32_t)floorf(kz_clamp_coord(v * (float)t.h));
+        const uint32_t texel =
+            *(const uint32_t *)(t.data + ((size_t)kz_wrap(y, t.h, t.addr_v) * t.w + kz_wrap(x, t.w, t.addr_u)) * 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = (float)((texel >> (8 * c)) & 255u) * (1.0f / 255.0f);
+        return;
+    }
+    const float fu = kz_clamp_coord(u * (float)(t.w * 16u) - 8.0f);
+    const float fv = kz_clamp_coord(v * (float)(t.h * 16u) - 8.0f);
+    const int32_t iu = (int32_t)floorf(fu), iv = (int32_t)floorf(fv);
+    const int32_t x0 = iu >> 4, y0 = iv >> 4;
+    const uint32_t ax = (uint32_t)(iu & 15), ay = (uint32_t)(iv & 15);
+    const uint32_t xa = kz_wrap(x0, t.w, t.addr_u), xb = kz_wrap(x0 + 1, t.w, t.addr_u);
+    const uint32_t ya = kz_wrap(y0, t.h, t.addr_v), yb = kz_wrap(y0 + 1, t.h, t.addr_v);
+    const uint32_t *row_a = (const uint32_t *)(t.data + (size_t)ya * t.w * 4);
+    const uint32_t *row_b = (const uint32_t *)(t.data + (size_t)yb * t.w * 4);
+    const uint32_t t00 = row_a[xa], t10 = row_a[xb], t01 = row_b[xa], t11 = row_b[xb];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t sh = 8 * c;
+        const uint32_t top = ((t00 >> sh) & 255u) * (16u - ax) + ((t10 >> sh) & 255u) * ax;
+        const uint32_t bot = ((t01 >> sh) & 255u) * (16u - ax) + ((t11 >> sh) & 255u) * ax;
+        out[c] = (float)(top * (16u - ay) + bot * ay) * (1.0f / 65280.0f);
+    }
+}
+
+__device__ __forceinline__ float kz_clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+__device__ __forceinline__ bool kz_depth_cmp(float z, float d) {
+#if KZ_DEPTH_OP == 0
+    return false;
+#elif KZ_DEPTH_OP == 1
+    return z < d;
+#elif KZ_DEPTH_OP == 2
+    return z == d;
+#elif KZ_DEPTH_OP == 3
+    return z <= d;
+#elif KZ_DEPTH_OP == 4
+    return z > d;
+#elif KZ_DEPTH_OP == 5
+    return z != d;
+#elif KZ_DEPTH_OP == 6
+    return z >= d;
+#else
+    return true;
+#endif
+}
+
+__device__ __forceinline__ float kz_blend_factor(int fct, const float s[4], const float d[4], int c) {
+    switch (fct) {
+    case 0: return 0.0f;
+    case 1: return 1.0f;
+    case 2: return s[c];
+    case 3: return 1.0f - s[c];
+    case 4: return d[c];
+    case 5: return 1.0f - d[c];
+    case 6: return s[3];
+    case 7: return 1.0f - s[3];
+    case 8: return d[3];
+    case 9: return 1.0f - d[3];
+    case 14: return c == 3 ? 1.0f : fminf(s[3], 1.0f - d[3]);
+    default: return 0.0f;
+    }
+}
+
+__device__ __forceinline__ float kz_blend_op(int op, float s, float sf, float d, float df) {
+    switch (op) {
+    case 0: return s * sf + d * df;
+    case 1: return s * sf - d * df;
+    case 2: return d * df - s * sf;
+    case 3: return fminf(s, d);
+    default: return fmaxf(s, d);
+    }
+}
+
+__device__ __forceinline__ uint32_t kz_encode8(float v) { return (uint32_t)(kz_clamp01(v) * 255.0f + 0.5f); }
+
+__device__ __forceinline__ uint32_t kz_rop(uint32_t dst, const float src[4]) {
+#if KZ_BLEND
+    float d[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) d[c] = (float)((dst >> (8 * c)) & 255u) * (1.0f / 255.0f);
+    float r[4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        r[c] = kz_blend_op(KZ_COLOR_OP, src[c], kz_blend_factor(KZ_SRC_COLOR, src, d, c), d[c],
+                           kz_blend_factor(KZ_DST_COLOR, src, d, c));
+    r[3] = kz_blend_op(KZ_ALPHA_OP, src[3], kz_blend_factor(KZ_SRC_ALPHA, src, d, 3), d[3],
+                       kz_blend_factor(KZ_DST_ALPHA, src, d, 3));
+#else
+    const float *r = src;
+#endif
+    uint32_t out = dst;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if ((KZ_WRITE_MASK >> c) & 1) out = (out & ~(255u << (8 * c))) | (kz_encode8(r[c]) << (8 * c));
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// the tile rasteriser
+
+struct KzLaneTri {
+    KzTri t;
+    KzCov c;
+    float a0[KZ_NVA], da1[KZ_NVA], da2[KZ_NVA];
+    int32_t cx0, cy0, cx1, cy1;  // bbox ∩ tile
+    bool live;
+};
+
+__device__ __forceinline__ bool kz_rect_has_tile(uint64_t r, uint32_t tx, uint32_t ty) {
+    if (r == KZ_RECT_EMPTY) return false;
+    const uint32_t tx0 = (uint32_t)(r & 0xFFFF), ty0 = (uint32_t)((r >> 16) & 0xFFFF);
+    const uint32_t tx1 = (uint32_t)((r >> 32) & 0xFFFF), ty1 = (uint32_t)(r >> 48);
+    return tx >= tx0 && tx <= tx1 && ty >= ty0 && ty <= ty1;
+}
+
+__device__ __forceinline__ uint32_t kz_pack_frag(uint32_t pix, uint32_t seq, uint32_t mask) {
+    return (pix << 10) | (seq << 4) | mask;
+}
+
+__device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_color, float *s_depth,
+                               uint32_t *s_claim, uint32_t *s_frag) {
+    const uint32_t lane = kz_lane();
+    const uint32_t n = f.tile_counts[tile];
+    if (n == 0) return;
+    const uint32_t tx = tile % f.tiles_x, ty = tile / f.tiles_x;
+    const int32_t X0 = (int32_t)(tx << KZ_TILE_SHIFT), Y0 = (int32_t)(ty << KZ_TILE_SHIFT);
+    const uint32_t *gcolor = (const uint32_t *)f.color;
+
+    for (uint32_t i = lane; i < KZ_TP; i += 64) {
+        const int32_t x = X0 + (int32_t)(i & (KZ_TS - 1)), y = Y0 + (int32_t)(i >> KZ_TILE_SHIFT);
+        if (x < (int32_t)f.width && y < (int32_t)f.height) {
+            const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
+#pragma unroll
+            for (int s = 0; s < KZ_SAMPLES; ++s) {
+                s_color[i * KZ_SAMPLES + s] = gcolor[gp + s];
+                s_depth[i * KZ_SAMPLES + s] = f.depth[gp + s];
+            }
+        }
+        s_claim[i] = KZ_NO_CLAIM;
+    }
+    __syncthreads();
+
+    const uint32_t base = f.tile_offsets[tile];
+    // Entries past list_capacity were dropped by kz_bin_scatter; such a tile
+    // takes its triangles from an in-order scan of all tile rectangles.
+    const bool overflow = (uint64_t)base + n > (uint64_t)f.list_capacity;
+    unsigned long long nfrag = 0, npass = 0;
+    uint32_t k = 0;
+
+    while (true) {
+        uint32_t g = 0;
+        bool valid;
+        if (!overflow) {
+            if (k >= n) break;
+            valid = k + lane < n;
+            if (valid) g = f.tile_lists[base + k + lane];
+        } else {
+            if (k >= f.total_tris) break;
+            g = k + lane;
+            valid = g < f.total_tris && kz_rect_has_tile(f.tri_rect[g], tx, ty);
+            if (kz_ballot(valid) == 0) {
+                k += 64;
+                continue;
+            }
+        }
+
+        // --- per-lane triangle: vertex shader + setup -------------------
+        KzLaneTri L;
+        L.live = false;
+        L.cx0 = L.cy0 = 0;
+        L.cx1 = L.cy1 = -1;
+        L.c.flip = 0;
+        L.c.th01 = L.c.th12 = L.c.th20 = 0;
+        L.t.x0 = L.t.y0 = L.t.d1x = L.t.d1y = L.t.d2x = L.t.d2y = 0;
+        L.t.S = 0;
+        L.t.invS = L.t.z0 = L.t.dz1 = L.t.dz2 = 0.0f;
+        L.t.iw0 = L.t.iw1 = L.t.iw2 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < KZ_NVA; ++i) L.a0[i] = L.da1[i] = L.da2[i] = 0.0f;
+        if (valid) {
+            float p[3][4], vary[3][KZ_NVA];
+            kz_fetch_triangle(f, g, p, vary);
+            if (kz_setup(f, p, L.t) == 1) {
+                L.cx0 = max(L.t.px0, X0);
+                L.cx1 = min(L.t.px1, X0 + KZ_TS - 1);
+                L.cy0 = max(L.t.py0, Y0);
+                L.cy1 = min(L.t.py1, Y0 + KZ_TS - 1);
+                L.live = L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
+                L.c = kz_cov_setup(L.t);
+#pragma unroll
+                for (int i = 0; i < KZ_NV; ++i) {
+                    L.a0[i] = vary[0][i];
+                    L.da1[i] = vary[1][i] - vary[0][i];
+                    L.da2[i] = vary[2][i] - vary[0][i];
+                }
+            }
+        }
+        const int32_t cw = L.cx1 - L.cx0 + 1;
+        const int32_t cand = L.live ? cw * (L.cy1 - L.cy0 + 1) : 0;
+        const bool small = cand <= KZ_SMALL_CAND;
+
+        // --- phase 1: covered pixels per triangle ------------------------
+        uint32_t cnt = 0;
+        {
+            const int32_t maxc = kz_wave_max(small ? cand : 0, lane);
+            int32_t cx = L.cx0, cy = L.cy0;
+            for (int32_t j = 0; j < maxc; ++j) {
+                if (small && j < cand) {
+                    if (kz_cover_mask(L.t, L.c, cx, cy)) ++cnt;
+                    if (++cx > L.cx1) {
+                        cx = L.cx0;
+                        ++cy;
+                    }
+                }
+            }
+        }
+        uint64_t bigm = kz_ballot(L.live && !small);
+        while (bigm) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(bigm);
+            bigm &= bigm - 1;
+            KzTri tb;
+            tb.x0 = kz_readlane_i(L.t.x0, b);
+            tb.y0 = kz_readlane_i(L.t.y0, b);
+            tb.d1x = kz_readlane_i(L.t.d1x, b);
+            tb.d1y = kz_readlane_i(L.t.d1y, b);
+            tb.d2x = kz_readlane_i(L.t.d2x, b);
+            tb.d2y = kz_readlane_i(L.t.d2y, b);
+            tb.S = (int64_t)(((uint64_t)(uint32_t)kz_readlane_i((int32_t)(uint32_t)(uint64_t)L.t.S, b)) |
+                             ((uint64_t)(uint32_t)kz_readlane_i((int32_t)((uint64_t)L.t.S >> 32), b) << 32));
+            KzCov cb;
+            cb.flip = kz_readlane_i(L.c.flip, b);
+            cb.th01 = kz_readlane_i(L.c.th01, b);
+            cb.th12 = kz_readlane_i(L.c.th12, b);
+            cb.th20 = kz_readlane_i(L.c.th20, b);
+            const int32_t bx0 = kz_readlane_i(L.cx0, b), by0 = kz_readlane_i(L.cy0, b);
+            const int32_t bw = kz_readlane_i(cw, b), bc = kz_readlane_i(cand, b);
+            uint32_t c = 0;
+            for (int32_t i0 = 0; i0 < bc; i0 += 64) {
+                const int32_t i = i0 + (int32_t)lane;
+                bool cov = false;
+                if (i < bc) cov = kz_cover_mask(tb, cb, bx0 + i % bw, by0 + i / bw) != 0;
+                c += (uint32_t)__popcll(kz_ballot(cov));
+            }
+            if (lane == b) cnt = c;
+        }
+
+        // --- batch cut: keep the longest lane prefix whose fragments fit ---
+        const uint32_t incl = kz_wave_incl_scan(cnt, lane);
+        const uint32_t take = (uint32_t)__popcll(kz_ballot(incl <= KZ_FRAG_CAP));
+        const bool taken = lane < take;
+        const uint32_t total = (uint32_t)kz_readlane_i((int32_t)incl, take - 1);
+        const uint32_t excl = incl - cnt;
+
+        // --- phase 2: write the fragment queue in primitive order ---------
+        if (taken && small && cand > 0) {
+            uint32_t w = excl;
+            int32_t cx = L.cx0, cy = L.cy0;
+            for (int32_t j = 0; j < cand; ++j) {
+                const uint32_t m = kz_cover_mask(L.t, L.c, cx, cy);
+                if (m) s_frag[w++] = kz_pack_frag((uint32_t)((cy - Y0) * KZ_TS + (cx - X0)), lane, m);
+                if (++cx > L.cx1) {
+                    cx = L.cx0;
+                    ++cy;
+                }
+            }
+        }
+        bigm = kz_ballot(taken && L.live && !small);
+        while (bigm) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(bigm);
+            bigm &= bigm - 1;
+            KzTri tb;
+            tb.x0 = kz_readlane_i(L.t.x0, b);
+            tb.y0 = kz_readlane_i(L.t.y0, b);
+            tb.d1x = kz_readlane_i(L.t.d1x, b);
+            tb.d1y = kz_readlane_i(L.t.d1y, b);
+            tb.d2x = kz_readlane_i(L.t.d2x, b);
+            tb.d2y = kz_readlane_i(L.t.d2y, b);
+            tb.S = (int64_t)(((uint64_t)(uint32_t)kz_readlane_i((int32_t)(uint32_t)(uint64_t)L.t.S, b)) |
+                             ((uint64_t)(uint32_t)kz_readlane_i((int32_t)((uint64_t)L.t.S >> 32), b) << 32));
+            KzCov cb;
+            cb.flip = kz_readlane_i(L.c.flip, b);
+            cb.th01 = kz_readlane_i(L.c.th01, b);
+            cb.th12 = kz_readlane_i(L.c.th12, b);
+            cb.th20 = kz_readlane_i(L.c.th20, b);
+            const int32_t bx0 = kz_readlane_i(L.cx0, b), by0 = kz_readlane_i(L.cy0, b);
+            const int32_t bw = kz_readlane_i(cw, b), bc = kz_readlane_i(cand, b);
+            uint32_t w = (uint32_t)kz_readlane_i((int32_t)excl, b);
+            for (int32_t i0 = 0; i0 < bc; i0 += 64) {
+                const int32_t i = i0 + (int32_t)lane;
+                uint32_t m = 0;
+                int32_t px = 0, py = 0;
+                if (i < bc) {
+                    px = bx0 + i % bw;
+                    py = by0 + i / bw;
+                    m = kz_cover_mask(tb, cb, px, py);
+                }
+                const uint64_t bal = kz_ballot(m != 0);
+                if (m) s_frag[w + kz_rank(bal)] = kz_pack_frag((uint32_t)((py - Y0) * KZ_TS + (px - X0)), b, m);
+                w += (uint32_t)__popcll(bal);
+            }
+        }
+        __syncthreads();
+
+        // --- resolve the queue 64 fragments at a time --------------------
+        for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+            const uint32_t qi = c0 + lane;
+            const bool active = qi < total;
+            const uint32_t e = active ? s_frag[qi] : 0u;
+            const uint32_t pix = e >> 10, seq = (e >> 4) & 63u, mask = e & 15u;
+            // the owning triangle's data, from lane `seq`
+            const int32_t x0 = kz_bperm_i(L.t.x0, seq), y0 = kz_bperm_i(L.t.y0, seq);
+            const int32_t d1x = kz_bperm_i(L.t.d1x, seq), d1y = kz_bperm_i(L.t.d1y, seq);
+            const int32_t d2x = kz_bperm_i(L.t.d2x, seq), d2y = kz_bperm_i(L.t.d2y, seq);
+            const int64_t S =
+                (int64_t)(((uint64_t)(uint32_t)kz_bperm_i((int32_t)(uint32_t)(uint64_t)L.t.S, seq)) |
+                          ((uint64_t)(uint32_t)kz_bperm_i((int32_t)((uint64_t)L.t.S >> 32), seq) << 32));
+            const float invS = kz_bperm_f(L.t.invS, seq);
+            const float z0 = kz_bperm_f(L.t.z0, seq), dz1 = kz_bperm_f(L.t.dz1, seq), dz2 = kz_bperm_f(L.t.dz2, seq);
+            const int32_t px = X0 + (int32_t)(pix & (KZ_TS - 1)), py = Y0 + (int32_t)(pix >> KZ_TILE_SHIFT);
+
+            float zs[KZ_SAMPLES];
+            uint32_t nf = 0;
+#pragma unroll
+            for (int s = 0; s < KZ_SAMPLES; ++s) {
+                zs[s] = 0.0f;
+                if ((mask >> s) & 1u) {
+                    ++nf;
+                    int64_t e01, e20, e12;
+                    kz_edges(x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + kz_sample_x[s], py * 16 + kz_sample_y[s], e01,
+                             e20, e12);
+                    const float l1 = (float)e20 * invS, l2 = (float)e01 * invS;
+                    const float z = (z0 + l1 * dz1) + l2 * dz2;
+                    zs[s] = fminf(fmaxf(z, f.z_lo), f.z_hi);
+                }
+            }
+            nfrag += nf;
+
+            // depth test + write, in primitive order per pixel
+            uint32_t pass = 0;
+#if KZ_DEPTH_TEST
+            bool act = active && mask != 0;
+            while (kz_ballot(act)) {
+                if (act) atomicMin(&s_claim[pix], seq);
+                __syncthreads();
+                const bool win = act && s_claim[pix] == seq;
+                if (win) {
+#pragma unroll
+                    for (int s = 0; s < KZ_SAMPLES; ++s) {
+                        if ((mask >> s) & 1u) {
+                            float &dd = s_depth[pix * KZ_SAMPLES + s];
+                            if (kz_depth_cmp(zs[s], dd)) {
+                                pass |= 1u << s;
+#if KZ_DEPTH_WRITE
+                                dd = zs[s];
+#endif
+                            }
+                        }
+                    }
+                    s_claim[pix] = KZ_NO_CLAIM;
+                    act = false;
+                }
+                __syncthreads();
+            }
+#else
+            pass = active ? mask : 0u;
+#endif
+            npass += (uint32_t)__popc(pass);
+
+            // varyings at the pixel centre and the fragment shader
+            float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            {
+                float a0[KZ_NVA], da1[KZ_NVA], da2[KZ_NVA];
+#pragma unroll
+                for (int i = 0; i < KZ_NV; ++i) {
+                    a0[i] = kz_bperm_f(L.a0[i], seq);
+                    da1[i] = kz_bperm_f(L.da1[i], seq);
+                    da2[i] = kz_bperm_f(L.da2[i], seq);
+                }
+                const float iw0 = kz_bperm_f(L.t.iw0, seq), iw1 = kz_bperm_f(L.t.iw1, seq),
+                            iw2 = kz_bperm_f(L.t.iw2, seq);
+                if (pass) {
+                    int64_t e01, e20, e12;
+                    kz_edges(x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + 8, py * 16 + 8, e01, e20, e12);
+                    const float l0 = (float)e12 * invS, l1 = (float)e20 * invS, l2 = (float)e01 * invS;
+                    const float q = ((l0 * iw0) + (l1 * iw1)) + (l2 * iw2);
+                    const float rq = 1.0f / q;
+                    const float b1 = (l1 * iw1) * rq, b2 = (l2 * iw2) * rq;
+                    float vary[KZ_NVA];
+#pragma unroll
+                    for (int i = 0; i < KZ_NV; ++i) vary[i] = (a0[i] + b1 * da1[i]) + b2 * da2[i];
+                    const float zc = fminf(fmaxf((z0 + l1 * dz1) + l2 * dz2, f.z_lo), f.z_hi);
+                    const float frag_coord[4] = {(float)px + 0.5f, (float)py + 0.5f, zc, q};
+                    kz_fragment(f, vary, frag_coord, col);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) col[c] = kz_clamp01(col[c]);
+                }
+            }
+
+            // colour: blend + write, in primitive order per pixel
+            bool act2 = pass != 0;
+            while (kz_ballot(act2)) {
+                if (act2) atomicMin(&s_claim[pix], seq);
+                __syncthreads();
+                const bool win = act2 && s_claim[pix] == seq;
+                if (win) {
+#pragma unroll
+                    for (int s = 0; s < KZ_SAMPLES; ++s)
+                        if ((pass >> s) & 1u) s_color[pix * KZ_SAMPLES + s] = kz_rop(s_color[pix * KZ_SAMPLES + s], col);
+                    s_claim[pix] = KZ_NO_CLAIM;
+                    act2 = false;
+                }
+                __syncthreads();
+            }
+        }
+        k += take;
+        __syncthreads();
+    }
+
+    for (uint32_t i = lane; i < KZ_TP; i += 64) {
+        const int32_t x = X0 + (int32_t)(i & (KZ_TS - 1)), y = Y0 + (int32_t)(i >> KZ_TILE_SHIFT);
+        if (x < (int32_t)f.width && y < (int32_t)f.height) {
+            const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
+#pragma unroll
+            for (int s = 0; s < KZ_SAMPLES; ++s) {
+                ((uint32_t *)f.color)[gp + s] = s_color[i * KZ_SAMPLES + s];
+                f.depth[gp + s] = s_depth[i * KZ_SAMPLES + s];
+            }
+        }
+    }
+    nfrag = kz_wave_sum_u64(nfrag, lane);
+    npass = kz_wave_sum_u64(npass, lane);
+    if (lane == 0) {
+        atomicAdd(&f.stats[KZ_STAT_FRAGMENTS], nfrag);
+        atomicAdd(&f.stats[KZ_STAT_DEPTH_PASSED], npass);
+    }
+}
+
+extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
+    __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
+    __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
+    __shared__ uint32_t s_claim[KZ_TP];
+    __shared__ uint32_t s_frag[KZ_FRAG_CAP];
+    kz_raster_tile(f, blockIdx.x, s_color, s_depth, s_claim, s_frag);
+}

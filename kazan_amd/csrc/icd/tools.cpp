@@ -1,0 +1,91 @@
+// tools.cpp — the shader-compiler entry points of kazan_tools.h.
+#include "kazan/kazan_tools.h"
+
+#include <cstring>
+#include <string>
+
+#include "../engine/engine.h"
+#include "../shader/backend_hip.h"
+#include "../shader/spirv_reader.h"
+#include "../shader/translate.h"
+
+namespace {
+
+int put(const std::string &s, char *out, size_t cap, size_t *out_len, int rc) {
+    if (out_len) *out_len = s.size();
+    if (out && cap) {
+        const size_t n = s.size() < cap - 1 ? s.size() : cap - 1;
+        std::memcpy(out, s.data(), n);
+        out[n] = 0;
+    }
+    return rc;
+}
+
+kz::ir::Stage stage_of(uint32_t s) {
+    return s == 0 ? kz::ir::Stage::Vertex : s == 4 ? kz::ir::Stage::Fragment : kz::ir::Stage::Compute;
+}
+
+}  // namespace
+
+extern "C" int kazan_spirv_disassemble(const uint32_t *words, size_t nwords, char *out, size_t cap, size_t *out_len) {
+    kz::SpirvModule m;
+    std::string err;
+    if (!kz::spirv_parse(words, nwords, m, err)) return put(err, out, cap, out_len, -1);
+    return put(kz::spirv_disassemble(m), out, cap, out_len, 0);
+}
+
+extern "C" int kazan_translate_to_ir_text(const uint32_t *words, size_t nwords, uint32_t stage, const char *entry,
+                                          char *out, size_t cap, size_t *out_len) {
+    kz::ir::Module m;
+    std::string err;
+    if (!kz::translate_spirv(words, nwords, entry ? entry : "main", stage_of(stage), {}, m, err))
+        return put(err, out, cap, out_len, -1);
+    return put(kz::ir::module_text(m), out, cap, out_len, 0);
+}
+
+extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t what, char *out, size_t cap,
+                                            size_t *out_len) {
+    kz::ir::Module vm, fm;
+    std::string err;
+    if (!kz::translate_spirv(d->vs_words, d->vs_nwords, "main", kz::ir::Stage::Vertex, {}, vm, err))
+        return put("vertex shader: " + err, out, cap, out_len, -1);
+    if (d->fs_words && !kz::translate_spirv(d->fs_words, d->fs_nwords, "main", kz::ir::Stage::Fragment, {}, fm, err))
+        return put("fragment shader: " + err, out, cap, out_len, -1);
+    kz::PipelineCodegenInput in;
+    in.vs = &vm;
+    in.fs = d->fs_words ? &fm : nullptr;
+    for (uint32_t i = 0; i < d->num_bindings && i < 8; ++i) in.bindings.push_back({i, d->binding_stride[i], d->binding_rate[i]});
+    for (uint32_t i = 0; i < d->num_attributes && i < 16; ++i)
+        in.attributes.push_back({d->attr_location[i], d->attr_binding[i], d->attr_format[i], d->attr_offset[i]});
+    for (uint32_t i = 0; i < d->num_ubos && i < 4; ++i) in.ubo_slot[{d->ubo_key[i] >> 16, d->ubo_key[i] & 0xFFFF}] = i;
+    for (uint32_t i = 0; i < d->num_textures && i < 4; ++i)
+        in.tex_slot[{d->texture_key[i] >> 16, d->texture_key[i] & 0xFFFF}] = i;
+    kz::PipelineCodegenOutput co;
+    if (!kz::generate_pipeline_source(in, co, err)) return put(err, out, cap, out_len, -1);
+    if (what == 0) return put(co.source, out, cap, out_len, 0);
+    kz::FixedState fx;
+    fx.tile_shift = d->tile_shift ? d->tile_shift : 4;
+    fx.samples = d->samples ? d->samples : 1;
+    fx.num_varyings = co.num_varyings;
+    fx.cull_mode = d->cull_mode;
+    fx.front_face = d->front_face;
+    fx.depth_test = d->depth_test;
+    fx.depth_write = d->depth_write;
+    fx.depth_op = d->depth_op;
+    fx.blend = d->blend;
+    fx.src_color = d->src_color;
+    fx.dst_color = d->dst_color;
+    fx.color_op = d->color_op;
+    fx.src_alpha = d->src_alpha;
+    fx.dst_alpha = d->dst_alpha;
+    fx.alpha_op = d->alpha_op;
+    fx.write_mask = d->write_mask;
+    if (what == 1) return put(kz::pipeline_full_source(fx, co.source), out, cap, out_len, 0);
+    std::vector<char> code;
+    std::string log;
+    bool hit = false;
+    if (!kz::compile_pipeline_code(fx, co.source, code, log, &hit)) return put(log, out, cap, out_len, -1);
+    return put("code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
+                   " varyings=" + std::to_string(co.num_varyings),
+               out, cap, out_len, 0);
+}

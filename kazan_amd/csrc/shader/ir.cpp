@@ -1,0 +1,264 @@
+// ir.cpp — construction helpers and the text form of the shader IR.
+//
+// The layout of the text (indentation, section names, `size: fixed 0x..;`,
+// `align: 0x..;`, `-> name: data_ptr;`, hints block, `entry_point: main;`)
+// is the one fixed by the reference's golden files
+// shader-compiler-ir/test_data/test_assert_ir_matches_file.kazan-ir and
+// shader-compiler-translate-spirv-to-ir/test_data/trivial_test.kazan-ir.
+#include "ir.h"
+
+#include <cstdio>
+#include <cstring>
+
+namespace kz {
+namespace ir {
+
+namespace {
+
+const char *scalar_text(Scalar s) {
+    switch (s) {
+    case Scalar::Void: return "void";
+    case Scalar::Bool: return "bool";
+    case Scalar::I32: return "i32";
+    case Scalar::F32: return "f32";
+    case Scalar::DataPtr: return "data_ptr";
+    case Scalar::Texture: return "texture";
+    }
+    return "?";
+}
+
+std::string hex(uint32_t v) {
+    char b[16];
+    std::snprintf(b, sizeof b, "0x%X", v);
+    return b;
+}
+
+const char *op_text(Op op) {
+    switch (op) {
+    case Op::Add: return "add";
+    case Op::FAdd: return "fadd";
+    case Op::FSub: return "fsub";
+    case Op::FMul: return "fmul";
+    case Op::FDiv: return "fdiv";
+    case Op::FNeg: return "fneg";
+    case Op::FMax: return "fmax";
+    case Op::FMin: return "fmin";
+    case Op::FClamp: return "fclamp";
+    case Op::Sqrt: return "sqrt";
+    case Op::Dot: return "dot";
+    case Op::Splat: return "splat";
+    case Op::MakeVector: return "make_vector";
+    case Op::Extract: return "extract";
+    case Op::Shuffle: return "shuffle";
+    case Op::Load: return "load";
+    case Op::Store: return "store";
+    case Op::PtrOffset: return "ptr_offset";
+    case Op::Sample: return "sample";
+    case Op::ConvSToF: return "convert_s_to_f";
+    case Op::ConvFToS: return "convert_f_to_s";
+    case Op::Select: return "select";
+    case Op::FOrdLt: return "ford_lt";
+    case Op::FOrdGt: return "ford_gt";
+    case Op::Kill: return "kill";
+    case Op::BreakBlock: return "break";
+    }
+    return "?";
+}
+
+struct Printer {
+    std::string out;
+    int indent = 0;
+    void line(const std::string &s) {
+        out.append(4 * indent, ' ');
+        out += s;
+        out += '\n';
+    }
+};
+
+std::string value_use(const Module &m, uint32_t id) {
+    const Value &v = m.value(id);
+    if (!v.is_const) return v.name;
+    std::string s;
+    if (v.type.lanes > 1) s += "<";
+    for (int i = 0; i < v.type.lanes; ++i) {
+        if (i) s += ", ";
+        if (v.type.scalar == Scalar::F32) {
+            float f;
+            std::memcpy(&f, &v.const_bits[i], 4);
+            char b[48];
+            std::snprintf(b, sizeof b, "%s_f32", hex(v.const_bits[i]).c_str());
+            s += b;
+        } else {
+            s += hex(v.const_bits[i]) + "_" + scalar_text(v.type.scalar);
+        }
+    }
+    if (v.type.lanes > 1) s += ">";
+    return s;
+}
+
+std::string variable_text(const Module &m, const Type &t, uint32_t align, uint32_t ptr) {
+    return type_text(t) + ", align: " + hex(align) + " -> " + m.value(ptr).name + ": data_ptr";
+}
+
+const char *builtin_text(BuiltIn b) {
+    switch (b) {
+    case BuiltIn::VertexPosition: return "vertex_position";
+    case BuiltIn::VertexIndex: return "vertex_index";
+    case BuiltIn::InstanceIndex: return "instance_index";
+    case BuiltIn::FragCoord: return "fragment_coord";
+    }
+    return "?";
+}
+
+void print_block(Printer &p, const Module &m, const InterfaceBlock &b, const char *name) {
+    p.line(std::string(name) + " {");
+    ++p.indent;
+    p.line("-> " + m.value(b.pointer).name + ": data_ptr;");
+    p.line("size: fixed " + hex(b.size) + ";");
+    p.line("align: " + hex(b.alignment) + ";");
+    for (const auto &mem : b.members) p.line(hex(mem.offset) + ": " + type_text(mem.type) + ": {};");
+    --p.indent;
+    p.line("}");
+}
+
+}  // namespace
+
+std::string type_text(const Type &t) {
+    if (t.lanes > 1) return "<" + std::to_string(t.lanes) + " x " + scalar_text(t.scalar) + ">";
+    return scalar_text(t.scalar);
+}
+
+uint32_t Module::new_value(const Type &t, const std::string &name) {
+    Value v;
+    v.id = (uint32_t)values.size() + 1;
+    v.type = t;
+    v.name = name.empty() ? "v" + std::to_string(v.id) : name;
+    values.push_back(v);
+    return v.id;
+}
+
+uint32_t Module::new_const_f32(float f) {
+    const uint32_t id = new_value(Type::f32(), "");
+    Value &v = value(id);
+    v.is_const = true;
+    std::memcpy(&v.const_bits[0], &f, 4);
+    return id;
+}
+
+uint32_t Module::new_const_i32(int32_t i) {
+    const uint32_t id = new_value(Type::i32(), "");
+    Value &v = value(id);
+    v.is_const = true;
+    v.const_bits[0] = (uint32_t)i;
+    return id;
+}
+
+std::string instruction_text(const Module &m, const Instruction &ins) {
+    std::string s = op_text(ins.op);
+    if (ins.op == Op::BreakBlock) {
+        s += " " + m.main.body.name + "[]";
+    } else {
+        s += " [";
+        for (size_t i = 0; i < ins.args.size(); ++i) {
+            if (i) s += ", ";
+            s += value_use(m, ins.args[i]);
+        }
+        s += "]";
+        switch (ins.op) {
+        case Op::Load:
+        case Op::Store: s += ", align: " + hex(ins.fields.empty() ? 4 : ins.fields[0]); break;
+        case Op::PtrOffset: s += ", offset: " + hex(ins.fields[0]); break;
+        case Op::Extract: s += ", index: " + std::to_string(ins.fields[0]); break;
+        case Op::Shuffle: {
+            s += ", lanes: [";
+            for (size_t i = 0; i < ins.fields.size(); ++i) s += (i ? ", " : "") + std::to_string(ins.fields[i]);
+            s += "]";
+            break;
+        }
+        default: break;
+        }
+        s += " -> [";
+        for (size_t i = 0; i < ins.results.size(); ++i) {
+            if (i) s += ", ";
+            const Value &v = m.value(ins.results[i]);
+            s += v.name + ": " + type_text(v.type);
+        }
+        s += "]";
+    }
+    if (ins.has_location)
+        s += " @ \"" + ins.loc.file + "\":" + std::to_string(ins.loc.line) + ":" + std::to_string(ins.loc.column);
+    return s;
+}
+
+std::string module_text(const Module &m) {
+    Printer p;
+    p.line("module {");
+    ++p.indent;
+    p.line("target_properties {");
+    ++p.indent;
+    p.line("data_pointer_underlying_type: i64,");
+    p.line("function_pointer_underlying_type: i64,");
+    --p.indent;
+    p.line("}");
+    p.line("built_in_inputs {");
+    ++p.indent;
+    for (const auto &v : m.built_in_inputs)
+        p.line(variable_text(m, v.type, v.alignment, v.pointer) + ": { kind: " + builtin_text(v.kind) + " };");
+    --p.indent;
+    p.line("}");
+    print_block(p, m, m.user_inputs, "user_inputs_block");
+    p.line("built_in_outputs {");
+    ++p.indent;
+    for (const auto &v : m.built_in_outputs)
+        p.line(variable_text(m, v.type, v.alignment, v.pointer) + ": { kind: " + builtin_text(v.kind) + " };");
+    --p.indent;
+    p.line("}");
+    print_block(p, m, m.user_outputs, "user_outputs_block");
+    p.line("invocation_global_variables {");
+    ++p.indent;
+    for (const auto &v : m.invocation_globals) p.line(variable_text(m, v.type, v.alignment, v.pointer) + ";");
+    --p.indent;
+    p.line("}");
+    if (!m.resources.empty()) {
+        p.line("resources {");
+        ++p.indent;
+        for (const auto &r : m.resources) {
+            const char *k = r.kind == ResourceKind::PushConstants ? "push_constants"
+                            : r.kind == ResourceKind::UniformBuffer ? "uniform_buffer"
+                                                                     : "combined_image_sampler";
+            std::string s = std::string(k) + " -> " + m.value(r.pointer).name + ": " + type_text(m.value(r.pointer).type);
+            if (r.kind != ResourceKind::PushConstants)
+                s += ": { set: " + std::to_string(r.set) + ", binding: " + std::to_string(r.binding) + " }";
+            p.line(s + ";");
+        }
+        --p.indent;
+        p.line("}");
+    }
+    p.line("fn " + m.main.name + "[] -> [] {");
+    ++p.indent;
+    p.line("hints {");
+    ++p.indent;
+    p.line("inlining_hint: none,");
+    p.line("side_effects: normal,");
+    --p.indent;
+    p.line("}");
+    p.line("{");
+    ++p.indent;
+    for (const auto &v : m.main.locals) p.line(variable_text(m, v.type, v.alignment, v.pointer) + ";");
+    --p.indent;
+    p.line("}");
+    p.line(m.main.body.name + " {");
+    ++p.indent;
+    for (const auto &ins : m.main.body.body) p.line(instruction_text(m, ins) + ";");
+    --p.indent;
+    p.line("}");
+    --p.indent;
+    p.line("}");
+    p.line("entry_point: " + m.main.name + ";");
+    --p.indent;
+    p.line("}");
+    return p.out;
+}
+
+}  // namespace ir
+}  // namespace kz

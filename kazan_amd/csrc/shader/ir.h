@@ -1,0 +1,166 @@
+// ir.h — the shader IR between SPIR-V and the HIP backend.
+//
+// It follows the shape of the reference's shader-compiler-ir
+// (shader-compiler-ir/src/module.rs:128-150): a module holds target
+// properties, the built-in and user input/output interface blocks
+// (interface.rs:134-141; user blocks are structs in memory whose members sit
+// at location*16 + component*4, io_layout.rs:6-13), the invocation-global
+// variables, functions made of structured blocks, and the entry point.
+// Values are SSA, typed, and printed in the reference's text form
+// (`add [arg0, arg1] -> [result: i32]`, instructions_impl.rs:608-622).
+//
+// The reference defines two data instructions (add, load;
+// instructions_impl.rs:395-432). This IR adds the ones the draw path's
+// shaders need — float arithmetic, store, pointer offsets, vector
+// construction/extraction, dot, the GLSL.std.450 subset, texture sampling —
+// and a `resources` section for push constants, uniform buffers and
+// combined image samplers, which the reference's IR does not model yet.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace kz {
+namespace ir {
+
+enum class Scalar : uint8_t { Void, Bool, I32, F32, DataPtr, Texture };
+
+struct Type {
+    Scalar scalar = Scalar::Void;
+    uint8_t lanes = 1;  // > 1: vector
+    bool operator==(const Type &o) const { return scalar == o.scalar && lanes == o.lanes; }
+    bool operator!=(const Type &o) const { return !(*this == o); }
+    static Type f32(int n = 1) { return Type{Scalar::F32, (uint8_t)n}; }
+    static Type i32(int n = 1) { return Type{Scalar::I32, (uint8_t)n}; }
+    static Type boolean(int n = 1) { return Type{Scalar::Bool, (uint8_t)n}; }
+    static Type ptr() { return Type{Scalar::DataPtr, 1}; }
+    static Type texture() { return Type{Scalar::Texture, 1}; }
+    uint32_t size_bytes() const { return scalar == Scalar::DataPtr ? 8u : 4u * lanes; }
+};
+
+std::string type_text(const Type &t);
+
+struct Value {
+    uint32_t id = 0;
+    Type type;
+    std::string name;
+    bool is_const = false;
+    uint32_t const_bits[4] = {0, 0, 0, 0};
+};
+
+enum class Op : uint8_t {
+    Add,      // integer add
+    FAdd,
+    FSub,
+    FMul,
+    FDiv,
+    FNeg,
+    FMax,
+    FMin,
+    FClamp,   // [x, lo, hi]
+    Sqrt,
+    Dot,      // ((a0*b0 + a1*b1) + a2*b2) + a3*b3
+    Splat,    // scalar -> vector
+    MakeVector,
+    Extract,  // field: index
+    Shuffle,  // fields: lane indices into concat(a, b)
+    Load,     // field: alignment
+    Store,    // field: alignment
+    PtrOffset,  // field: byte offset
+    Sample,   // [texture, coord] -> <4 x f32>
+    ConvSToF,
+    ConvFToS,
+    Select,   // [cond, a, b]
+    FOrdLt,
+    FOrdGt,
+    Kill,     // discard the fragment
+    BreakBlock,
+};
+
+struct Location {
+    std::string file;
+    uint32_t line = 0, column = 0;
+};
+
+struct Instruction {
+    Op op;
+    std::vector<uint32_t> args;     // value ids
+    std::vector<uint32_t> results;  // value ids
+    std::vector<uint32_t> fields;   // immediates
+    bool has_location = false;
+    Location loc;
+};
+
+enum class BuiltIn : uint8_t { VertexPosition, VertexIndex, InstanceIndex, FragCoord };
+
+struct InterfaceVariable {
+    Type type;
+    uint32_t alignment = 4;
+    uint32_t pointer = 0;  // value id of its data_ptr
+    BuiltIn kind = BuiltIn::VertexPosition;
+};
+
+struct BlockMember {
+    uint32_t offset = 0;
+    Type type;
+};
+
+struct InterfaceBlock {
+    uint32_t pointer = 0;  // value id
+    uint32_t size = 0;
+    uint32_t alignment = 16;
+    std::vector<BlockMember> members;
+};
+
+enum class ResourceKind : uint8_t { PushConstants, UniformBuffer, CombinedImageSampler };
+
+struct Resource {
+    ResourceKind kind;
+    uint32_t set = 0, binding = 0;
+    uint32_t pointer = 0;  // value id (data_ptr or texture)
+};
+
+struct Variable {
+    Type type;
+    uint32_t alignment = 4;
+    uint32_t pointer = 0;
+    uint32_t size = 0;  // bytes of storage
+};
+
+struct Block {
+    std::string name;
+    std::vector<Instruction> body;
+};
+
+struct Function {
+    std::string name;
+    std::vector<Variable> locals;
+    Block body;
+};
+
+enum class Stage : uint8_t { Vertex, Fragment, Compute };
+
+struct Module {
+    Stage stage = Stage::Vertex;
+    std::vector<InterfaceVariable> built_in_inputs, built_in_outputs;
+    InterfaceBlock user_inputs, user_outputs;
+    std::vector<Variable> invocation_globals;
+    std::vector<Resource> resources;
+    std::vector<Value> values;  // index = id - 1
+    Function main;
+
+    Value &value(uint32_t id) { return values[id - 1]; }
+    const Value &value(uint32_t id) const { return values[id - 1]; }
+    uint32_t new_value(const Type &t, const std::string &name);
+    uint32_t new_const_f32(float v);
+    uint32_t new_const_i32(int32_t v);
+};
+
+// The module in the reference's textual form.
+std::string module_text(const Module &m);
+std::string instruction_text(const Module &m, const Instruction &ins);
+
+}  // namespace ir
+}  // namespace kz

@@ -1,0 +1,930 @@
+// translate.cpp — SPIR-V -> shader IR for the draw path's shader subset.
+#include "translate.h"
+
+#include <cstring>
+#include <map>
+#include <set>
+
+#include "spirv_reader.h"
+
+namespace kz {
+
+namespace {
+
+using ir::Instruction;
+using ir::Op;
+using ir::Type;
+
+enum SOp : uint16_t {
+    OpNop = 0, OpSource = 3, OpSourceExtension = 4, OpName = 5, OpMemberName = 6, OpString = 7, OpLine = 8,
+    OpExtension = 10, OpExtInstImport = 11, OpExtInst = 12, OpMemoryModel = 14, OpEntryPoint = 15,
+    OpExecutionMode = 16, OpCapability = 17, OpTypeVoid = 19, OpTypeBool = 20, OpTypeInt = 21, OpTypeFloat = 22,
+    OpTypeVector = 23, OpTypeMatrix = 24, OpTypeImage = 25, OpTypeSampler = 26, OpTypeSampledImage = 27,
+    OpTypeArray = 28, OpTypeStruct = 30, OpTypePointer = 32, OpTypeFunction = 33, OpConstantTrue = 41,
+    OpConstantFalse = 42, OpConstant = 43, OpConstantComposite = 44, OpSpecConstantTrue = 48,
+    OpSpecConstantFalse = 49, OpSpecConstant = 50, OpFunction = 54, OpFunctionEnd = 56, OpVariable = 59,
+    OpLoad = 61, OpStore = 62, OpAccessChain = 65, OpInBoundsAccessChain = 66, OpDecorate = 71,
+    OpMemberDecorate = 72, OpVectorShuffle = 79, OpCompositeConstruct = 80, OpCompositeExtract = 81,
+    OpImageSampleImplicitLod = 87, OpImageSampleExplicitLod = 88, OpConvertFToS = 110, OpConvertSToF = 111,
+    OpFNegate = 127, OpIAdd = 128, OpFAdd = 129, OpFSub = 131, OpFMul = 133, OpFDiv = 136,
+    OpVectorTimesScalar = 142, OpMatrixTimesVector = 145, OpDot = 148, OpSelect = 169, OpFOrdLessThan = 184,
+    OpFOrdGreaterThan = 186, OpNoLine = 317, OpModuleProcessed = 330, OpSelectionMerge = 247, OpLoopMerge = 246,
+    OpLabel = 248, OpBranch = 249, OpBranchConditional = 250, OpKill = 252, OpReturn = 253,
+};
+
+enum Storage : uint32_t {
+    SC_UniformConstant = 0, SC_Input = 1, SC_Uniform = 2, SC_Output = 3, SC_Private = 6, SC_Function = 7,
+    SC_PushConstant = 9,
+};
+
+enum Deco : uint32_t {
+    D_SpecId = 1, D_Block = 2, D_ColMajor = 5, D_ArrayStride = 6, D_MatrixStride = 7, D_BuiltIn = 11, D_Flat = 14,
+    D_Location = 30, D_Component = 31, D_Binding = 33, D_DescriptorSet = 34, D_Offset = 35,
+};
+
+enum BuiltInS : uint32_t { BI_Position = 0, BI_PointSize = 1, BI_FragCoord = 15, BI_VertexIndex = 42,
+                           BI_InstanceIndex = 43 };
+
+struct SType {
+    enum K { Void, Bool, Int, Float, Vector, Matrix, Array, Struct, Pointer, Function, Image, SampledImage, Sampler };
+    K k = Void;
+    uint32_t width = 32, is_signed = 0;
+    uint32_t elem = 0, count = 0;  // vector/matrix/array
+    std::vector<uint32_t> members;
+    uint32_t storage = 0, pointee = 0;
+};
+
+// A translated SPIR-V id.
+struct Val {
+    std::vector<uint32_t> ir;  // one IR value; four for a matrix (columns)
+    uint32_t type = 0;         // SPIR-V type id of the value (pointee type for pointers)
+    bool is_ptr = false;
+    uint32_t storage = 0;
+    uint32_t mat_stride = 16;  // pointers to matrices: column stride
+    int layout = 0;            // pointers: 0 natural, 1 explicit (Offset/strides), 2 I/O locations
+    std::string builtin_struct;  // pointer to a gl_PerVertex-style block
+    uint32_t struct_var = 0;
+};
+
+struct Translator {
+    const SpirvModule &sm;
+    ir::Module &m;
+    std::string &err;
+    std::map<uint32_t, SType> types;
+    std::map<uint32_t, std::string> names;
+    std::map<uint32_t, std::vector<std::vector<uint32_t>>> decos;  // id -> [deco, params...]
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<std::vector<uint32_t>>> mdecos;
+    std::map<uint32_t, Val> vals;
+    std::map<uint32_t, std::pair<uint32_t, std::vector<uint32_t>>> consts;  // id -> (type, words)
+    std::map<uint32_t, std::vector<uint8_t>> spec;
+    uint32_t glsl_set = 0;
+    std::string file_name;
+    std::map<uint32_t, std::string> strings;
+    // builtin member map for block variables: (var id, member) -> IR pointer
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> block_builtin_ptr;
+    ir::Location cur_loc;
+    bool have_loc = false;
+
+    Translator(const SpirvModule &s, ir::Module &mm, std::string &e) : sm(s), m(mm), err(e) {}
+
+    bool fail(const std::string &s) {
+        if (err.empty()) err = s;
+        return false;
+    }
+
+    const std::vector<uint32_t> *deco(uint32_t id, uint32_t d) const {
+        auto it = decos.find(id);
+        if (it == decos.end()) return nullptr;
+        for (const auto &v : it->second)
+            if (v[0] == d) return &v;
+        return nullptr;
+    }
+    const std::vector<uint32_t> *mdeco(uint32_t id, uint32_t mem, uint32_t d) const {
+        auto it = mdecos.find({id, mem});
+        if (it == mdecos.end()) return nullptr;
+        for (const auto &v : it->second)
+            if (v[0] == d) return &v;
+        return nullptr;
+    }
+
+    // IR type of a SPIR-V scalar/vector type; Void for anything else.
+    Type ir_type(uint32_t tid) const {
+        auto it = types.find(tid);
+        if (it == types.end()) return Type{};
+        const SType &t = it->second;
+        switch (t.k) {
+        case SType::Bool: return Type::boolean();
+        case SType::Int: return Type::i32();
+        case SType::Float: return Type::f32();
+        case SType::Vector: {
+            Type e = ir_type(t.elem);
+            e.lanes = (uint8_t)t.count;
+            return e;
+        }
+        case SType::SampledImage: return Type::texture();
+        default: return Type{};
+        }
+    }
+
+    // Byte size of a type under a layout (0 natural, 1 explicit, 2 I/O).
+    uint32_t size_of(uint32_t tid, int layout) const {
+        const SType &t = types.at(tid);
+        switch (t.k) {
+        case SType::Bool:
+        case SType::Int:
+        case SType::Float: return 4;
+        case SType::Vector: return layout == 2 ? 16 : 4 * t.count;
+        case SType::Matrix: return t.count * (layout == 2 ? 16 : 16);
+        case SType::Array: {
+            uint32_t n = const_u32(t.count);
+            uint32_t stride = array_stride(tid, layout);
+            return n * stride;
+        }
+        case SType::Struct: {
+            uint32_t end = 0;
+            for (uint32_t i = 0; i < t.members.size(); ++i) {
+                const uint32_t off = member_offset(tid, i, layout);
+                end = std::max(end, off + size_of(t.members[i], layout));
+            }
+            return end;
+        }
+        default: return 0;
+        }
+    }
+    uint32_t array_stride(uint32_t tid, int layout) const {
+        const SType &t = types.at(tid);
+        if (layout == 1) {
+            if (auto d = deco(tid, D_ArrayStride)) return (*d)[1];
+        }
+        if (layout == 2) return 16 * ((size_of(t.elem, 2) + 15) / 16);
+        const uint32_t s = size_of(t.elem, layout);
+        return (s + 3) / 4 * 4;
+    }
+    uint32_t member_offset(uint32_t sid, uint32_t i, int layout) const {
+        if (layout == 1) {
+            if (auto d = mdeco(sid, i, D_Offset)) return (*d)[1];
+        }
+        const SType &t = types.at(sid);
+        uint32_t off = 0;
+        for (uint32_t k = 0; k < i; ++k) {
+            uint32_t s = size_of(t.members[k], layout);
+            if (layout == 2) s = (s + 15) / 16 * 16;
+            off += s;
+        }
+        return off;
+    }
+    uint32_t matrix_stride(uint32_t struct_id, uint32_t member, int layout) const {
+        if (layout == 1 && struct_id) {
+            if (auto d = mdeco(struct_id, member, D_MatrixStride)) return (*d)[1];
+        }
+        return 16;
+    }
+    uint32_t const_u32(uint32_t id) const {
+        auto it = consts.find(id);
+        if (it == consts.end() || it->second.second.empty()) return 0;
+        return it->second.second[0];
+    }
+
+    uint32_t emit(Op op, std::vector<uint32_t> args, const Type &rt, std::vector<uint32_t> fields = {},
+                  const std::string &name = "") {
+        Instruction ins;
+        ins.op = op;
+        ins.args = std::move(args);
+        ins.fields = std::move(fields);
+        uint32_t r = 0;
+        if (rt.scalar != ir::Scalar::Void) {
+            r = m.new_value(rt, name);
+            ins.results.push_back(r);
+        }
+        if (have_loc) {
+            ins.has_location = true;
+            ins.loc = cur_loc;
+        }
+        m.main.body.body.push_back(std::move(ins));
+        return r;
+    }
+
+    uint32_t ptr_offset(uint32_t base, uint32_t off) {
+        if (off == 0) return base;
+        return emit(Op::PtrOffset, {base}, Type::ptr(), {off});
+    }
+
+    std::string value_name(uint32_t sid) {
+        auto it = names.find(sid);
+        if (it != names.end() && !it->second.empty()) return it->second;
+        return "id_" + std::to_string(sid);
+    }
+
+    bool collect();
+    bool build_interface(uint32_t entry_fn, const std::vector<uint32_t> &iface, ir::Stage stage);
+    bool translate_function(uint32_t fn);
+    bool load(uint32_t rtype, uint32_t rid, const Val &p);
+    bool store(const Val &p, const Val &v);
+    bool access_chain(uint32_t rid, uint32_t ptr_type, const Val &base, const std::vector<uint32_t> &idx);
+    const Val *get(uint32_t id);
+};
+
+bool Translator::collect() {
+    for (const auto &ins : sm.instructions) {
+        const auto &o = ins.operands;
+        switch (ins.opcode) {
+        case OpName: names[o[0]] = spirv_string(o, 1); break;
+        case OpString: strings[o[0]] = spirv_string(o, 1); break;
+        case OpDecorate: {
+            std::vector<uint32_t> d(o.begin() + 1, o.end());
+            decos[o[0]].push_back(d);
+            break;
+        }
+        case OpMemberDecorate: {
+            std::vector<uint32_t> d(o.begin() + 2, o.end());
+            mdecos[{o[0], o[1]}].push_back(d);
+            break;
+        }
+        case OpExtInstImport:
+            if (spirv_string(o, 1) != "GLSL.std.450") return fail("unsupported extended instruction set");
+            glsl_set = o[0];
+            break;
+        case OpTypeVoid: types[o[0]].k = SType::Void; break;
+        case OpTypeBool: types[o[0]].k = SType::Bool; break;
+        case OpTypeInt: {
+            SType t;
+            t.k = SType::Int;
+            t.width = o[1];
+            t.is_signed = o[2];
+            if (t.width != 32) return fail("only 32-bit integer types are supported");
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypeFloat: {
+            SType t;
+            t.k = SType::Float;
+            t.width = o[1];
+            if (t.width != 32) return fail("only 32-bit float types are supported");
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypeVector:
+        case OpTypeMatrix: {
+            SType t;
+            t.k = ins.opcode == OpTypeVector ? SType::Vector : SType::Matrix;
+            t.elem = o[1];
+            t.count = o[2];
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypeArray: {
+            SType t;
+            t.k = SType::Array;
+            t.elem = o[1];
+            t.count = o[2];  // constant id
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypeStruct: {
+            SType t;
+            t.k = SType::Struct;
+            t.members.assign(o.begin() + 1, o.end());
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypePointer: {
+            SType t;
+            t.k = SType::Pointer;
+            t.storage = o[1];
+            t.pointee = o[2];
+            types[o[0]] = t;
+            break;
+        }
+        case OpTypeFunction: types[o[0]].k = SType::Function; break;
+        case OpTypeImage: types[o[0]].k = SType::Image; break;
+        case OpTypeSampler: types[o[0]].k = SType::Sampler; break;
+        case OpTypeSampledImage: {
+            SType t;
+            t.k = SType::SampledImage;
+            t.elem = o[1];
+            types[o[0]] = t;
+            break;
+        }
+        case OpConstant:
+        case OpSpecConstant: {
+            std::vector<uint32_t> w(o.begin() + 2, o.end());
+            if (ins.opcode == OpSpecConstant) {
+                if (auto d = deco(o[1], D_SpecId)) {
+                    auto it = spec.find((*d)[1]);
+                    if (it != spec.end() && it->second.size() >= 4) std::memcpy(&w[0], it->second.data(), 4);
+                }
+            }
+            consts[o[1]] = {o[0], w};
+            break;
+        }
+        case OpConstantTrue:
+        case OpConstantFalse:
+        case OpSpecConstantTrue:
+        case OpSpecConstantFalse:
+            consts[o[1]] = {o[0], {(uint32_t)(ins.opcode == OpConstantTrue || ins.opcode == OpSpecConstantTrue)}};
+            break;
+        case OpConstantComposite: {
+            std::vector<uint32_t> w;
+            for (size_t i = 2; i < o.size(); ++i) {
+                auto it = consts.find(o[i]);
+                if (it == consts.end()) return fail("composite constant of non-scalar constants is not supported");
+                w.push_back(it->second.second[0]);
+            }
+            consts[o[1]] = {o[0], w};
+            break;
+        }
+        default: break;
+        }
+    }
+    return true;
+}
+
+const Val *Translator::get(uint32_t id) {
+    auto it = vals.find(id);
+    if (it != vals.end()) return &it->second;
+    auto c = consts.find(id);
+    if (c == consts.end()) {
+        fail("use of undefined or unsupported id %" + std::to_string(id));
+        return nullptr;
+    }
+    const uint32_t tid = c->second.first;
+    const Type t = ir_type(tid);
+    Val v;
+    v.type = tid;
+    if (t.scalar == ir::Scalar::Void) {
+        fail("unsupported constant type");
+        return nullptr;
+    }
+    const uint32_t vid = m.new_value(t, "");
+    ir::Value &iv = m.value(vid);
+    iv.is_const = true;
+    for (size_t i = 0; i < c->second.second.size() && i < 4; ++i) iv.const_bits[i] = c->second.second[i];
+    v.ir.push_back(vid);
+    vals[id] = v;
+    return &vals[id];
+}
+
+bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> &iface, ir::Stage stage) {
+    (void)entry_fn;
+    m.user_inputs.pointer = m.new_value(Type::ptr(), "user_inputs_block");
+    m.user_outputs.pointer = m.new_value(Type::ptr(), "user_outputs_block");
+    m.user_inputs.alignment = m.user_outputs.alignment = 16;
+    std::set<uint32_t> in_iface(iface.begin(), iface.end());
+
+    // Walk every module-scope variable in declaration order.
+    for (const auto &ins : sm.instructions) {
+        if (ins.opcode == OpFunction) break;
+        if (ins.opcode != OpVariable) continue;
+        const uint32_t ptype = ins.operands[0], id = ins.operands[1], sc = ins.operands[2];
+        const SType &pt = types.at(ptype);
+        const uint32_t pointee = pt.pointee;
+        Val v;
+        v.is_ptr = true;
+        v.type = pointee;
+        v.storage = sc;
+        if (sc == SC_Input || sc == SC_Output) {
+            if (!in_iface.count(id)) continue;
+            const bool out = sc == SC_Output;
+            v.layout = 2;
+            if (auto bi = deco(id, D_BuiltIn)) {
+                const uint32_t b = (*bi)[1];
+                ir::InterfaceVariable iv;
+                iv.type = ir_type(pointee);
+                iv.alignment = 4;
+                if (b == BI_Position && out) {
+                    iv.kind = ir::BuiltIn::VertexPosition;
+                    iv.pointer = m.new_value(Type::ptr(), "vertex_position_out");
+                    m.built_in_outputs.push_back(iv);
+                } else if (b == BI_VertexIndex && !out) {
+                    iv.kind = ir::BuiltIn::VertexIndex;
+                    iv.pointer = m.new_value(Type::ptr(), "vertex_index_in");
+                    m.built_in_inputs.push_back(iv);
+                } else if (b == BI_InstanceIndex && !out) {
+                    iv.kind = ir::BuiltIn::InstanceIndex;
+                    iv.pointer = m.new_value(Type::ptr(), "instance_index_in");
+                    m.built_in_inputs.push_back(iv);
+                } else if (b == BI_FragCoord && !out) {
+                    iv.kind = ir::BuiltIn::FragCoord;
+                    iv.pointer = m.new_value(Type::ptr(), "fragment_coord_in");
+                    m.built_in_inputs.push_back(iv);
+                } else {
+                    return fail("unsupported built-in " + std::to_string(b));
+                }
+                v.ir.push_back(iv.pointer);
+                v.layout = 0;
+                vals[id] = v;
+                continue;
+            }
+            const SType &st = types.at(pointee);
+            if (st.k == SType::Struct && mdeco(pointee, 0, D_BuiltIn)) {
+                // gl_PerVertex-style block: each built-in member is its own
+                // interface variable.
+                for (uint32_t i = 0; i < st.members.size(); ++i) {
+                    auto bi = mdeco(pointee, i, D_BuiltIn);
+                    if (!bi) return fail("block mixes built-in and user members");
+                    const uint32_t b = (*bi)[1];
+                    if (b == BI_Position && out) {
+                        ir::InterfaceVariable iv;
+                        iv.type = ir_type(st.members[i]);
+                        iv.alignment = 4;
+                        iv.kind = ir::BuiltIn::VertexPosition;
+                        iv.pointer = m.new_value(Type::ptr(), "vertex_position_out");
+                        m.built_in_outputs.push_back(iv);
+                        block_builtin_ptr[{id, i}] = iv.pointer;
+                    }
+                    // other per-vertex built-ins (point size, clip distances)
+                    // are written to a discarded slot
+                }
+                v.builtin_struct = "per_vertex";
+                v.struct_var = id;
+                vals[id] = v;
+                continue;
+            }
+            auto loc = deco(id, D_Location);
+            if (!loc) return fail("user interface variable without a Location");
+            uint32_t comp = 0;
+            if (auto c = deco(id, D_Component)) comp = (*c)[1];
+            const uint32_t off = (*loc)[1] * 16 + comp * 4;
+            ir::InterfaceBlock &blk = out ? m.user_outputs : m.user_inputs;
+            // members are recorded per scalar/vector leaf
+            std::vector<std::pair<uint32_t, uint32_t>> stack{{pointee, off}};
+            while (!stack.empty()) {
+                auto [t, o] = stack.back();
+                stack.pop_back();
+                const SType &tt = types.at(t);
+                if (tt.k == SType::Array || tt.k == SType::Matrix) {
+                    const uint32_t n = tt.k == SType::Array ? const_u32(tt.count) : tt.count;
+                    for (uint32_t i = n; i-- > 0;) stack.push_back({tt.elem, o + 16 * i});
+                } else if (tt.k == SType::Struct) {
+                    return fail("struct-typed interface variables are not supported");
+                } else {
+                    ir::BlockMember bm;
+                    bm.offset = o;
+                    bm.type = ir_type(t);
+                    blk.members.push_back(bm);
+                    blk.size = std::max(blk.size, (o + 16) / 16 * 16);
+                }
+            }
+            v.ir.push_back(ptr_offset(out ? m.user_outputs.pointer : m.user_inputs.pointer, off));
+            vals[id] = v;
+        } else if (sc == SC_PushConstant || sc == SC_Uniform || sc == SC_UniformConstant) {
+            ir::Resource r;
+            if (sc == SC_PushConstant) {
+                r.kind = ir::ResourceKind::PushConstants;
+                r.pointer = m.new_value(Type::ptr(), "push_constants");
+            } else {
+                auto set = deco(id, D_DescriptorSet), bind = deco(id, D_Binding);
+                r.set = set ? (*set)[1] : 0;
+                r.binding = bind ? (*bind)[1] : 0;
+                if (sc == SC_Uniform) {
+                    r.kind = ir::ResourceKind::UniformBuffer;
+                    r.pointer = m.new_value(Type::ptr(), value_name(id));
+                } else {
+                    if (types.at(pointee).k != SType::SampledImage)
+                        return fail("only combined image samplers are supported in UniformConstant storage");
+                    r.kind = ir::ResourceKind::CombinedImageSampler;
+                    r.pointer = m.new_value(Type::texture(), value_name(id));
+                }
+            }
+            m.resources.push_back(r);
+            v.layout = 1;
+            v.ir.push_back(r.pointer);
+            vals[id] = v;
+        } else if (sc == SC_Private) {
+            ir::Variable gv;
+            gv.size = size_of(pointee, 0);
+            gv.type = ir_type(pointee);
+            gv.alignment = 4;
+            gv.pointer = m.new_value(Type::ptr(), "global_var" + std::to_string(m.invocation_globals.size() + 1));
+            m.invocation_globals.push_back(gv);
+            v.layout = 0;
+            v.ir.push_back(gv.pointer);
+            vals[id] = v;
+        } else {
+            return fail("unsupported storage class " + std::to_string(sc));
+        }
+    }
+    (void)stage;
+    return true;
+}
+
+bool Translator::load(uint32_t rtype, uint32_t rid, const Val &p) {
+    const SType &t = types.at(p.type);
+    Val v;
+    v.type = rtype;
+    if (t.k == SType::SampledImage) {
+        v.ir = p.ir;
+        vals[rid] = v;
+        return true;
+    }
+    if (t.k == SType::Matrix) {
+        const Type col = ir_type(t.elem);
+        for (uint32_t c = 0; c < t.count; ++c) {
+            const uint32_t pc = ptr_offset(p.ir[0], c * p.mat_stride);
+            v.ir.push_back(emit(Op::Load, {pc}, col, {4}, value_name(rid) + "_c" + std::to_string(c)));
+        }
+        vals[rid] = v;
+        return true;
+    }
+    const Type rt = ir_type(rtype);
+    if (rt.scalar == ir::Scalar::Void) return fail("load of an unsupported type");
+    v.ir.push_back(emit(Op::Load, {p.ir[0]}, rt, {4}, value_name(rid)));
+    vals[rid] = v;
+    return true;
+}
+
+bool Translator::store(const Val &p, const Val &v) {
+    const SType &t = types.at(p.type);
+    if (t.k == SType::Matrix) {
+        for (uint32_t c = 0; c < t.count; ++c) emit(Op::Store, {ptr_offset(p.ir[0], c * p.mat_stride), v.ir[c]}, Type{}, {4});
+        return true;
+    }
+    if (ir_type(p.type).scalar == ir::Scalar::Void) return fail("store of an unsupported type");
+    emit(Op::Store, {p.ir[0], v.ir[0]}, Type{}, {4});
+    return true;
+}
+
+bool Translator::access_chain(uint32_t rid, uint32_t ptr_type, const Val &base, const std::vector<uint32_t> &idx) {
+    Val r = base;
+    r.type = types.at(ptr_type).pointee;
+    uint32_t cur = base.type;
+    size_t first = 0;
+    uint32_t ptr = base.ir.empty() ? 0 : base.ir[0];
+    if (!base.builtin_struct.empty()) {
+        if (idx.empty()) return fail("access chain into a per-vertex block needs an index");
+        const uint32_t mem = const_u32(idx[0]);
+        auto it = block_builtin_ptr.find({base.struct_var, mem});
+        if (it == block_builtin_ptr.end()) {
+            // an unused per-vertex built-in: give it private storage
+            ir::Variable lv;
+            lv.type = ir_type(types.at(cur).members[mem]);
+            lv.size = 16;
+            lv.pointer = m.new_value(Type::ptr(), "discarded_builtin" + std::to_string(m.main.locals.size()));
+            m.main.locals.push_back(lv);
+            block_builtin_ptr[{base.struct_var, mem}] = lv.pointer;
+            it = block_builtin_ptr.find({base.struct_var, mem});
+        }
+        ptr = it->second;
+        cur = types.at(cur).members[mem];
+        r.builtin_struct.clear();
+        r.layout = 0;
+        first = 1;
+    }
+    uint32_t off = 0;
+    for (size_t i = first; i < idx.size(); ++i) {
+        if (!consts.count(idx[i])) return fail("dynamic access-chain indices are not supported");
+        const uint32_t k = const_u32(idx[i]);
+        const SType &t = types.at(cur);
+        switch (t.k) {
+        case SType::Struct:
+            if (k >= t.members.size()) return fail("struct index out of range");
+            off += member_offset(cur, k, r.layout);
+            r.mat_stride = matrix_stride(cur, k, r.layout);
+            cur = t.members[k];
+            break;
+        case SType::Array:
+            off += k * array_stride(cur, r.layout);
+            cur = t.elem;
+            break;
+        case SType::Matrix:
+            off += k * r.mat_stride;
+            cur = t.elem;
+            break;
+        case SType::Vector:
+            off += 4 * k;
+            cur = t.elem;
+            break;
+        default: return fail("access chain through a non-composite type");
+        }
+    }
+    r.ir = {ptr_offset(ptr, off)};
+    r.type = cur;
+    vals[rid] = r;
+    return true;
+}
+
+bool Translator::translate_function(uint32_t fn) {
+    bool in_fn = false, seen_label = false, done = false;
+    for (const auto &ins : sm.instructions) {
+        const auto &o = ins.operands;
+        if (!in_fn) {
+            if (ins.opcode == OpFunction && o[1] == fn) {
+                in_fn = true;
+                m.main.name = "main";
+            }
+            continue;
+        }
+        if (ins.opcode == OpFunctionEnd) break;
+        if (done) continue;
+        auto V = [&](uint32_t id) -> const Val * { return get(id); };
+        switch (ins.opcode) {
+        case OpLine:
+            cur_loc.file = strings.count(o[0]) ? strings[o[0]] : value_name(o[0]);
+            cur_loc.line = o[1];
+            cur_loc.column = o[2];
+            have_loc = true;
+            break;
+        case OpNoLine: have_loc = false; break;
+        case OpLabel:
+            if (seen_label) return fail("structured control flow is not supported yet (more than one block)");
+            seen_label = true;
+            m.main.body.name = value_name(o[0]);
+            break;
+        case OpVariable: {
+            if (o[2] != SC_Function) return fail("non-function variable inside a function");
+            const uint32_t pointee = types.at(o[0]).pointee;
+            ir::Variable lv;
+            lv.type = ir_type(pointee);
+            lv.size = size_of(pointee, 0);
+            lv.alignment = 4;
+            lv.pointer = m.new_value(Type::ptr(), value_name(o[1]));
+            m.main.locals.push_back(lv);
+            Val v;
+            v.is_ptr = true;
+            v.type = pointee;
+            v.storage = SC_Function;
+            v.ir.push_back(lv.pointer);
+            vals[o[1]] = v;
+            if (o.size() > 3) {
+                const Val *init = V(o[3]);
+                if (!init || !store(v, *init)) return false;
+            }
+            break;
+        }
+        case OpLoad: {
+            const Val *p = V(o[2]);
+            if (!p || !load(o[0], o[1], *p)) return false;
+            break;
+        }
+        case OpStore: {
+            const Val *p = V(o[0]);
+            const Val *v = V(o[1]);
+            if (!p || !v || !store(*p, *v)) return false;
+            break;
+        }
+        case OpAccessChain:
+        case OpInBoundsAccessChain: {
+            const Val *b = V(o[2]);
+            if (!b) return false;
+            std::vector<uint32_t> idx(o.begin() + 3, o.end());
+            if (!access_chain(o[1], o[0], *b, idx)) return false;
+            break;
+        }
+        case OpCompositeExtract: {
+            const Val *c = V(o[2]);
+            if (!c) return false;
+            const SType &ct = types.at(c->type);
+            Val r;
+            r.type = o[0];
+            if (ct.k == SType::Matrix) {
+                const uint32_t col = c->ir.at(o[3]);
+                if (o.size() == 4) {
+                    r.ir.push_back(col);
+                } else {
+                    r.ir.push_back(emit(Op::Extract, {col}, ir_type(o[0]), {o[4]}, value_name(o[1])));
+                }
+            } else if (ct.k == SType::Vector) {
+                if (o.size() != 4) return fail("nested extract from a vector");
+                r.ir.push_back(emit(Op::Extract, {c->ir[0]}, ir_type(o[0]), {o[3]}, value_name(o[1])));
+            } else {
+                return fail("composite extract from an unsupported type");
+            }
+            vals[o[1]] = r;
+            break;
+        }
+        case OpCompositeConstruct: {
+            const SType &rt = types.at(o[0]);
+            Val r;
+            r.type = o[0];
+            if (rt.k == SType::Vector) {
+                std::vector<uint32_t> parts;
+                for (size_t i = 2; i < o.size(); ++i) {
+                    const Val *p = V(o[i]);
+                    if (!p) return false;
+                    const Type pt = m.value(p->ir[0]).type;
+                    if (pt.lanes == 1) {
+                        parts.push_back(p->ir[0]);
+                    } else {
+                        for (int l = 0; l < pt.lanes; ++l)
+                            parts.push_back(emit(Op::Extract, {p->ir[0]}, Type{pt.scalar, 1}, {(uint32_t)l}));
+                    }
+                }
+                r.ir.push_back(emit(Op::MakeVector, parts, ir_type(o[0]), {}, value_name(o[1])));
+            } else if (rt.k == SType::Matrix) {
+                for (size_t i = 2; i < o.size(); ++i) {
+                    const Val *p = V(o[i]);
+                    if (!p) return false;
+                    r.ir.push_back(p->ir[0]);
+                }
+            } else {
+                return fail("composite construct of an unsupported type");
+            }
+            vals[o[1]] = r;
+            break;
+        }
+        case OpVectorShuffle: {
+            const Val *a = V(o[2]);
+            const Val *b = V(o[3]);
+            if (!a || !b) return false;
+            std::vector<uint32_t> lanes(o.begin() + 4, o.end());
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(Op::Shuffle, {a->ir[0], b->ir[0]}, ir_type(o[0]), lanes, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpFNegate:
+        case OpConvertSToF:
+        case OpConvertFToS: {
+            const Val *a = V(o[2]);
+            if (!a) return false;
+            const Op op = ins.opcode == OpFNegate ? Op::FNeg : ins.opcode == OpConvertSToF ? Op::ConvSToF : Op::ConvFToS;
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(op, {a->ir[0]}, ir_type(o[0]), {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpIAdd:
+        case OpFAdd:
+        case OpFSub:
+        case OpFMul:
+        case OpFDiv:
+        case OpDot:
+        case OpFOrdLessThan:
+        case OpFOrdGreaterThan: {
+            const Val *a = V(o[2]);
+            const Val *b = V(o[3]);
+            if (!a || !b) return false;
+            Op op;
+            switch (ins.opcode) {
+            case OpIAdd: op = Op::Add; break;
+            case OpFAdd: op = Op::FAdd; break;
+            case OpFSub: op = Op::FSub; break;
+            case OpFMul: op = Op::FMul; break;
+            case OpFDiv: op = Op::FDiv; break;
+            case OpDot: op = Op::Dot; break;
+            case OpFOrdLessThan: op = Op::FOrdLt; break;
+            default: op = Op::FOrdGt; break;
+            }
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(op, {a->ir[0], b->ir[0]}, ir_type(o[0]), {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpSelect: {
+            const Val *c = V(o[2]);
+            const Val *a = V(o[3]);
+            const Val *b = V(o[4]);
+            if (!c || !a || !b) return false;
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(Op::Select, {c->ir[0], a->ir[0], b->ir[0]}, ir_type(o[0]), {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpVectorTimesScalar: {
+            const Val *v = V(o[2]);
+            const Val *s = V(o[3]);
+            if (!v || !s) return false;
+            const Type vt = ir_type(o[0]);
+            const uint32_t sp = emit(Op::Splat, {s->ir[0]}, vt);
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(Op::FMul, {v->ir[0], sp}, vt, {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpMatrixTimesVector: {
+            const Val *mm = V(o[2]);
+            const Val *v = V(o[3]);
+            if (!mm || !v) return false;
+            const Type vt = ir_type(o[0]);
+            uint32_t acc = 0;
+            for (size_t c = 0; c < mm->ir.size(); ++c) {
+                const uint32_t vc = emit(Op::Extract, {v->ir[0]}, Type::f32(), {(uint32_t)c});
+                const uint32_t prod = emit(Op::FMul, {mm->ir[c], emit(Op::Splat, {vc}, vt)}, vt);
+                acc = c == 0 ? prod : emit(Op::FAdd, {acc, prod}, vt);
+            }
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(acc);
+            m.value(acc).name = value_name(o[1]);
+            vals[o[1]] = r;
+            break;
+        }
+        case OpExtInst: {
+            if (o[2] != glsl_set) return fail("unknown extended instruction set");
+            const uint32_t inst = o[3];
+            std::vector<const Val *> a;
+            for (size_t i = 4; i < o.size(); ++i) {
+                const Val *p = V(o[i]);
+                if (!p) return false;
+                a.push_back(p);
+            }
+            const Type rt = ir_type(o[0]);
+            uint32_t res = 0;
+            auto one = [&]() { return m.new_const_f32(1.0f); };
+            switch (inst) {
+            case 31: res = emit(Op::Sqrt, {a[0]->ir[0]}, rt); break;                                    // Sqrt
+            case 32: res = emit(Op::FDiv, {one(), emit(Op::Sqrt, {a[0]->ir[0]}, rt)}, rt); break;      // InverseSqrt
+            case 37: res = emit(Op::FMin, {a[0]->ir[0], a[1]->ir[0]}, rt); break;                      // FMin
+            case 40: res = emit(Op::FMax, {a[0]->ir[0], a[1]->ir[0]}, rt); break;                      // FMax
+            case 43: res = emit(Op::FClamp, {a[0]->ir[0], a[1]->ir[0], a[2]->ir[0]}, rt); break;       // FClamp
+            case 66: {                                                                                  // Length
+                const uint32_t d = emit(Op::Dot, {a[0]->ir[0], a[0]->ir[0]}, Type::f32());
+                res = emit(Op::Sqrt, {d}, rt);
+                break;
+            }
+            case 69: {                                                                                  // Normalize
+                const uint32_t d = emit(Op::Dot, {a[0]->ir[0], a[0]->ir[0]}, Type::f32());
+                const uint32_t rl = emit(Op::FDiv, {one(), emit(Op::Sqrt, {d}, Type::f32())}, Type::f32());
+                res = emit(Op::FMul, {a[0]->ir[0], emit(Op::Splat, {rl}, rt)}, rt);
+                break;
+            }
+            default: return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
+            }
+            m.value(res).name = value_name(o[1]);
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(res);
+            vals[o[1]] = r;
+            break;
+        }
+        case OpImageSampleImplicitLod:
+        case OpImageSampleExplicitLod: {
+            const Val *tex = V(o[2]);
+            const Val *coord = V(o[3]);
+            if (!tex || !coord) return false;
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(Op::Sample, {tex->ir[0], coord->ir[0]}, ir_type(o[0]), {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
+        case OpKill:
+            emit(Op::Kill, {}, Type{});
+            emit(Op::BreakBlock, {}, Type{});
+            done = true;
+            break;
+        case OpReturn:
+            emit(Op::BreakBlock, {}, Type{});
+            done = true;
+            break;
+        case OpSelectionMerge:
+        case OpLoopMerge:
+        case OpBranch:
+        case OpBranchConditional:
+            return fail("structured control flow is not supported yet");
+        default:
+            return fail("unsupported SPIR-V opcode " + std::to_string(ins.opcode) + " in function body");
+        }
+    }
+    if (!in_fn) return fail("entry point function not found");
+    if (!done) return fail("entry function does not end in OpReturn or OpKill");
+    return true;
+}
+
+}  // namespace
+
+bool translate_spirv(const uint32_t *words, size_t nwords, const std::string &entry_name, ir::Stage stage,
+                     const std::vector<SpecConstant> &spec, ir::Module &out, std::string &err) {
+    SpirvModule sm;
+    if (!spirv_parse(words, nwords, sm, err)) return false;
+    out = ir::Module();
+    out.stage = stage;
+    Translator tr(sm, out, err);
+    for (const auto &s : spec) tr.spec[s.id] = s.bytes;
+    if (!tr.collect()) return false;
+    const uint32_t model = stage == ir::Stage::Vertex ? 0u : stage == ir::Stage::Fragment ? 4u : 5u;
+    uint32_t fn = 0;
+    std::vector<uint32_t> iface;
+    for (const auto &ins : sm.instructions) {
+        if (ins.opcode == OpCapability) {
+            const uint32_t cap = ins.operands[0];
+            if (cap != 0 && cap != 1 && cap != 11) return tr.fail("unsupported capability " + std::to_string(cap));
+        } else if (ins.opcode == OpMemoryModel) {
+            if (ins.operands[0] != 0) return tr.fail("only the Logical addressing model is supported");
+        } else if (ins.opcode == OpEntryPoint && ins.operands[0] == model) {
+            size_t next;
+            if (spirv_string(ins.operands, 2, &next) == entry_name) {
+                fn = ins.operands[1];
+                iface.assign(ins.operands.begin() + next, ins.operands.end());
+            }
+        }
+    }
+    if (!fn) return tr.fail("entry point \"" + entry_name + "\" not found for this stage");
+    bool upper_left = stage != ir::Stage::Fragment;
+    for (const auto &ins : sm.instructions) {
+        if (ins.opcode != OpExecutionMode || ins.operands[0] != fn) continue;
+        const uint32_t mode = ins.operands[1];
+        if (mode == 7) upper_left = true;
+        else if (mode == 8) return tr.fail("OriginLowerLeft is not supported (Vulkan requires OriginUpperLeft)");
+    }
+    if (!upper_left) return tr.fail("fragment shader without OriginUpperLeft");
+    if (!tr.build_interface(fn, iface, stage)) return false;
+    return tr.translate_function(fn);
+}
+
+}  // namespace kz
