@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "kernels.h"
@@ -57,11 +58,15 @@ bool Engine::init(int hip_device, std::string &err) {
         return false;
     }
     std::memset(stats_h_, 0, KZ_STAT_COUNT * sizeof(unsigned long long));
+    // KAZAN_LIST_CAPACITY_LIMIT caps the per-tile list storage (entries);
+    // tiles whose lists do not fit render through the in-order scan path
+    if (const char *e = std::getenv("KAZAN_LIST_CAPACITY_LIMIT")) list_limit_ = std::strtoull(e, nullptr, 10);
     return true;
 }
 
 hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
-    const uint64_t want_list = std::max<uint64_t>(list_capacity_, 4ull * tris + 65536ull);
+    uint64_t want_list = std::max<uint64_t>(list_capacity_, 4ull * tris + 65536ull);
+    if (list_limit_) want_list = std::min<uint64_t>(want_list, list_limit_);
     if (tiles <= cap_tiles_ && tris <= cap_tris_ && want_list <= list_capacity_) return hipSuccess;
     // buffers may still be in use by queued work
     KZ_TRY(hipStreamSynchronize(stream_));
@@ -214,7 +219,8 @@ void Engine::collect_stats() {
     // A submission whose lists overflowed rendered correctly through the
     // in-order scan fallback; size the lists for the next one.
     if (last_.counters[KZ_STAT_LIST_OVERFLOW]) {
-        const uint64_t need = last_.counters[KZ_STAT_LIST_ENTRIES] + last_.counters[KZ_STAT_LIST_ENTRIES] / 4;
+        uint64_t need = last_.counters[KZ_STAT_LIST_ENTRIES] + last_.counters[KZ_STAT_LIST_ENTRIES] / 4;
+        if (list_limit_) need = std::min<uint64_t>(need, list_limit_);
         if (need > list_capacity_ && need < 0xFFFFFFF0ull) {
             hipStreamSynchronize(stream_);
             if (grow(tile_lists_, need) == hipSuccess && grow(sort_tmp_, need) == hipSuccess)

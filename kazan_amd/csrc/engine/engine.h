@@ -110,6 +110,7 @@ class Engine {
     hipError_t mark(int slot, bool end);
 
     bool kernel_timing_ = false;
+    uint64_t list_limit_ = 0;
     std::vector<std::pair<int, hipEvent_t>> timing_events_;  // (slot, start) then end, alternating
     std::vector<hipEvent_t> event_pool_;
     size_t events_used_ = 0;

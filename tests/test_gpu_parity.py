@@ -91,10 +91,19 @@ def test_dense_depth_and_blend(ctx, oracle):
     check_parity(ctx, oracle, s)
 
 
-def test_huge_triangles_list_overflow(ctx, oracle):
-    # 24 triangles each spanning the whole 1080p target: more (triangle,
-    # tile) pairs than the first list allocation holds, so the tiles take the
+@pytest.fixture
+def small_list_ctx(driver, monkeypatch):
+    # a device whose tile lists hold at most 40,000 (triangle, tile) pairs
+    from kazan_amd.scenes.renderer import Context
+    monkeypatch.setenv("KAZAN_LIST_CAPACITY_LIMIT", "40000")
+    return Context(driver)
+
+
+def test_huge_triangles_list_overflow(small_list_ctx, oracle):
+    # 24 triangles each spanning the whole 1080p target: ~196,000 (triangle,
+    # tile) pairs, more than the device's lists hold, so most tiles take the
     # in-order scan path
+    ctx = small_list_ctx
     r = np.random.default_rng(13)
     tris = []
     for k in range(24):
