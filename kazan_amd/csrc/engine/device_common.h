@@ -92,5 +92,10 @@ enum {
     KZ_STAT_COUNT = 8
 };
 
+// Statistics are accumulated in KZ_STAT_STRIPES copies (a workgroup adds to
+// copy blockIdx % KZ_STAT_STRIPES) so that no single address takes every
+// workgroup's atomics; kz_stats_reduce folds them at the end of a submission.
+#define KZ_STAT_STRIPES 256
+
 // Tile rectangle packing: tx0 | ty0 << 16 | tx1 << 32 | ty1 << 48.
 #define KZ_RECT_EMPTY (~0ULL)

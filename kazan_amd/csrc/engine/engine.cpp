@@ -21,6 +21,9 @@ hipError_t grow(T *&p, uint64_t count) {
     return hipMalloc((void **)&p, std::max<uint64_t>(count, 1) * sizeof(T));
 }
 
+// striped partial counters followed by their totals
+constexpr uint64_t kStatWords = (uint64_t)(KZ_STAT_STRIPES + 1) * KZ_STAT_COUNT;
+
 #define KZ_TRY(x)                          \
     do {                                   \
         hipError_t e_ = (x);               \
@@ -52,7 +55,7 @@ bool Engine::init(int hip_device, std::string &err) {
     hipError_t e;
     if ((e = hipSetDevice(device_)) != hipSuccess || (e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)) ||
         (e = hipEventCreate(&ev_start_)) || (e = hipEventCreate(&ev_end_)) ||
-        (e = hipMalloc((void **)&stats_d_, KZ_STAT_COUNT * sizeof(unsigned long long))) ||
+        (e = hipMalloc((void **)&stats_d_, kStatWords * sizeof(unsigned long long))) ||
         (e = hipHostMalloc((void **)&stats_h_, KZ_STAT_COUNT * sizeof(unsigned long long), hipHostMallocDefault))) {
         err = hipGetErrorString(e);
         return false;
@@ -98,7 +101,7 @@ hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
 hipError_t Engine::begin_submit() {
     KZ_TRY(hipSetDevice(device_));
     if (!submit_open_) {
-        KZ_TRY(kz_launch_fill_u32(stream_, (uint32_t *)stats_d_, KZ_STAT_COUNT * 2, 0u));
+        KZ_TRY(kz_launch_fill_u32(stream_, (uint32_t *)stats_d_, kStatWords * 2, 0u));
         KZ_TRY(hipEventRecord(ev_start_, stream_));
         pending_batches_ = pending_launches_ = 0;
         submit_open_ = true;
@@ -186,7 +189,9 @@ hipError_t Engine::execute(const std::vector<Op> &ops) {
 
 hipError_t Engine::end_submit(hipEvent_t fence_event) {
     KZ_TRY(hipEventRecord(ev_end_, stream_));
-    KZ_TRY(hipMemcpyAsync(stats_h_, stats_d_, KZ_STAT_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    KZ_TRY(kz_launch_stats_reduce(stream_, stats_d_));
+    KZ_TRY(hipMemcpyAsync(stats_h_, stats_d_ + (size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT,
+                          KZ_STAT_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           stream_));
     if (fence_event) KZ_TRY(hipEventRecord(fence_event, stream_));
     return hipSuccess;

@@ -75,18 +75,68 @@ __global__ __launch_bounds__(kScanThreads) void kz_bin_scan(KzFrame f, uint32_t 
     }
 }
 
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Reserves list slots for every lane with a tile this step, one atomic per
+// distinct tile per wave; returns this lane's slot (or ~0u).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *cursor, uint32_t tile, uint32_t lane) {
+    uint64_t pending = __ballot(tile != 0xFFFFFFFFu);
+    uint32_t my_slot = 0xFFFFFFFFu;
+    while (pending) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)tile, leader);
+        const uint64_t peers = __ballot(tile == t) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[t], (uint32_t)__popcll(peers));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if ((peers >> lane) & 1ull)
+            my_slot = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+        pending &= ~peers;
+    }
+    return my_slot;
+}
+
 __global__ __launch_bounds__(256) void kz_bin_scatter(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= f.total_tris) return;
-    const uint64_t r = f.tri_rect[g];
-    if (r == KZ_RECT_EMPTY) return;
-    const uint32_t tx0 = (uint32_t)(r & 0xFFFF), ty0 = (uint32_t)((r >> 16) & 0xFFFF);
-    const uint32_t tx1 = (uint32_t)((r >> 32) & 0xFFFF), ty1 = (uint32_t)(r >> 48);
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-        for (uint32_t tx = tx0; tx <= tx1; ++tx) {
-            const uint32_t slot = atomicAdd(&f.tile_cursor[ty * f.tiles_x + tx], 1u);
+    const uint32_t lane = lane_id();
+    uint64_t r = g < f.total_tris ? f.tri_rect[g] : KZ_RECT_EMPTY;
+    uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;
+    if (r != KZ_RECT_EMPTY) {
+        tx0 = (uint32_t)(r & 0xFFFF);
+        ty0 = (uint32_t)((r >> 16) & 0xFFFF);
+        tx1 = (uint32_t)((r >> 32) & 0xFFFF);
+        ty1 = (uint32_t)(r >> 48);
+    }
+    const uint32_t ntiles = tx0 <= tx1 ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
+    uint32_t steps = ntiles;
+    for (uint32_t d = 32; d > 0; d >>= 1) {
+        const uint32_t o = __shfl_xor(steps, d, 64);
+        steps = steps > o ? steps : o;
+    }
+    uint32_t tx = tx0, ty = ty0;
+    for (uint32_t k = 0; k < steps; ++k) {
+        const bool act = k < ntiles;
+        const uint32_t slot = wave_reserve(f.tile_cursor, act ? ty * f.tiles_x + tx : 0xFFFFFFFFu, lane);
+        if (act) {
             if (slot < f.list_capacity) f.tile_lists[slot] = g;
+            if (++tx > tx1) {
+                tx = tx0;
+                ++ty;
+            }
         }
+    }
+}
+
+// Sums the KZ_STAT_STRIPES partial counters into the totals that follow them.
+__global__ void kz_stats_reduce(unsigned long long *stats) {
+    const uint32_t c = threadIdx.x;
+    if (c >= KZ_STAT_COUNT) return;
+    unsigned long long s = 0;
+    for (uint32_t k = 0; k < KZ_STAT_STRIPES; ++k) s += stats[(size_t)k * KZ_STAT_COUNT + c];
+    stats[(size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT + c] = s;
 }
 
 __device__ void bitonic_lds(uint32_t *s, uint32_t P) {
@@ -220,6 +270,11 @@ hipError_t kz_launch_bin_scan(hipStream_t st, const KzFrame &f, uint32_t ntiles)
 hipError_t kz_launch_bin_scatter(hipStream_t st, const KzFrame &f) {
     if (f.total_tris == 0) return hipSuccess;
     kz_bin_scatter<<<(f.total_tris + 255) / 256, 256, 0, st>>>(f);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_stats_reduce(hipStream_t st, unsigned long long *stats) {
+    kz_stats_reduce<<<1, 64, 0, st>>>(stats);
     return hipGetLastError();
 }
 

@@ -149,14 +149,18 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
     float zf[3], iw[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float xd = p[k][0] / p[k][3], yd = p[k][1] / p[k][3], zd = p[k][2] / p[k][3];
+        // x / 1 == x exactly: skip the IEEE divisions when w is exactly one
+        const bool w1 = p[k][3] == 1.0f;
+        const float xd = w1 ? p[k][0] : p[k][0] / p[k][3];
+        const float yd = w1 ? p[k][1] : p[k][1] / p[k][3];
+        const float zd = w1 ? p[k][2] : p[k][2] / p[k][3];
         const float xf = xd * f.vp_hw + f.vp_ox;
         const float yf = yd * f.vp_hh + f.vp_oy;
         zf[k] = zd * f.vp_dz + f.vp_zmin;
         if (!(xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f)) return 2;
         sx[k] = (int32_t)__builtin_rintf(xf * 16.0f);
         sy[k] = (int32_t)__builtin_rintf(yf * 16.0f);
-        iw[k] = 1.0f / p[k][3];
+        iw[k] = w1 ? 1.0f : 1.0f / p[k][3];
     }
     t.x0 = sx[0];
     t.y0 = sy[0];
@@ -250,10 +254,29 @@ __device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, 
 // ---------------------------------------------------------------------------
 // binning, pass 1
 
+// One atomic per distinct tile per wave: lanes that hit the same tile in
+// this step elect a leader that adds their number. `tile` is ~0u for lanes
+// without a tile this step.
+__device__ __forceinline__ void kz_wave_count_tiles(uint32_t *counts, uint32_t tile, uint32_t lane) {
+    uint64_t pending = kz_ballot(tile != 0xFFFFFFFFu);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t t = (uint32_t)kz_readlane_i((int32_t)tile, leader);
+        const uint64_t peers = kz_ballot(tile == t) & pending;
+        if (lane == leader) atomicAdd(&counts[t], (uint32_t)__popcll(peers));
+        pending &= ~peers;
+    }
+}
+
+__device__ __forceinline__ unsigned long long *kz_stats_stripe(const KzFrame &f) {
+    return f.stats + (size_t)(blockIdx.x & (KZ_STAT_STRIPES - 1)) * KZ_STAT_COUNT;
+}
+
 extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = kz_lane();
     uint32_t n_setup = 0, n_unsup = 0;
+    uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;
     if (g < f.total_tris) {
         float p[3][4], vary[3][KZ_NVA];
         kz_fetch_triangle(f, g, p, vary);
@@ -263,22 +286,36 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
         if (st == 1) {
             n_setup = 1;
             if (t.px0 <= t.px1 && t.py0 <= t.py1) {
-                const uint32_t tx0 = (uint32_t)t.px0 >> KZ_TILE_SHIFT, tx1 = (uint32_t)t.px1 >> KZ_TILE_SHIFT;
-                const uint32_t ty0 = (uint32_t)t.py0 >> KZ_TILE_SHIFT, ty1 = (uint32_t)t.py1 >> KZ_TILE_SHIFT;
+                tx0 = (uint32_t)t.px0 >> KZ_TILE_SHIFT;
+                tx1 = (uint32_t)t.px1 >> KZ_TILE_SHIFT;
+                ty0 = (uint32_t)t.py0 >> KZ_TILE_SHIFT;
+                ty1 = (uint32_t)t.py1 >> KZ_TILE_SHIFT;
                 rect = (uint64_t)tx0 | ((uint64_t)ty0 << 16) | ((uint64_t)tx1 << 32) | ((uint64_t)ty1 << 48);
-                for (uint32_t ty = ty0; ty <= ty1; ++ty)
-                    for (uint32_t tx = tx0; tx <= tx1; ++tx) atomicAdd(&f.tile_counts[ty * f.tiles_x + tx], 1u);
             }
         } else if (st == 2) {
             n_unsup = 1;
         }
         f.tri_rect[g] = rect;
     }
+    // walk every lane's tile rectangle in lock step, row-major
+    const bool has = tx0 <= tx1;
+    const uint32_t ntiles = has ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
+    const int32_t steps = kz_wave_max((int32_t)ntiles, lane);
+    uint32_t tx = tx0, ty = ty0;
+    for (int32_t k = 0; k < steps; ++k) {
+        const bool act = (uint32_t)k < ntiles;
+        kz_wave_count_tiles(f.tile_counts, act ? ty * f.tiles_x + tx : 0xFFFFFFFFu, lane);
+        if (act && ++tx > tx1) {
+            tx = tx0;
+            ++ty;
+        }
+    }
     const uint64_t ms = kz_ballot(n_setup != 0), mu = kz_ballot(n_unsup != 0), mi = kz_ballot(g < f.total_tris);
     if (lane == 0) {
-        if (mi) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
-        if (ms) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
-        if (mu) atomicAdd(&f.stats[KZ_STAT_TRIANGLES_UNSUPPORTED], (unsigned long long)__popcll(mu));
+        unsigned long long *st = kz_stats_stripe(f);
+        if (mi) atomicAdd(&st[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
+        if (ms) atomicAdd(&st[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
+        if (mu) atomicAdd(&st[KZ_STAT_TRIANGLES_UNSUPPORTED], (unsigned long long)__popcll(mu));
     }
 }
 
@@ -740,8 +777,9 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
     nfrag = kz_wave_sum_u64(nfrag, lane);
     npass = kz_wave_sum_u64(npass, lane);
     if (lane == 0) {
-        atomicAdd(&f.stats[KZ_STAT_FRAGMENTS], nfrag);
-        atomicAdd(&f.stats[KZ_STAT_DEPTH_PASSED], npass);
+        unsigned long long *st = kz_stats_stripe(f);
+        atomicAdd(&st[KZ_STAT_FRAGMENTS], nfrag);
+        atomicAdd(&st[KZ_STAT_DEPTH_PASSED], npass);
     }
 }
 
