@@ -55,8 +55,40 @@ def cpu_baseline(scene, fragments, seconds):
             "sample": "full %s frame (%d fragments), %d repetitions, median" % (scene.name, fragments, len(times))}
 
 
+def measure(scene, form, args):
+    """Times K frames of `scene` on a device created in pipeline form `form`
+    (KAZAN_PIPELINE is read at device creation), then one pass with
+    per-kernel HIP events."""
+    from kazan_amd.scenes.renderer import Context, SceneRenderer
+    os.environ["KAZAN_PIPELINE"] = form
+    ctx = Context()
+    r = SceneRenderer(ctx, scene)
+    print("bench[%s]: %s on %s, %d triangles" % (form, scene.name, ctx.device_name, scene.num_triangles),
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        r.render(args.warmup)
+    # timed region: K frames back to back, bracketed by queue waits
+    ctx.wait_idle()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.submit(r.cb, wait=False)
+    ctx.wait_idle()
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    print("bench[%s]: %d frames in %.3f s, gpu %.3f ms total" % (form, args.steps, t1 - t0, st["gpu_ms"]),
+          file=sys.stderr, flush=True)
+    ctx.set_kernel_timing(True)
+    n_prof = max(3, min(args.steps, 10))
+    r.render(n_prof)
+    ktimes = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-plain", action="store_true", help="measure only the balanced pipeline form")
+    ap.add_argument("--plain-only", action="store_true", help="measure only the plain pipeline form")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -70,35 +102,17 @@ def main():
         return 2
 
     from kazan_amd.scenes import generators as g
-    from kazan_amd.scenes.renderer import Context, SceneRenderer
 
     scene = g.WORKLOADS[args.workload]()
-    ctx = Context()
-    r = SceneRenderer(ctx, scene)
-    print("bench: %s on %s, %d triangles" % (scene.name, ctx.device_name, scene.num_triangles), file=sys.stderr,
-          flush=True)
-    if args.warmup:
-        r.render(args.warmup)
-
-    # timed region: K frames back to back
-    ctx.wait_idle()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.submit(r.cb, wait=False)
-    ctx.wait_idle()
-    t1 = time.perf_counter()
-    st = ctx.stats()
-    wall = t1 - t0
+    forms = {}
+    for form in (["plain"] if args.plain_only else []) + ([] if args.plain_only else ["balanced", "plain"]):
+        if form == "plain" and args.skip_plain:
+            continue
+        forms[form] = measure(scene, form, args)
+    main_form = "plain" if args.plain_only else "balanced"
+    m = forms[main_form]
+    st, wall, ktimes, n_prof = m["stats"], m["wall"], m["ktimes"], m["n_prof"]
     frags_per_frame = st["fragments"] // args.steps
-    print("bench: %d frames in %.3f s, gpu %.3f ms total" % (args.steps, wall, st["gpu_ms"]), file=sys.stderr,
-          flush=True)
-
-    # per-kernel times (HIP events on the queue's stream), separate pass
-    ctx.set_kernel_timing(True)
-    n_prof = max(5, min(args.steps, 20))
-    r.render(n_prof)
-    ktimes = ctx.kernel_times()
-    ctx.set_kernel_timing(False)
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     dom_ms, dom_n = ktimes[dom]
     avg_launch_s = dom_ms / max(dom_n, 1) / 1e3
@@ -133,6 +147,12 @@ def main():
             "gpu_ms_per_frame": st["gpu_ms"] / args.steps,
             "batches_per_frame": st["batches"] / args.steps,
             "kernel_ms_per_frame": {k: v[0] / n_prof for k, v in ktimes.items()},
+            "pipeline": main_form,
+            "forms": {f: {"fragments_per_s": v["stats"]["fragments"] / v["wall"],
+                          "ms_per_frame": v["wall"] * 1e3 / args.steps,
+                          "gpu_ms_per_frame": v["stats"]["gpu_ms"] / args.steps,
+                          "launches_per_frame": v["stats"]["launches"] / args.steps}
+                      for f, v in forms.items()},
         },
         "roofline": {
             "kernel": "kz_raster" if dom == "raster" else "kz_" + dom,
@@ -147,6 +167,9 @@ def main():
             "avg_launch_ms": avg_launch_s * 1e3,
         },
     }
+    if "plain" in forms and "balanced" in forms:
+        out["config"]["balanced_vs_plain"] = (forms["balanced"]["stats"]["fragments"] / forms["balanced"]["wall"]) / (
+            forms["plain"]["stats"]["fragments"] / forms["plain"]["wall"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, frags_per_frame, args.cpu_seconds)
     print(json.dumps(out), flush=True)

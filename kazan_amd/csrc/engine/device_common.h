@@ -58,6 +58,8 @@ struct KzFrame {
     float *depth;
     uint32_t width, height;
     uint32_t tiles_x, tiles_y;
+    uint32_t tile_shift;
+    uint32_t pad1;
 
     // viewport transform constants, precomputed on the host in the same
     // order the oracle uses: hw = w*0.5f, ox = x + hw, ...
@@ -77,8 +79,25 @@ struct KzFrame {
     uint32_t *sort_tmp;      // [list_capacity] merge scratch for long lists
     uint32_t list_capacity;
     uint32_t pad0;
-    uint64_t *tri_rect;      // [total_tris] packed tile rectangle or ~0
+    uint64_t *tri_rect;      // [total_tris] packed pixel bounding box or ~0
     unsigned long long *stats;  // KZ_STAT_* counters
+
+    // Work items (load-balanced mode). A work item is a square sub-tile of
+    // a screen tile: level L splits the tile into 4^L squares of side
+    // TS >> L. Item geometry is packed as tile << 6 | L << 4 | sub.
+    // In the static mode every tile is one item and item i is tile i.
+    uint32_t num_items;
+    uint32_t balanced;        // 1: persistent raster over `queue`
+    uint32_t *tile_level;     // [tiles] split level
+    uint32_t *tile_item_base; // [tiles] first item of the tile
+    uint32_t *items;          // [num_items] packed geometry
+    uint32_t *item_counts;    // [num_items]
+    uint32_t *item_offsets;   // [num_items]
+    uint32_t *item_cursor;    // [num_items]
+    uint32_t *queue;          // [num_items] item ids, most expensive first
+    uint32_t *queue_head;     // persistent raster work counter
+    float *tile_cost;         // [tiles] predicted cost (triangle-equivalents)
+    uint32_t *tile_frags;     // [tiles] measured fragments (training labels) or null
 };
 
 enum {
@@ -97,5 +116,17 @@ enum {
 // workgroup's atomics; kz_stats_reduce folds them at the end of a submission.
 #define KZ_STAT_STRIPES 256
 
-// Tile rectangle packing: tx0 | ty0 << 16 | tx1 << 32 | ty1 << 48.
+// Per-triangle pixel bounding box (inclusive, clipped to the scissor),
+// packed as px0 | py0 << 16 | px1 << 32 | py1 << 48; ~0 = no pixels.
 #define KZ_RECT_EMPTY (~0ULL)
+
+#define KZ_ITEM_PACK(tile, level, sub) (((tile) << 6) | ((level) << 4) | (sub))
+#define KZ_ITEM_TILE(it) ((it) >> 6)
+#define KZ_ITEM_LEVEL(it) (((it) >> 4) & 3u)
+#define KZ_ITEM_SUB(it) ((it) & 15u)
+
+// Split thresholds of the balanced mode, in predicted cost units
+// (triangle-equivalents of raster work): a tile becomes 4 items above
+// KZ_SPLIT_4 and 16 items above KZ_SPLIT_16.
+#define KZ_SPLIT_4 1536.0f
+#define KZ_SPLIT_16 6144.0f

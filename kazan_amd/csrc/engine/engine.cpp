@@ -42,6 +42,14 @@ Engine::~Engine() {
     hipFree(tile_lists_);
     hipFree(sort_tmp_);
     hipFree(tri_rect_);
+    hipFree(tile_level_);
+    hipFree(tile_item_base_);
+    hipFree(items_);
+    hipFree(item_counts_);
+    hipFree(item_offsets_);
+    hipFree(item_cursor_);
+    hipFree(queue_);
+    hipFree(ctrl_);
     hipFree(stats_d_);
     if (stats_h_) hipHostFree(stats_h_);
     for (hipEvent_t e : event_pool_) hipEventDestroy(e);
@@ -64,7 +72,25 @@ bool Engine::init(int hip_device, std::string &err) {
     // KAZAN_LIST_CAPACITY_LIMIT caps the per-tile list storage (entries);
     // tiles whose lists do not fit render through the in-order scan path
     if (const char *e = std::getenv("KAZAN_LIST_CAPACITY_LIMIT")) list_limit_ = std::strtoull(e, nullptr, 10);
+    if (const char *e = std::getenv("KAZAN_PIPELINE")) balanced_ = std::strcmp(e, "plain") != 0;
     return true;
+}
+
+hipError_t Engine::ensure_item_scratch(uint32_t tiles) {
+    if (tiles <= cap_items_tiles_) return hipSuccess;
+    KZ_TRY(hipStreamSynchronize(stream_));
+    const uint32_t t = std::max(tiles, cap_items_tiles_ * 2);
+    const uint64_t items = 16ull * t;
+    KZ_TRY(grow(tile_level_, t));
+    KZ_TRY(grow(tile_item_base_, t));
+    KZ_TRY(grow(items_, items));
+    KZ_TRY(grow(item_counts_, items));
+    KZ_TRY(grow(item_offsets_, items));
+    KZ_TRY(grow(item_cursor_, items));
+    KZ_TRY(grow(queue_, items));
+    if (!ctrl_) KZ_TRY(hipMalloc((void **)&ctrl_, 128 * sizeof(uint32_t)));
+    cap_items_tiles_ = t;
+    return hipSuccess;
 }
 
 hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
@@ -123,28 +149,76 @@ hipError_t Engine::run_batch(const Op &op) {
     f.list_capacity = list_capacity_;
     f.tri_rect = tri_rect_;
     f.stats = stats_d_;
+    f.tile_shift = op.tile_shift;
     void *args[] = {&f};
+    const uint32_t tri_blocks = (f.total_tris + 255) / 256;
+
+    if (!balanced_) {
+        KZ_TRY(mark(KS_CLEAR_COUNTS, false));
+        KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
+        KZ_TRY(mark(KS_CLEAR_COUNTS, true));
+        KZ_TRY(mark(KS_BIN_COUNT, false));
+        KZ_TRY(hipModuleLaunchKernel(op.kernels->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+        KZ_TRY(mark(KS_BIN_COUNT, true));
+        KZ_TRY(mark(KS_BIN_SCAN, false));
+        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, nullptr, list_capacity_,
+                              stats_d_));
+        KZ_TRY(mark(KS_BIN_SCAN, true));
+        KZ_TRY(mark(KS_BIN_SCATTER, false));
+        KZ_TRY(kz_launch_bin_scatter(stream_, f));
+        KZ_TRY(mark(KS_BIN_SCATTER, true));
+        KZ_TRY(mark(KS_BIN_SORT, false));
+        KZ_TRY(kz_launch_sort_lists(stream_, tile_counts_, tile_offsets_, ntiles, nullptr, tile_lists_, sort_tmp_,
+                                    list_capacity_));
+        KZ_TRY(mark(KS_BIN_SORT, true));
+        KZ_TRY(mark(KS_RASTER, false));
+        KZ_TRY(hipModuleLaunchKernel(op.kernels->raster, ntiles, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
+        KZ_TRY(mark(KS_RASTER, true));
+        pending_batches_ += 1;
+        pending_launches_ += 6;
+        return hipSuccess;
+    }
+
+    // balanced: tiles -> work items by predicted cost, per-item lists, queue
+    KZ_TRY(ensure_item_scratch(ntiles));
+    const uint32_t max_items = 16 * ntiles;
+    f.balanced = 1;
+    f.tile_level = tile_level_;
+    f.tile_item_base = tile_item_base_;
+    f.items = items_;
+    f.item_counts = item_counts_;
+    f.item_offsets = item_offsets_;
+    f.item_cursor = item_cursor_;
+    f.queue = queue_;
+    f.queue_head = ctrl_;
     KZ_TRY(mark(KS_CLEAR_COUNTS, false));
     KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
+    KZ_TRY(kz_launch_fill_u32(stream_, ctrl_, 128, 0u));
     KZ_TRY(mark(KS_CLEAR_COUNTS, true));
     KZ_TRY(mark(KS_BIN_COUNT, false));
-    KZ_TRY(hipModuleLaunchKernel(op.kernels->bin_count, (f.total_tris + 255) / 256, 1, 1, 256, 1, 1, 0, stream_, args,
-                                 nullptr));
+    KZ_TRY(hipModuleLaunchKernel(op.kernels->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
     KZ_TRY(mark(KS_BIN_COUNT, true));
     KZ_TRY(mark(KS_BIN_SCAN, false));
-    KZ_TRY(kz_launch_bin_scan(stream_, f, ntiles));
+    KZ_TRY(kz_launch_plan(stream_, f, ntiles, ctrl_));
+    KZ_TRY(kz_launch_plan_emit(stream_, f, ntiles));
+    KZ_TRY(kz_launch_item_count(stream_, f));
+    KZ_TRY(kz_launch_scan(stream_, item_counts_, item_offsets_, item_cursor_, max_items, ctrl_ + 1, list_capacity_,
+                          stats_d_));
     KZ_TRY(mark(KS_BIN_SCAN, true));
     KZ_TRY(mark(KS_BIN_SCATTER, false));
-    KZ_TRY(kz_launch_bin_scatter(stream_, f));
+    KZ_TRY(kz_launch_item_scatter(stream_, f));
     KZ_TRY(mark(KS_BIN_SCATTER, true));
     KZ_TRY(mark(KS_BIN_SORT, false));
-    KZ_TRY(kz_launch_bin_sort(stream_, f, ntiles));
+    KZ_TRY(kz_launch_sort_lists(stream_, item_counts_, item_offsets_, max_items, ctrl_ + 1, tile_lists_, sort_tmp_,
+                                list_capacity_));
+    KZ_TRY(kz_launch_queue(stream_, f, ctrl_, max_items));
     KZ_TRY(mark(KS_BIN_SORT, true));
     KZ_TRY(mark(KS_RASTER, false));
-    KZ_TRY(hipModuleLaunchKernel(op.kernels->raster, ntiles, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
+    KZ_TRY(hipModuleLaunchKernel(op.kernels->raster_persistent, std::min(op.kernels->persistent_blocks, max_items), 1,
+                                 1, 64, 1, 1, 0, stream_, args, nullptr));
     KZ_TRY(mark(KS_RASTER, true));
     pending_batches_ += 1;
-    pending_launches_ += 6;
+    pending_launches_ += 14;
     return hipSuccess;
 }
 

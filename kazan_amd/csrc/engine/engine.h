@@ -35,7 +35,9 @@ struct FixedState {
 struct PipelineKernels {
     hipModule_t module = nullptr;
     hipFunction_t bin_count = nullptr;
-    hipFunction_t raster = nullptr;
+    hipFunction_t raster = nullptr;             // one workgroup per tile
+    hipFunction_t raster_persistent = nullptr;  // pops the work-item queue
+    uint32_t persistent_blocks = 0;             // resident workgroups that fill the device
 };
 
 // Builds the full kernel source (prelude + shared structs + generated shader
@@ -100,6 +102,15 @@ class Engine {
 
     uint32_t list_capacity() const { return list_capacity_; }
 
+    // Pipeline form, fixed at device creation from KAZAN_PIPELINE:
+    //   "plain"    every vkCmdDraw* is its own batch, one raster workgroup
+    //              per 16x16 tile in tile order;
+    //   "balanced" (default) consecutive compatible draws share one batch,
+    //              tiles are split into work items by predicted cost and a
+    //              persistent raster kernel takes the most expensive first.
+    bool balanced() const { return balanced_; }
+    bool merge_draws() const { return balanced_; }
+
     // Per-kernel timing: an event pair around every batch kernel on the
     // engine's stream. Off by default (it adds a little latency per launch).
     void set_kernel_timing(bool on) { kernel_timing_ = on; }
@@ -110,7 +121,13 @@ class Engine {
     hipError_t mark(int slot, bool end);
 
     bool kernel_timing_ = false;
+    bool balanced_ = true;
     uint64_t list_limit_ = 0;
+    // balanced-mode scratch, sized for 16 work items per tile
+    uint32_t *tile_level_ = nullptr, *tile_item_base_ = nullptr, *items_ = nullptr, *item_counts_ = nullptr,
+             *item_offsets_ = nullptr, *item_cursor_ = nullptr, *queue_ = nullptr, *ctrl_ = nullptr;
+    uint32_t cap_items_tiles_ = 0;
+    hipError_t ensure_item_scratch(uint32_t tiles);
     std::vector<std::pair<int, hipEvent_t>> timing_events_;  // (slot, start) then end, alternating
     std::vector<hipEvent_t> event_pool_;
     size_t events_used_ = 0;

@@ -12,6 +12,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -151,10 +152,19 @@ bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, 
         return false;
     }
     if ((e = hipModuleGetFunction(&out.bin_count, out.module, "kz_bin_count")) != hipSuccess ||
-        (e = hipModuleGetFunction(&out.raster, out.module, "kz_raster")) != hipSuccess) {
+        (e = hipModuleGetFunction(&out.raster, out.module, "kz_raster")) != hipSuccess ||
+        (e = hipModuleGetFunction(&out.raster_persistent, out.module, "kz_raster_persistent")) != hipSuccess) {
         err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
         return false;
     }
+    // persistent grid: as many workgroups as can be resident at once
+    int per_cu = 0, dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, out.raster_persistent, 64, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 8;
+    out.persistent_blocks = (uint32_t)(std::max(cus, 1) * per_cu);
     return true;
 }
 

@@ -290,7 +290,8 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
                 tx1 = (uint32_t)t.px1 >> KZ_TILE_SHIFT;
                 ty0 = (uint32_t)t.py0 >> KZ_TILE_SHIFT;
                 ty1 = (uint32_t)t.py1 >> KZ_TILE_SHIFT;
-                rect = (uint64_t)tx0 | ((uint64_t)ty0 << 16) | ((uint64_t)tx1 << 32) | ((uint64_t)ty1 << 48);
+                rect = (uint64_t)(uint32_t)t.px0 | ((uint64_t)(uint32_t)t.py0 << 16) |
+                       ((uint64_t)(uint32_t)t.px1 << 32) | ((uint64_t)(uint32_t)t.py1 << 48);
             }
         } else if (st == 2) {
             n_unsup = 1;
@@ -453,28 +454,33 @@ struct KzLaneTri {
     bool live;
 };
 
-__device__ __forceinline__ bool kz_rect_has_tile(uint64_t r, uint32_t tx, uint32_t ty) {
+// Does pixel box r overlap the square [X0, X0 + RW) x [Y0, Y0 + RW)?
+__device__ __forceinline__ bool kz_box_overlaps(uint64_t r, int32_t X0, int32_t Y0, int32_t RW) {
     if (r == KZ_RECT_EMPTY) return false;
-    const uint32_t tx0 = (uint32_t)(r & 0xFFFF), ty0 = (uint32_t)((r >> 16) & 0xFFFF);
-    const uint32_t tx1 = (uint32_t)((r >> 32) & 0xFFFF), ty1 = (uint32_t)(r >> 48);
-    return tx >= tx0 && tx <= tx1 && ty >= ty0 && ty <= ty1;
+    const int32_t x0 = (int32_t)(r & 0xFFFF), y0 = (int32_t)((r >> 16) & 0xFFFF);
+    const int32_t x1 = (int32_t)((r >> 32) & 0xFFFF), y1 = (int32_t)(r >> 48);
+    return x1 >= X0 && x0 < X0 + RW && y1 >= Y0 && y0 < Y0 + RW;
 }
 
 __device__ __forceinline__ uint32_t kz_pack_frag(uint32_t pix, uint32_t seq, uint32_t mask) {
     return (pix << 10) | (seq << 4) | mask;
 }
 
-__device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_color, float *s_depth,
-                               uint32_t *s_claim, uint32_t *s_frag) {
+// Rasterises every triangle of one list into the square of side 1 << RS at
+// (X0, Y0) (a whole tile, or a sub-tile of the balanced mode). `list` holds
+// the n triangle ids overlapping the square, in primitive order; when it
+// overflowed, the triangles come from an in-order scan of all pixel boxes.
+__device__ void kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_t RS, uint32_t list_base,
+                               uint32_t n, uint32_t stats_tile, uint32_t *s_color, float *s_depth, uint32_t *s_claim,
+                               uint32_t *s_frag) {
     const uint32_t lane = kz_lane();
-    const uint32_t n = f.tile_counts[tile];
     if (n == 0) return;
-    const uint32_t tx = tile % f.tiles_x, ty = tile / f.tiles_x;
-    const int32_t X0 = (int32_t)(tx << KZ_TILE_SHIFT), Y0 = (int32_t)(ty << KZ_TILE_SHIFT);
+    const int32_t RW = 1 << RS;
     const uint32_t *gcolor = (const uint32_t *)f.color;
 
-    for (uint32_t i = lane; i < KZ_TP; i += 64) {
-        const int32_t x = X0 + (int32_t)(i & (KZ_TS - 1)), y = Y0 + (int32_t)(i >> KZ_TILE_SHIFT);
+    for (uint32_t j = lane; j < (uint32_t)(RW * RW); j += 64) {
+        const int32_t x = X0 + (int32_t)(j & (RW - 1)), y = Y0 + (int32_t)(j >> RS);
+        const uint32_t i = (uint32_t)((y - Y0) * KZ_TS + (x - X0));
         if (x < (int32_t)f.width && y < (int32_t)f.height) {
             const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
 #pragma unroll
@@ -487,9 +493,9 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
     }
     __syncthreads();
 
-    const uint32_t base = f.tile_offsets[tile];
-    // Entries past list_capacity were dropped by kz_bin_scatter; such a tile
-    // takes its triangles from an in-order scan of all tile rectangles.
+    const uint32_t base = list_base;
+    // Entries past list_capacity were dropped by the scatter kernels; such a
+    // list takes its triangles from an in-order scan of all pixel boxes.
     const bool overflow = (uint64_t)base + n > (uint64_t)f.list_capacity;
     unsigned long long nfrag = 0, npass = 0;
     uint32_t k = 0;
@@ -504,7 +510,7 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
         } else {
             if (k >= f.total_tris) break;
             g = k + lane;
-            valid = g < f.total_tris && kz_rect_has_tile(f.tri_rect[g], tx, ty);
+            valid = g < f.total_tris && kz_box_overlaps(f.tri_rect[g], X0, Y0, RW);
             if (kz_ballot(valid) == 0) {
                 k += 64;
                 continue;
@@ -529,9 +535,9 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
             kz_fetch_triangle(f, g, p, vary);
             if (kz_setup(f, p, L.t) == 1) {
                 L.cx0 = max(L.t.px0, X0);
-                L.cx1 = min(L.t.px1, X0 + KZ_TS - 1);
+                L.cx1 = min(L.t.px1, X0 + RW - 1);
                 L.cy0 = max(L.t.py0, Y0);
-                L.cy1 = min(L.t.py1, Y0 + KZ_TS - 1);
+                L.cy1 = min(L.t.py1, Y0 + RW - 1);
                 L.live = L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
                 L.c = kz_cov_setup(L.t);
 #pragma unroll
@@ -763,8 +769,9 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
         __syncthreads();
     }
 
-    for (uint32_t i = lane; i < KZ_TP; i += 64) {
-        const int32_t x = X0 + (int32_t)(i & (KZ_TS - 1)), y = Y0 + (int32_t)(i >> KZ_TILE_SHIFT);
+    for (uint32_t j = lane; j < (uint32_t)(RW * RW); j += 64) {
+        const int32_t x = X0 + (int32_t)(j & (RW - 1)), y = Y0 + (int32_t)(j >> RS);
+        const uint32_t i = (uint32_t)((y - Y0) * KZ_TS + (x - X0));
         if (x < (int32_t)f.width && y < (int32_t)f.height) {
             const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
 #pragma unroll
@@ -780,13 +787,45 @@ __device__ void kz_raster_tile(const KzFrame &f, uint32_t tile, uint32_t *s_colo
         unsigned long long *st = kz_stats_stripe(f);
         atomicAdd(&st[KZ_STAT_FRAGMENTS], nfrag);
         atomicAdd(&st[KZ_STAT_DEPTH_PASSED], npass);
+        if (f.tile_frags) atomicAdd(&f.tile_frags[stats_tile], (uint32_t)nfrag);
     }
+    __syncthreads();
 }
 
+// Static mode: one workgroup per screen tile, in tile order.
 extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];
     __shared__ uint32_t s_frag[KZ_FRAG_CAP];
-    kz_raster_tile(f, blockIdx.x, s_color, s_depth, s_claim, s_frag);
+    const uint32_t tile = blockIdx.x;
+    const int32_t X0 = (int32_t)((tile % f.tiles_x) << KZ_TILE_SHIFT);
+    const int32_t Y0 = (int32_t)((tile / f.tiles_x) << KZ_TILE_SHIFT);
+    kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, f.tile_offsets[tile], f.tile_counts[tile], tile, s_color, s_depth,
+                   s_claim, s_frag);
+}
+
+// Balanced mode: persistent workgroups pop work items (tiles or sub-tiles)
+// from the queue, most expensive first.
+extern "C" __global__ __launch_bounds__(64) void kz_raster_persistent(KzFrame f) {
+    __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
+    __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
+    __shared__ uint32_t s_claim[KZ_TP];
+    __shared__ uint32_t s_frag[KZ_FRAG_CAP];
+    const uint32_t lane = kz_lane();
+    const uint32_t n_items = f.queue_head[1];
+    while (true) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(&f.queue_head[0], 1u);
+        q = (uint32_t)kz_readlane_i((int32_t)q, 0);
+        if (q >= n_items) break;
+        const uint32_t item = f.queue[q];
+        const uint32_t geo = f.items[item];
+        const uint32_t tile = KZ_ITEM_TILE(geo), level = KZ_ITEM_LEVEL(geo), sub = KZ_ITEM_SUB(geo);
+        const uint32_t RS = KZ_TILE_SHIFT - level;
+        const int32_t X0 = (int32_t)(((tile % f.tiles_x) << KZ_TILE_SHIFT) + ((sub & ((1u << level) - 1)) << RS));
+        const int32_t Y0 = (int32_t)(((tile / f.tiles_x) << KZ_TILE_SHIFT) + ((sub >> level) << RS));
+        kz_raster_rect(f, X0, Y0, RS, f.item_offsets[item], f.item_counts[item], tile, s_color, s_depth, s_claim,
+                       s_frag);
+    }
 }

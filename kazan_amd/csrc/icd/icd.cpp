@@ -1038,7 +1038,7 @@ static void record_draw(CommandBuffer *cb, uint32_t count, uint32_t instances, u
     }
 
     Op *batch = nullptr;
-    if (st.batch_open && !cb->ops.empty() && cb->ops.back().kind == Op::Batch) {
+    if (st.batch_open && cb->device->engine->merge_draws() && !cb->ops.empty() && cb->ops.back().kind == Op::Batch) {
         Op &last = cb->ops.back();
         if (last.kernels == &p->kernels && std::memcmp(&last.frame.vp_hw, &f.vp_hw, sizeof(float) * 8) == 0 &&
             last.frame.sc_x0 == f.sc_x0 && last.frame.sc_y0 == f.sc_y0 && last.frame.sc_x1 == f.sc_x1 &&
