@@ -55,13 +55,30 @@ def cpu_baseline(scene, fragments, seconds):
             "sample": "full %s frame (%d fragments), %d repetitions, median" % (scene.name, fragments, len(times))}
 
 
+FORMS = {
+    # name: (KAZAN_PIPELINE, KAZAN_COST_MODEL or None for the default model)
+    "learned": ("balanced", None),
+    "balanced_static": ("balanced", "none"),
+    "plain": ("plain", "none"),
+}
+
+
 def measure(scene, form, args):
     """Times K frames of `scene` on a device created in pipeline form `form`
-    (KAZAN_PIPELINE is read at device creation), then one pass with
+    (the environment is read at device creation), then one pass with
     per-kernel HIP events."""
     from kazan_amd.scenes.renderer import Context, SceneRenderer
-    os.environ["KAZAN_PIPELINE"] = form
+    pipeline, model = FORMS[form]
+    os.environ["KAZAN_PIPELINE"] = pipeline
+    if model is None:
+        os.environ.pop("KAZAN_COST_MODEL", None)
+    else:
+        os.environ["KAZAN_COST_MODEL"] = model
     ctx = Context()
+    has_model = bool(ctx.d.vkHasCostModelKAZAN(ctx.device))
+    if form == "learned" and not has_model:
+        print("bench: no cost model (kazan_amd/cost_model.bin); 'learned' runs the static split rule",
+              file=sys.stderr, flush=True)
     r = SceneRenderer(ctx, scene)
     print("bench[%s]: %s on %s, %d triangles" % (form, scene.name, ctx.device_name, scene.num_triangles),
           file=sys.stderr, flush=True)
@@ -82,13 +99,14 @@ def measure(scene, form, args):
     r.render(n_prof)
     ktimes = ctx.kernel_times()
     ctx.set_kernel_timing(False)
-    return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r}
+    return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r,
+            "cost_model": has_model}
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--skip-plain", action="store_true", help="measure only the balanced pipeline form")
-    ap.add_argument("--plain-only", action="store_true", help="measure only the plain pipeline form")
+    ap.add_argument("--forms", default="learned,balanced_static,plain",
+                    help="pipeline forms to measure, first = the reported one (%s)" % ",".join(FORMS))
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -104,12 +122,15 @@ def main():
     from kazan_amd.scenes import generators as g
 
     scene = g.WORKLOADS[args.workload]()
+    order = [f for f in args.forms.split(",") if f]
+    for f in order:
+        if f not in FORMS:
+            print("unknown form %s (known: %s)" % (f, ", ".join(FORMS)), file=sys.stderr)
+            return 2
     forms = {}
-    for form in (["plain"] if args.plain_only else []) + ([] if args.plain_only else ["balanced", "plain"]):
-        if form == "plain" and args.skip_plain:
-            continue
+    for form in order:
         forms[form] = measure(scene, form, args)
-    main_form = "plain" if args.plain_only else "balanced"
+    main_form = order[0]
     m = forms[main_form]
     st, wall, ktimes, n_prof = m["stats"], m["wall"], m["ktimes"], m["n_prof"]
     frags_per_frame = st["fragments"] // args.steps
@@ -151,7 +172,9 @@ def main():
             "forms": {f: {"fragments_per_s": v["stats"]["fragments"] / v["wall"],
                           "ms_per_frame": v["wall"] * 1e3 / args.steps,
                           "gpu_ms_per_frame": v["stats"]["gpu_ms"] / args.steps,
-                          "launches_per_frame": v["stats"]["launches"] / args.steps}
+                          "launches_per_frame": v["stats"]["launches"] / args.steps,
+                          "split_entries_per_frame": v["stats"]["split_entries"] / args.steps,
+                          "cost_model": v["cost_model"]}
                       for f, v in forms.items()},
         },
         "roofline": {
@@ -167,9 +190,11 @@ def main():
             "avg_launch_ms": avg_launch_s * 1e3,
         },
     }
-    if "plain" in forms and "balanced" in forms:
-        out["config"]["balanced_vs_plain"] = (forms["balanced"]["stats"]["fragments"] / forms["balanced"]["wall"]) / (
-            forms["plain"]["stats"]["fragments"] / forms["plain"]["wall"])
+    if "plain" in forms:
+        rate = lambda f: forms[f]["stats"]["fragments"] / forms[f]["wall"]
+        for f in forms:
+            if f != "plain":
+                out["config"]["%s_vs_plain" % f] = rate(f) / rate("plain")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, frags_per_frame, args.cpu_seconds)
     print(json.dumps(out), flush=True)

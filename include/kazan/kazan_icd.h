@@ -54,9 +54,17 @@ extern "C" {
 
 #define KAZAN_ICD_VERSION 5u
 
-VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetInstanceProcAddr(VkInstance instance, const char *pName);
-VKAPI_ATTR VkResult VKAPI_CALL vk_icdNegotiateLoaderICDInterfaceVersion(uint32_t *pSupportedVersion);
-VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetPhysicalDeviceProcAddr(VkInstance instance, const char *pName);
+/* The library is built with hidden visibility: these three (and the tools
+ * of kazan_tools.h) are its only exported C names. */
+#ifndef KAZAN_EXPORT
+#define KAZAN_EXPORT __attribute__((visibility("default")))
+#endif
+
+KAZAN_EXPORT VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetInstanceProcAddr(VkInstance instance,
+                                                                                const char *pName);
+KAZAN_EXPORT VKAPI_ATTR VkResult VKAPI_CALL vk_icdNegotiateLoaderICDInterfaceVersion(uint32_t *pSupportedVersion);
+KAZAN_EXPORT VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetPhysicalDeviceProcAddr(VkInstance instance,
+                                                                                      const char *pName);
 
 /* Counters of vkGetDrawStatsKAZAN, in order. */
 enum KazanDrawStat {
@@ -67,6 +75,7 @@ enum KazanDrawStat {
     KAZAN_STAT_DEPTH_PASSED = 4,          /* samples that passed the depth test */
     KAZAN_STAT_LIST_ENTRIES = 5,          /* binned (triangle, tile) pairs */
     KAZAN_STAT_LIST_OVERFLOW = 6,         /* batches whose tile lists overflowed */
+    KAZAN_STAT_SPLIT_ENTRIES = 7,         /* entries of split tiles' sub-tile lists */
     KAZAN_STAT_COUNT = 8
 };
 
@@ -79,13 +88,15 @@ typedef VkResult(VKAPI_PTR *PFN_vkGetDrawStatsKAZAN)(VkDevice device, uint64_t *
 
 /* Kernels of one draw batch, in launch order, for per-kernel timing. */
 enum KazanKernel {
-    KAZAN_KERNEL_CLEAR_COUNTS = 0,
-    KAZAN_KERNEL_BIN_COUNT = 1,
-    KAZAN_KERNEL_BIN_SCAN = 2,
-    KAZAN_KERNEL_BIN_SCATTER = 3,
-    KAZAN_KERNEL_BIN_SORT = 4,
-    KAZAN_KERNEL_RASTER = 5,
-    KAZAN_KERNEL_COUNT = 6
+    KAZAN_KERNEL_CLEAR_COUNTS = 0, /* counter resets */
+    KAZAN_KERNEL_BIN_COUNT = 1,    /* vertex shader + setup: pixel boxes */
+    KAZAN_KERNEL_BIN_HIST = 2,     /* (triangle, tile) pairs per tile */
+    KAZAN_KERNEL_BIN_SCAN = 3,     /* list offsets */
+    KAZAN_KERNEL_BIN_SCATTER = 4,  /* tile lists */
+    KAZAN_KERNEL_BIN_SORT = 5,     /* lists into primitive order */
+    KAZAN_KERNEL_PLAN = 6,         /* cost model, split, sub-tile lists, queue */
+    KAZAN_KERNEL_RASTER = 7,
+    KAZAN_KERNEL_COUNT = 8
 };
 
 /* vkSetKernelTimingKAZAN: records a HIP event pair around every batch kernel
@@ -95,6 +106,56 @@ enum KazanKernel {
  * statistics read. Neither has a counterpart in the reference. */
 typedef void(VKAPI_PTR *PFN_vkSetKernelTimingKAZAN)(VkDevice device, VkBool32 enable);
 typedef VkResult(VKAPI_PTR *PFN_vkGetKernelTimesKAZAN)(VkDevice device, float *pMs, uint32_t *pLaunches);
+
+/* The learned tile-cost model of the balanced pipeline form (no counterpart
+ * in the reference, which has no draw path to balance).
+ *
+ * vkSetTileProfileKAZAN: from the next submission on, every batch also
+ * computes the 16 per-tile cost-model features (and, with a model, the
+ * predicted fragments), and the plain form records each tile's fragment
+ * count (the model's training label) and its raster workgroup's
+ * shader-clock cycles. vkGetTileProfileKAZAN waits for the queue and copies
+ * the last batch's tile count, features [tiles * 16], fragments, cycles and
+ * predicted fragments [tiles each] (zeros where the batch produced none);
+ * null arrays are skipped.
+ *
+ * vkSetCostModelKAZAN replaces the model with a KzCostModel blob (the
+ * format of cost_model.bin: magic "KZCM", version 1, size-checked);
+ * VK_ERROR_INITIALIZATION_FAILED leaves the current one in place.
+ * vkHasCostModelKAZAN: whether the balanced form predicts costs (VK_FALSE:
+ * it splits tiles by binned triangle count). */
+typedef void(VKAPI_PTR *PFN_vkSetTileProfileKAZAN)(VkDevice device, VkBool32 enable);
+typedef VkResult(VKAPI_PTR *PFN_vkGetTileProfileKAZAN)(VkDevice device, uint32_t *pTileCount, float *pFeatures,
+                                                        float *pFragments, float *pCycles, float *pPredicted);
+typedef VkResult(VKAPI_PTR *PFN_vkSetCostModelKAZAN)(VkDevice device, const void *pBlob, size_t size);
+typedef VkBool32(VKAPI_PTR *PFN_vkHasCostModelKAZAN)(VkDevice device);
+
+/* vkFillClusterSceneKAZAN: the cost model's training-data generator. Writes
+ * the Gaussian-cluster scene of `seed` (round(triangles * cluster_frac)
+ * triangles around the target's centre with sigma = sigma_frac * (width,
+ * height) pixels, the rest uniform; vertices at offsets U[-radius,
+ * radius]^2 px; z and RGBA uniform [0, 1)) into `buffer` at `offset` as
+ * 3 * triangles vertices of vec4 position (NDC, w = 1) + vec4 colour, on the
+ * device, and waits for it. It follows the host generator
+ * (kazan_amd/scenes/generators.py gaussian_cluster) in double precision, so
+ * a fresh frame costs no host work or upload. */
+typedef struct KazanClusterScene {
+    uint64_t seed;
+    uint32_t triangles, width, height, pad;
+    double cluster_frac, sigma_frac, radius;
+} KazanClusterScene;
+typedef VkResult(VKAPI_PTR *PFN_vkFillClusterSceneKAZAN)(VkDevice device, VkBuffer buffer, VkDeviceSize offset,
+                                                         const KazanClusterScene *pScene);
+
+/* Batched launch. In the balanced form a command buffer's batches are
+ * autotuned over their first submissions (tile size 8..64 and persistent
+ * raster grid, one timed candidate per submission; KAZAN_AUTOTUNE=0 turns
+ * this off), after which the whole command buffer is captured into a
+ * hipGraph on its next submission and replayed as one graph launch
+ * (KAZAN_GRAPHS=0 turns this off; re-recording, or any reallocation of the
+ * device's scratch buffers, makes it capture again).
+ * vkGetGraphStatsKAZAN: graph captures and graph launches so far. */
+typedef void(VKAPI_PTR *PFN_vkGetGraphStatsKAZAN)(VkDevice device, uint32_t *pCaptures, uint32_t *pLaunches);
 
 #ifdef __cplusplus
 }

@@ -28,10 +28,15 @@
 extern "C" {
 #endif
 
+#ifndef KAZAN_EXPORT
+#define KAZAN_EXPORT __attribute__((visibility("default")))
+#endif
+
 /* stage: 0 = vertex, 4 = fragment, 5 = compute (SPIR-V execution models) */
-int kazan_spirv_disassemble(const uint32_t *words, size_t nwords, char *out, size_t cap, size_t *out_len);
-int kazan_translate_to_ir_text(const uint32_t *words, size_t nwords, uint32_t stage, const char *entry, char *out,
-                               size_t cap, size_t *out_len);
+KAZAN_EXPORT int kazan_spirv_disassemble(const uint32_t *words, size_t nwords, char *out, size_t cap,
+                                         size_t *out_len);
+KAZAN_EXPORT int kazan_translate_to_ir_text(const uint32_t *words, size_t nwords, uint32_t stage, const char *entry,
+                                            char *out, size_t cap, size_t *out_len);
 
 typedef struct KazanPipelineDesc {
     const uint32_t *vs_words;
@@ -54,10 +59,36 @@ typedef struct KazanPipelineDesc {
     uint32_t num_textures, texture_key[4];
 } KazanPipelineDesc;
 
+/* The learned tile-cost model's kernels on the current HIP device (these
+ * need a GPU; no counterpart in the reference, which has no draw path to
+ * balance).
+ *
+ * kazan_cost_model_eval: the bf16 MFMA inference kernel of the balanced
+ * pipeline over n rows of raw tile features ([n][16], the features
+ * vkGetTileProfileKAZAN returns) with a KzCostModel blob (cost_model.bin's
+ * bytes); writes the predicted fragments per row to out[n].
+ *
+ * kazan_cost_model_train: `steps` minibatch Adam steps (beta 0.9/0.999,
+ * eps 1e-8) of the MFMA training kernel on the squared error between the
+ * model output (log2(1 + fragments)) and y[n], from normalised features
+ * x[n][16]. state = [weights | first moments | second moments], each
+ * KAZAN_COST_PARAMS floats laid out w1 [64][16], w2 [64][64], w3 [64][64],
+ * b1, b2, b3, w4 [64], b4; it is updated in place. step0 = steps already
+ * taken (Adam bias correction, minibatch stream); batch = samples per
+ * step, a multiple of 64; loss_out[steps] (may be null) = each step's mean
+ * squared error.
+ *
+ * Both return 0, or a negative value on invalid input or a HIP error. */
+#define KAZAN_COST_PARAMS 9473
+KAZAN_EXPORT int kazan_cost_model_eval(const void *blob, size_t bytes, const float *features, uint32_t n,
+                                       float *out);
+KAZAN_EXPORT int kazan_cost_model_train(const float *x, const float *y, uint32_t n, float *state, uint32_t steps,
+                                        uint32_t step0, uint32_t batch, float lr, uint64_t seed, float *loss_out);
+
 /* what: 0 = generated shader source, 1 = full kernel source, 2 = compile to a
  * code object and report its size and whether the cache held it */
-int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,
-                                 size_t *out_len);
+KAZAN_EXPORT int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,
+                                              size_t *out_len);
 
 #ifdef __cplusplus
 }

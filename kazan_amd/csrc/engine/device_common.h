@@ -82,22 +82,107 @@ struct KzFrame {
     uint64_t *tri_rect;      // [total_tris] packed pixel bounding box or ~0
     unsigned long long *stats;  // KZ_STAT_* counters
 
-    // Work items (load-balanced mode). A work item is a square sub-tile of
-    // a screen tile: level L splits the tile into 4^L squares of side
-    // TS >> L. Item geometry is packed as tile << 6 | L << 4 | sub.
-    // In the static mode every tile is one item and item i is tile i.
-    uint32_t num_items;
-    uint32_t balanced;        // 1: persistent raster over `queue`
-    uint32_t *tile_level;     // [tiles] split level
-    uint32_t *tile_item_base; // [tiles] first item of the tile
-    uint32_t *items;          // [num_items] packed geometry
-    uint32_t *item_counts;    // [num_items]
-    uint32_t *item_offsets;   // [num_items]
-    uint32_t *item_cursor;    // [num_items]
-    uint32_t *queue;          // [num_items] item ids, most expensive first
-    uint32_t *queue_head;     // persistent raster work counter
-    float *tile_cost;         // [tiles] predicted cost (triangle-equivalents)
-    uint32_t *tile_frags;     // [tiles] measured fragments (training labels) or null
+    // Load-balanced mode. A work item is a square sub-tile of a screen
+    // tile: level L splits the tile into 4^L squares of side TS >> L (L = 0
+    // is the tile itself). Queue entries pack an item as
+    // tile << 6 | L << 4 | sub. Level-0 items rasterise their tile's list;
+    // the sub-tiles of a split tile ("split slot" r = split_slot[tile]) get
+    // lists of their own, r * 16 + sub, cut from the tile's list in order.
+    uint32_t balanced;          // 1: persistent raster over `queue`
+    uint32_t split_capacity;    // entries of split_lists
+    uint32_t *queue;            // [16 * tiles] items, most expensive first
+    uint32_t *ctrl;             // KZ_CTRL_* counters
+    uint32_t *tile_level;       // [tiles] split level
+    uint32_t *split_slot;       // [tiles] split slot of a split tile
+    uint32_t *split_info;       // [2 * split slots] (first chunk, tile) of a slot
+    uint32_t *chunk_tile;       // [max_chunks] tile of each KZ_SPLIT_CHUNK-entry chunk
+    uint32_t *chunk_counts;     // [chunks * 16] entries per sub-tile, then write bases
+    uint32_t *split_counts;     // [split slots * 16]
+    uint32_t *split_offsets;    // [split slots * 16]
+    uint32_t *split_lists;      // [split_capacity]
+    uint32_t max_chunks;
+    uint32_t pipe_info;         // KZ_PIPE_* bits of the pipeline (cost-model features)
+
+    // Cost model inputs and outputs (null when unused): per-triangle depth
+    // range and per-tile pair statistics gathered while binning, the 16
+    // features derived from them, the model's predicted fragments per tile
+    // (the plan splits by cost = binned triangles + frag_weight * predicted
+    // fragments, in list entries), and, in the static form with tile
+    // profiling, each tile's fragments (training labels) and raster cycles.
+    uint32_t *tri_z;            // [total_tris] z range, min | max << 16 (16-bit unorm)
+    uint32_t *tile_acc;         // [tiles * KZ_ACC]
+    float *tile_feat;           // [tiles * KZ_FEATURES]
+    float *tile_pred;           // [tiles]
+    uint32_t *tile_frags;       // [tiles]
+    uint32_t *tile_cycles;      // [tiles]
+    float frag_weight;
+    uint32_t pad3;
+};
+
+// Per-tile pair statistics (tile_acc) over the (triangle, tile) pairs; the
+// overlap of a pair is the area of the triangle's pixel box inside the tile.
+#define KZ_ACC 9
+#define KZ_ACC_AREA 0   // sum of overlaps, pixels
+#define KZ_ACC_HIST 1   // 5 bins: overlap <= 2, <= 8, <= 32, <= 128, > 128 pixels
+#define KZ_ACC_SHARED 6 // pairs whose triangle's box spans more than one tile
+#define KZ_ACC_ZMIN 7   // max of (65535 - 16-bit min z)
+#define KZ_ACC_ZMAX 8   // max of 16-bit max z
+
+// pipe_info bits
+#define KZ_PIPE_LOG2_SAMPLES(p) ((p) & 7u)
+#define KZ_PIPE_BLEND(p) (((p) >> 3) & 1u)
+#define KZ_PIPE_DEPTH(p) (((p) >> 4) & 1u)
+#define KZ_PIPE_VARYINGS(p) (((p) >> 8) & 31u)
+#define KZ_PIPE_TEXTURES(p) (((p) >> 16) & 7u)
+#define KZ_PIPE_UBOS(p) (((p) >> 20) & 7u)
+#define KZ_PIPE_PACK(log2s, blend, depth, nv, ntex, nubo)                                                   \
+    ((uint32_t)(log2s) | ((uint32_t)(blend) << 3) | ((uint32_t)(depth) << 4) | ((uint32_t)(nv) << 8) | \
+     ((uint32_t)(ntex) << 16) | ((uint32_t)(nubo) << 20))
+
+// Learned per-tile cost model: a bf16 MLP 16 -> 64 -> 64 -> 64 -> 1 with
+// fp32 accumulation and ReLU, on features normalised as
+// (x - feat_mean) * feat_scale. The output y predicts log2(1 + fragments)
+// of the tile. frag_weight converts fragments to list-entry equivalents of
+// raster time (fitted from measured tile cycles). Weights are bf16 bit
+// patterns, stored [out][in] (the layout the MFMA B operand reads).
+#define KZ_FEATURES 16
+#define KZ_HIDDEN 64
+struct KzCostModel {
+    uint32_t magic;    // KZ_COST_MODEL_MAGIC
+    uint32_t version;  // 1
+    uint32_t bytes;    // sizeof(KzCostModel)
+    float frag_weight;
+    float feat_mean[KZ_FEATURES];
+    float feat_scale[KZ_FEATURES];
+    uint16_t w1[KZ_HIDDEN * KZ_FEATURES];
+    uint16_t w2[KZ_HIDDEN * KZ_HIDDEN];
+    uint16_t w3[KZ_HIDDEN * KZ_HIDDEN];
+    float b1[KZ_HIDDEN], b2[KZ_HIDDEN], b3[KZ_HIDDEN];
+    float w4[KZ_HIDDEN];
+    float b4;
+    uint32_t pad[3];
+};
+#define KZ_COST_MODEL_MAGIC 0x4D435A4Bu  // "KZCM"
+
+// Training: the fp32 parameter vector (master weights; Adam keeps two more
+// vectors of the same layout after it) and the hyper-parameters of one run
+// of the training kernel.
+#define KZ_P_W1 0
+#define KZ_P_W2 (KZ_P_W1 + KZ_HIDDEN * KZ_FEATURES)
+#define KZ_P_W3 (KZ_P_W2 + KZ_HIDDEN * KZ_HIDDEN)
+#define KZ_P_B1 (KZ_P_W3 + KZ_HIDDEN * KZ_HIDDEN)
+#define KZ_P_B2 (KZ_P_B1 + KZ_HIDDEN)
+#define KZ_P_B3 (KZ_P_B2 + KZ_HIDDEN)
+#define KZ_P_W4 (KZ_P_B3 + KZ_HIDDEN)
+#define KZ_P_B4 (KZ_P_W4 + KZ_HIDDEN)
+#define KZ_PARAMS (KZ_P_B4 + 1)
+struct KzTrainParams {
+    float lr, beta1, beta2, eps;
+    uint32_t steps;   // Adam steps in this run
+    uint32_t step0;   // steps taken before (bias correction, sample stream)
+    uint32_t batch;   // samples per step, a multiple of 64
+    uint32_t n;       // samples in the data set
+    uint64_t seed;    // minibatch sampling stream
 };
 
 enum {
@@ -108,8 +193,22 @@ enum {
     KZ_STAT_DEPTH_PASSED = 4,
     KZ_STAT_LIST_ENTRIES = 5,
     KZ_STAT_LIST_OVERFLOW = 6,
+    KZ_STAT_SPLIT_ENTRIES = 7,
     KZ_STAT_COUNT = 8
 };
+
+// ctrl words of the balanced mode
+enum {
+    KZ_CTRL_HEAD = 0,      // persistent raster: items taken beyond the first round
+    KZ_CTRL_ITEMS = 1,     // queue length
+    KZ_CTRL_SPLITS = 2,    // split slots used
+    KZ_CTRL_CHUNKS = 3,    // chunks of split tiles' lists
+    KZ_CTRL_SPLIT_MAX = 4, // largest sub-tile list total of any batch so far (sizing)
+    KZ_CTRL_WORDS = 64
+};
+
+// Entries of a split tile's list that one wave cuts into sub-tile lists.
+#define KZ_SPLIT_CHUNK 1024u
 
 // Statistics are accumulated in KZ_STAT_STRIPES copies (a workgroup adds to
 // copy blockIdx % KZ_STAT_STRIPES) so that no single address takes every

@@ -1,9 +1,13 @@
 // engine.cpp — replays recorded command lists on the device's HIP stream.
 #include "engine.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iterator>
 
 #include "kernels.h"
 
@@ -42,14 +46,12 @@ Engine::~Engine() {
     hipFree(tile_lists_);
     hipFree(sort_tmp_);
     hipFree(tri_rect_);
-    hipFree(tile_level_);
-    hipFree(tile_item_base_);
-    hipFree(items_);
-    hipFree(item_counts_);
-    hipFree(item_offsets_);
-    hipFree(item_cursor_);
-    hipFree(queue_);
-    hipFree(ctrl_);
+    for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
+                        chunk_tile_, chunk_counts_, split_lists_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
+        hipFree(p);
+    hipFree(tile_feat_);
+    hipFree(tile_pred_);
+    hipFree(cost_model_d_);
     hipFree(stats_d_);
     if (stats_h_) hipHostFree(stats_h_);
     for (hipEvent_t e : event_pool_) hipEventDestroy(e);
@@ -73,23 +75,136 @@ bool Engine::init(int hip_device, std::string &err) {
     // tiles whose lists do not fit render through the in-order scan path
     if (const char *e = std::getenv("KAZAN_LIST_CAPACITY_LIMIT")) list_limit_ = std::strtoull(e, nullptr, 10);
     if (const char *e = std::getenv("KAZAN_PIPELINE")) balanced_ = std::strcmp(e, "plain") != 0;
+    if (const char *e = std::getenv("KAZAN_AUTOTUNE")) autotune_ = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("KAZAN_GRAPHS")) graphs_ = std::strcmp(e, "0") != 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) == hipSuccess && cus > 0)
+        split_grid_ = 4u * (uint32_t)cus;
+    // the cost model: $KAZAN_COST_MODEL, else cost_model.bin beside this library
+    std::string path;
+    if (const char *e = std::getenv("KAZAN_COST_MODEL")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&kz_launch_fill_u32, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/cost_model.bin";
+        }
+    }
+    if (!path.empty() && path != "none") {
+        std::ifstream in(path, std::ios::binary);
+        if (in) {
+            std::vector<char> blob((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+            if (set_cost_model(blob.data(), blob.size())) cost_model_src_ = path;
+        }
+    }
     return true;
 }
 
-hipError_t Engine::ensure_item_scratch(uint32_t tiles) {
-    if (tiles <= cap_items_tiles_) return hipSuccess;
+bool Engine::set_cost_model(const void *blob, size_t bytes) {
+    KzCostModel m;
+    if (!blob || bytes != sizeof m) return false;
+    std::memcpy(&m, blob, sizeof m);
+    if (m.magic != KZ_COST_MODEL_MAGIC || m.version != 1 || m.bytes != sizeof m) return false;
+    if (!cost_model_d_ && hipMalloc((void **)&cost_model_d_, sizeof m) != hipSuccess) {
+        cost_model_d_ = nullptr;
+        return false;
+    }
+    if (hipStreamSynchronize(stream_) != hipSuccess ||
+        hipMemcpy(cost_model_d_, &m, sizeof m, hipMemcpyHostToDevice) != hipSuccess)
+        return false;
+    frag_weight_ = m.frag_weight;
+    ++generation_;  // captured graphs carry the old model's frag_weight
+    cost_model_src_ = "memory";
+    return true;
+}
+
+hipError_t Engine::ensure_model_scratch(uint32_t tiles, uint32_t tris) {
+    if (tiles <= cap_model_tiles_ && tris <= cap_model_tris_) return hipSuccess;
     KZ_TRY(hipStreamSynchronize(stream_));
-    const uint32_t t = std::max(tiles, cap_items_tiles_ * 2);
-    const uint64_t items = 16ull * t;
-    KZ_TRY(grow(tile_level_, t));
-    KZ_TRY(grow(tile_item_base_, t));
-    KZ_TRY(grow(items_, items));
-    KZ_TRY(grow(item_counts_, items));
-    KZ_TRY(grow(item_offsets_, items));
-    KZ_TRY(grow(item_cursor_, items));
-    KZ_TRY(grow(queue_, items));
-    if (!ctrl_) KZ_TRY(hipMalloc((void **)&ctrl_, 128 * sizeof(uint32_t)));
-    cap_items_tiles_ = t;
+    if (tiles > cap_model_tiles_) {
+        const uint32_t t = std::max(tiles, cap_model_tiles_ * 2);
+        KZ_TRY(regrow(tile_acc_, (uint64_t)KZ_ACC * t));
+        KZ_TRY(regrow(tile_feat_, (uint64_t)KZ_FEATURES * t));
+        KZ_TRY(regrow(tile_pred_, t));
+        KZ_TRY(regrow(tile_frags_, t));
+        KZ_TRY(regrow(tile_cycles_, t));
+        cap_model_tiles_ = t;
+    }
+    if (tris > cap_model_tris_) {
+        const uint32_t t = std::max(tris, cap_model_tris_ + cap_model_tris_ / 2);
+        KZ_TRY(regrow(tri_z_, t));
+        cap_model_tris_ = t;
+    }
+    return hipSuccess;
+}
+
+namespace {
+hipError_t copy_u32_as_float(float *dst, const uint32_t *src, uint32_t n) {
+    std::vector<uint32_t> c(n);
+    KZ_TRY(hipMemcpy(c.data(), src, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) dst[i] = (float)c[i];
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t Engine::read_tile_profile(uint32_t *tiles, float *features, float *frags, float *cycles, float *pred) {
+    KZ_TRY(hipSetDevice(device_));
+    KZ_TRY(hipStreamSynchronize(stream_));
+    const uint32_t n = tile_feat_ ? last_profile_tiles_ : 0;
+    if (tiles) *tiles = n;
+    if (n == 0) return hipSuccess;
+    if (features)
+        KZ_TRY(hipMemcpy(features, tile_feat_, (size_t)n * KZ_FEATURES * sizeof(float), hipMemcpyDeviceToHost));
+    if (frags) {
+        if (last_profile_labels_) KZ_TRY(copy_u32_as_float(frags, tile_frags_, n));
+        else std::fill(frags, frags + n, 0.0f);
+    }
+    if (cycles) {
+        if (last_profile_labels_) KZ_TRY(copy_u32_as_float(cycles, tile_cycles_, n));
+        else std::fill(cycles, cycles + n, 0.0f);
+    }
+    if (pred) {
+        if (last_profile_pred_) KZ_TRY(hipMemcpy(pred, tile_pred_, n * sizeof(float), hipMemcpyDeviceToHost));
+        else std::fill(pred, pred + n, 0.0f);
+    }
+    return hipSuccess;
+}
+
+// Called after ensure_scratch (the chunk tables follow the list capacity).
+hipError_t Engine::ensure_split_scratch(uint32_t tiles) {
+    // a split tile's list fits the list storage, so its chunks number at
+    // most list_capacity / KZ_SPLIT_CHUNK plus one partial chunk per tile
+    const uint64_t want_chunks = (uint64_t)list_capacity_ / KZ_SPLIT_CHUNK + tiles;
+    const uint32_t want_split = std::max(split_capacity_, list_capacity_);
+    if (tiles <= cap_split_tiles_ && want_chunks <= max_chunks_ && want_split <= split_capacity_) return hipSuccess;
+    KZ_TRY(hipStreamSynchronize(stream_));
+    if (tiles > cap_split_tiles_) {
+        const uint32_t t = std::max(tiles, cap_split_tiles_ * 2);
+        KZ_TRY(regrow(tile_level_, t));
+        KZ_TRY(regrow(split_slot_, t));
+        KZ_TRY(regrow(split_info_, 2ull * t));
+        KZ_TRY(regrow(queue_, 16ull * t));
+        KZ_TRY(regrow(split_counts_, 16ull * t));
+        KZ_TRY(regrow(split_offsets_, 16ull * t));
+        if (!ctrl_) {
+            KZ_TRY(hipMalloc((void **)&ctrl_, KZ_CTRL_WORDS * sizeof(uint32_t)));
+            KZ_TRY(kz_launch_fill_u32(stream_, ctrl_, KZ_CTRL_WORDS, 0u));
+        }
+        cap_split_tiles_ = t;
+    }
+    if (want_chunks > max_chunks_) {
+        const uint64_t c = std::min<uint64_t>(std::max<uint64_t>(want_chunks, (uint64_t)max_chunks_ * 3 / 2),
+                                              0x0FFFFFFFull);
+        KZ_TRY(regrow(chunk_tile_, c));
+        KZ_TRY(regrow(chunk_counts_, 16 * c));
+        max_chunks_ = (uint32_t)c;
+    }
+    if (want_split > split_capacity_) {
+        KZ_TRY(regrow(split_lists_, want_split));
+        split_capacity_ = want_split;
+    }
     return hipSuccess;
 }
 
@@ -101,21 +216,21 @@ hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
     KZ_TRY(hipStreamSynchronize(stream_));
     if (tiles > cap_tiles_) {
         const uint32_t t = std::max(tiles, cap_tiles_ * 2);
-        KZ_TRY(grow(tile_counts_, t));
-        KZ_TRY(grow(tile_offsets_, t));
-        KZ_TRY(grow(tile_cursor_, t));
+        KZ_TRY(regrow(tile_counts_, t));
+        KZ_TRY(regrow(tile_offsets_, t));
+        KZ_TRY(regrow(tile_cursor_, t));
         cap_tiles_ = t;
     }
     if (tris > cap_tris_) {
         const uint32_t t = std::max(tris, cap_tris_ + cap_tris_ / 2);
-        KZ_TRY(grow(tri_rect_, t));
+        KZ_TRY(regrow(tri_rect_, t));
         cap_tris_ = t;
     }
     if (want_list > list_capacity_) {
         const uint64_t c = std::min<uint64_t>(std::max<uint64_t>(want_list, (uint64_t)list_capacity_ * 3 / 2),
                                               0xFFFFFFF0ull);
-        KZ_TRY(grow(tile_lists_, c));
-        KZ_TRY(grow(sort_tmp_, c));
+        KZ_TRY(regrow(tile_lists_, c));
+        KZ_TRY(regrow(sort_tmp_, c));
         list_capacity_ = (uint32_t)c;
     }
     return hipSuccess;
@@ -135,11 +250,33 @@ hipError_t Engine::begin_submit() {
     return hipSuccess;
 }
 
-hipError_t Engine::run_batch(const Op &op) {
+// All allocations a batch needs at tile size 1 << shift (so that a graph
+// capture of the batch allocates nothing).
+hipError_t Engine::prepare_batch(const Op &op, uint32_t shift) {
+    const KzFrame &f = op.frame;
+    if (f.total_tris == 0) return hipSuccess;
+    const uint32_t ts = 1u << shift;
+    const uint32_t ntiles = ((f.width + ts - 1) >> shift) * ((f.height + ts - 1) >> shift);
+    KZ_TRY(ensure_scratch(ntiles, f.total_tris));
+    if (balanced_) KZ_TRY(ensure_split_scratch(ntiles));
+    if ((cost_model_d_ != nullptr && balanced_) || tile_profile_) KZ_TRY(ensure_model_scratch(ntiles, f.total_tris));
+    return hipSuccess;
+}
+
+hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     KzFrame f = op.frame;
     if (f.total_tris == 0) return hipSuccess;
+    std::string err;
+    const PipelineKernels *kern = op.variants->get(shift, &err);
+    if (!kern) {
+        std::fprintf(stderr, "kazan: tile size %u: %s\n", 1u << shift, err.c_str());
+        return hipErrorInvalidValue;
+    }
+    const uint32_t ts = 1u << shift;
+    f.tiles_x = (f.width + ts - 1) >> shift;
+    f.tiles_y = (f.height + ts - 1) >> shift;
     const uint32_t ntiles = f.tiles_x * f.tiles_y;
-    KZ_TRY(ensure_scratch(ntiles, f.total_tris));
+    KZ_TRY(prepare_batch(op, shift));
     f.draws = op.d_draws;
     f.tile_counts = tile_counts_;
     f.tile_offsets = tile_offsets_;
@@ -149,76 +286,109 @@ hipError_t Engine::run_batch(const Op &op) {
     f.list_capacity = list_capacity_;
     f.tri_rect = tri_rect_;
     f.stats = stats_d_;
-    f.tile_shift = op.tile_shift;
+    f.tile_shift = shift;
+    if (balanced_) {
+        f.balanced = 1;
+        f.split_capacity = split_capacity_;
+        f.queue = queue_;
+        f.ctrl = ctrl_;
+        f.tile_level = tile_level_;
+        f.split_slot = split_slot_;
+        f.split_info = split_info_;
+        f.chunk_tile = chunk_tile_;
+        f.chunk_counts = chunk_counts_;
+        f.split_counts = split_counts_;
+        f.split_offsets = split_offsets_;
+        f.split_lists = split_lists_;
+        f.max_chunks = max_chunks_;
+    }
+    // the cost model's inputs (and, in the static form, its labels) when a
+    // model will run or tiles are being profiled
+    // (predictions also run in a profiled static batch, to measure the
+    // model's error against the fragments the rasteriser counts)
+    const bool model = cost_model_d_ != nullptr && (balanced_ || tile_profile_);
+    const bool acc = model || tile_profile_;
+    const bool labels = tile_profile_ && !balanced_;
+    if (acc) {
+        f.tri_z = tri_z_;
+        f.tile_acc = tile_acc_;
+        f.tile_feat = tile_feat_;
+        if (model && balanced_) {
+            f.tile_pred = tile_pred_;
+            f.frag_weight = frag_weight_;
+        }
+        if (labels) {
+            f.tile_frags = tile_frags_;
+            f.tile_cycles = tile_cycles_;
+        }
+    }
     void *args[] = {&f};
     const uint32_t tri_blocks = (f.total_tris + 255) / 256;
+    uint32_t launches = 0;
 
-    if (!balanced_) {
-        KZ_TRY(mark(KS_CLEAR_COUNTS, false));
-        KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
-        KZ_TRY(mark(KS_CLEAR_COUNTS, true));
-        KZ_TRY(mark(KS_BIN_COUNT, false));
-        KZ_TRY(hipModuleLaunchKernel(op.kernels->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
-        KZ_TRY(mark(KS_BIN_COUNT, true));
-        KZ_TRY(mark(KS_BIN_SCAN, false));
-        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, nullptr, list_capacity_,
-                              stats_d_));
-        KZ_TRY(mark(KS_BIN_SCAN, true));
-        KZ_TRY(mark(KS_BIN_SCATTER, false));
-        KZ_TRY(kz_launch_bin_scatter(stream_, f));
-        KZ_TRY(mark(KS_BIN_SCATTER, true));
-        KZ_TRY(mark(KS_BIN_SORT, false));
-        KZ_TRY(kz_launch_sort_lists(stream_, tile_counts_, tile_offsets_, ntiles, nullptr, tile_lists_, sort_tmp_,
-                                    list_capacity_));
-        KZ_TRY(mark(KS_BIN_SORT, true));
-        KZ_TRY(mark(KS_RASTER, false));
-        KZ_TRY(hipModuleLaunchKernel(op.kernels->raster, ntiles, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
-        KZ_TRY(mark(KS_RASTER, true));
-        pending_batches_ += 1;
-        pending_launches_ += 6;
-        return hipSuccess;
-    }
-
-    // balanced: tiles -> work items by predicted cost, per-item lists, queue
-    KZ_TRY(ensure_item_scratch(ntiles));
-    const uint32_t max_items = 16 * ntiles;
-    f.balanced = 1;
-    f.tile_level = tile_level_;
-    f.tile_item_base = tile_item_base_;
-    f.items = items_;
-    f.item_counts = item_counts_;
-    f.item_offsets = item_offsets_;
-    f.item_cursor = item_cursor_;
-    f.queue = queue_;
-    f.queue_head = ctrl_;
+    // binning: pixel boxes, pairs per tile, offsets, lists, primitive order
     KZ_TRY(mark(KS_CLEAR_COUNTS, false));
     KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
-    KZ_TRY(kz_launch_fill_u32(stream_, ctrl_, 128, 0u));
+    if (acc) KZ_TRY(kz_launch_fill_u32(stream_, tile_acc_, (uint64_t)KZ_ACC * ntiles, 0u));
     KZ_TRY(mark(KS_CLEAR_COUNTS, true));
     KZ_TRY(mark(KS_BIN_COUNT, false));
-    KZ_TRY(hipModuleLaunchKernel(op.kernels->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+    KZ_TRY(hipModuleLaunchKernel(kern->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
     KZ_TRY(mark(KS_BIN_COUNT, true));
+    KZ_TRY(mark(KS_BIN_HIST, false));
+    KZ_TRY(acc ? kz_launch_bin_hist_acc(stream_, f) : kz_launch_bin_hist(stream_, f));
+    KZ_TRY(mark(KS_BIN_HIST, true));
     KZ_TRY(mark(KS_BIN_SCAN, false));
-    KZ_TRY(kz_launch_plan(stream_, f, ntiles, ctrl_));
-    KZ_TRY(kz_launch_plan_emit(stream_, f, ntiles));
-    KZ_TRY(kz_launch_item_count(stream_, f));
-    KZ_TRY(kz_launch_scan(stream_, item_counts_, item_offsets_, item_cursor_, max_items, ctrl_ + 1, list_capacity_,
-                          stats_d_));
+    KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_));
     KZ_TRY(mark(KS_BIN_SCAN, true));
     KZ_TRY(mark(KS_BIN_SCATTER, false));
-    KZ_TRY(kz_launch_item_scatter(stream_, f));
+    KZ_TRY(kz_launch_bin_scatter(stream_, f));
     KZ_TRY(mark(KS_BIN_SCATTER, true));
     KZ_TRY(mark(KS_BIN_SORT, false));
-    KZ_TRY(kz_launch_sort_lists(stream_, item_counts_, item_offsets_, max_items, ctrl_ + 1, tile_lists_, sort_tmp_,
-                                list_capacity_));
-    KZ_TRY(kz_launch_queue(stream_, f, ctrl_, max_items));
+    KZ_TRY(kz_launch_sort_lists(stream_, f, ntiles));
     KZ_TRY(mark(KS_BIN_SORT, true));
-    KZ_TRY(mark(KS_RASTER, false));
-    KZ_TRY(hipModuleLaunchKernel(op.kernels->raster_persistent, std::min(op.kernels->persistent_blocks, max_items), 1,
-                                 1, 64, 1, 1, 0, stream_, args, nullptr));
-    KZ_TRY(mark(KS_RASTER, true));
+    launches += acc ? 7 : 6;
+
+    if (!balanced_) {
+        if (acc) {
+            KZ_TRY(kz_launch_tile_features(stream_, f, ntiles));
+            launches += 1;
+        }
+        if (model) {
+            KZ_TRY(kz_launch_cost_mlp(stream_, cost_model_d_, tile_feat_, ntiles, tile_pred_));
+            launches += 1;
+        }
+        KZ_TRY(mark(KS_RASTER, false));
+        KZ_TRY(hipModuleLaunchKernel(kern->raster, ntiles, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
+        KZ_TRY(mark(KS_RASTER, true));
+        launches += 1;
+    } else {
+        // predicted costs, split expensive tiles, cut their sub-tile lists,
+        // order the items
+        KZ_TRY(mark(KS_PLAN, false));
+        if (acc) {
+            KZ_TRY(kz_launch_tile_features(stream_, f, ntiles));
+            launches += 1;
+        }
+        if (model) {
+            KZ_TRY(kz_launch_cost_mlp(stream_, cost_model_d_, tile_feat_, ntiles, tile_pred_));
+            launches += 1;
+        }
+        KZ_TRY(kz_launch_plan(stream_, f, ntiles));
+        KZ_TRY(kz_launch_split(stream_, f, split_grid_));
+        KZ_TRY(mark(KS_PLAN, true));
+        KZ_TRY(mark(KS_RASTER, false));
+        const uint32_t grid = std::max(1u, std::min(kern->persistent_blocks / std::max(grid_div, 1u), 16 * ntiles));
+        KZ_TRY(hipModuleLaunchKernel(kern->raster_persistent, grid, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
+        KZ_TRY(mark(KS_RASTER, true));
+        launches += 5;
+    }
+    if (acc) {
+        last_profile_tiles_ = ntiles;
+        last_profile_pred_ = model;
+        last_profile_labels_ = labels;
+    }
     pending_batches_ += 1;
-    pending_launches_ += 14;
+    pending_launches_ += launches;
     return hipSuccess;
 }
 
@@ -235,15 +405,166 @@ hipError_t Engine::mark(int slot, bool end) {
     return hipSuccess;
 }
 
-hipError_t Engine::execute(const std::vector<Op> &ops) {
-    for (const Op &op : ops) {
+BatchTuning::~BatchTuning() {
+    for (hipEvent_t e : ev)
+        if (e) hipEventDestroy(e);
+}
+
+void GraphCache::reset() {
+    if (exec) hipGraphExecDestroy(exec);
+    exec = nullptr;
+    generation = 0;
+}
+
+template <class T>
+hipError_t Engine::regrow(T *&p, uint64_t count) {
+    ++generation_;
+    return grow(p, count);
+}
+
+// Tuning candidates of a batch: every tile size whose kernels fit (full
+// grid) first; then, at the fastest tile size, a half and a quarter grid.
+hipError_t Engine::pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool &timed) {
+    shift = op.tile_shift;
+    grid_div = 1;
+    timed = false;
+    if (!autotune()) return hipSuccess;
+    if (!op.tuning) {
+        op.tuning = std::make_shared<BatchTuning>();
+        BatchTuning &t = *op.tuning;
+        for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s)
+            if (op.variants->fits(s)) t.cand.push_back({s, 1u});
+        t.ms.assign(t.cand.size(), -1.0f);
+        KZ_TRY(hipEventCreate(&t.ev[0]));
+        KZ_TRY(hipEventCreate(&t.ev[1]));
+    }
+    BatchTuning &t = *op.tuning;
+    if (t.done) {
+        shift = t.shift;
+        grid_div = t.grid_div;
+        return hipSuccess;
+    }
+    if (t.pending >= 0) {  // the previous submission's trial
+        KZ_TRY(hipEventSynchronize(t.ev[1]));
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess) t.ms[t.pending] = ms;
+        t.pending = -1;
+    }
+    // skip candidates whose kernels do not build
+    while (t.next < t.cand.size() && !op.variants->get(t.cand[t.next].first)) ++t.next;
+    if (t.next == t.cand.size() && !t.grid_phase) {
+        t.grid_phase = true;
+        int best = -1;
+        for (size_t i = 0; i < t.cand.size(); ++i)
+            if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
+        if (best >= 0) {
+            t.cand.push_back({t.cand[best].first, 2u});
+            t.cand.push_back({t.cand[best].first, 4u});
+            t.ms.resize(t.cand.size(), -1.0f);
+        }
+    }
+    if (t.next < t.cand.size()) {
+        shift = t.cand[t.next].first;
+        grid_div = t.cand[t.next].second;
+        t.pending = (int)t.next++;
+        timed = true;
+        return hipSuccess;
+    }
+    int best = -1;
+    for (size_t i = 0; i < t.cand.size(); ++i)
+        if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
+    t.done = true;
+    t.shift = best >= 0 ? t.cand[best].first : op.tile_shift;
+    t.grid_div = best >= 0 ? t.cand[best].second : 1u;
+    shift = t.shift;
+    grid_div = t.grid_div;
+    return hipSuccess;
+}
+
+hipError_t Engine::execute(std::vector<Op> &ops, GraphCache *cache) {
+    bool graphable = cache != nullptr && graphs_ && !kernel_timing_;
+    for (Op &op : ops)
+        if (op.kind == Op::Batch && autotune() && !(op.tuning && op.tuning->done)) graphable = false;
+    if (!graphable) return execute_eager(ops);
+    // every allocation before the capture
+    for (Op &op : ops)
+        if (op.kind == Op::Batch) {
+            uint32_t shift, grid_div;
+            bool timed;
+            KZ_TRY(pick_config(op, shift, grid_div, timed));
+            if (!op.variants->get(shift)) return execute_eager(ops);
+            KZ_TRY(prepare_batch(op, shift));
+        }
+    if (!cache->exec || cache->generation != generation_) {
+        cache->reset();
+        const uint32_t b0 = pending_batches_, l0 = pending_launches_;
+        KZ_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+        const uint64_t gen = generation_;
+        const hipError_t e = launch_ops(ops);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(stream_, &graph);
+        cache->batches = pending_batches_ - b0;
+        cache->launches = pending_launches_ - l0;
+        pending_batches_ = b0;
+        pending_launches_ = l0;
+        if (e != hipSuccess || e2 != hipSuccess || gen != generation_) {
+            if (graph) hipGraphDestroy(graph);
+            if (e != hipSuccess) return e;
+            if (e2 != hipSuccess) return e2;
+            return execute_eager(ops);  // something was reallocated mid-capture
+        }
+        const hipError_t e3 = hipGraphInstantiate(&cache->exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e3 != hipSuccess) {
+            cache->exec = nullptr;
+            return e3;
+        }
+        cache->generation = generation_;
+        ++graph_captures_;
+    }
+    KZ_TRY(hipGraphLaunch(cache->exec, stream_));
+    pending_batches_ += cache->batches;
+    pending_launches_ += cache->launches;
+    ++graph_launches_;
+    return hipSuccess;
+}
+
+hipError_t Engine::execute_eager(std::vector<Op> &ops) {
+    for (Op &op : ops)
+        if (op.kind == Op::Batch) {
+            uint32_t shift, grid_div;
+            bool timed;
+            KZ_TRY(pick_config(op, shift, grid_div, timed));
+            if (timed) KZ_TRY(hipEventRecord(op.tuning->ev[0], stream_));
+            KZ_TRY(run_batch(op, shift, grid_div));
+            if (timed) KZ_TRY(hipEventRecord(op.tuning->ev[1], stream_));
+        } else {
+            KZ_TRY(launch_ops_one(op));
+        }
+    return hipSuccess;
+}
+
+hipError_t Engine::launch_ops(std::vector<Op> &ops) {
+    for (Op &op : ops) {
+        if (op.kind == Op::Batch) {
+            const BatchTuning *t = op.tuning.get();
+            KZ_TRY(run_batch(op, t && t->done ? t->shift : op.tile_shift, t && t->done ? t->grid_div : 1u));
+        } else {
+            KZ_TRY(launch_ops_one(op));
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t Engine::launch_ops_one(const Op &op) {
+    {
         switch (op.kind) {
         case Op::Fill:
             KZ_TRY(kz_launch_fill_rect_u32(stream_, op.fill_ptr, op.fill_width, op.x0, op.y0, op.w, op.h, op.fill_spp,
                                            op.fill_value));
             ++pending_launches_;
             break;
-        case Op::Batch: KZ_TRY(run_batch(op)); break;
+        case Op::Batch: break;
         case Op::Copy:
             if (op.bytes) KZ_TRY(hipMemcpyAsync(op.dst, op.src, op.bytes, hipMemcpyDefault, stream_));
             break;
@@ -302,10 +623,23 @@ void Engine::collect_stats() {
         if (list_limit_) need = std::min<uint64_t>(need, list_limit_);
         if (need > list_capacity_ && need < 0xFFFFFFF0ull) {
             hipStreamSynchronize(stream_);
-            if (grow(tile_lists_, need) == hipSuccess && grow(sort_tmp_, need) == hipSuccess)
+            if (regrow(tile_lists_, need) == hipSuccess && regrow(sort_tmp_, need) == hipSuccess)
                 list_capacity_ = (uint32_t)need;
             else
                 list_capacity_ = 0, cap_tris_ = 0;  // next ensure_scratch reallocates
+        }
+    }
+    // Likewise for sub-tile lists (their overflow fallback filters the
+    // parent tile's list): the device keeps the largest batch total.
+    if (ctrl_ && last_.counters[KZ_STAT_SPLIT_ENTRIES] > split_capacity_) {
+        uint32_t mx = 0;
+        if (hipMemcpy(&mx, ctrl_ + KZ_CTRL_SPLIT_MAX, sizeof mx, hipMemcpyDeviceToHost) == hipSuccess &&
+            mx > split_capacity_) {
+            const uint64_t need = std::min<uint64_t>((uint64_t)mx + mx / 4, 0xFFFFFFF0ull);
+            if (regrow(split_lists_, need) == hipSuccess)
+                split_capacity_ = (uint32_t)need;
+            else
+                split_capacity_ = 0;  // next ensure_split_scratch reallocates
         }
     }
 }

@@ -49,6 +49,58 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
 // Loads the code object on the current device.
 bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, std::string &err);
 
+// Tile sizes (log2 of the side) a pipeline can run with.
+constexpr uint32_t kMinTileShift = 3, kMaxTileShift = 6;
+
+// LDS bytes of one raster workgroup (colour, depth, claims, fragment queue).
+inline uint32_t raster_lds_bytes(uint32_t tile_shift, uint32_t samples) {
+    const uint32_t tp = 1u << (2 * tile_shift);
+    return tp * samples * 8 + tp * 4 + tp * 16;
+}
+
+// A pipeline's kernels for every tile size, compiled and loaded on first
+// use (the autotuner tries several; code objects come from the cache).
+class PipelineVariants {
+  public:
+    PipelineVariants() = default;
+    PipelineVariants(const PipelineVariants &) = delete;
+    PipelineVariants &operator=(const PipelineVariants &) = delete;
+    ~PipelineVariants();
+    FixedState fixed;  // tile_shift: the pipeline's default tile size
+    std::string shader_source;
+    // null when the tile size does not fit the workgroup's LDS or the
+    // compile failed (err says why)
+    const PipelineKernels *get(uint32_t tile_shift, std::string *err = nullptr);
+    bool fits(uint32_t tile_shift) const {
+        return tile_shift >= kMinTileShift && tile_shift <= kMaxTileShift &&
+               raster_lds_bytes(tile_shift, fixed.samples) <= 160u * 1024u;
+    }
+
+  private:
+    PipelineKernels k_[kMaxTileShift + 1];
+    bool tried_[kMaxTileShift + 1] = {};
+    std::string err_[kMaxTileShift + 1];
+};
+
+// Autotuning of one batch (balanced form): its first submissions time
+// candidate (tile size, persistent-raster grid) pairs with HIP events, one
+// candidate per submission; then the fastest is kept. Pixels never depend
+// on the choice.
+struct BatchTuning {
+    BatchTuning() = default;
+    BatchTuning(const BatchTuning &) = delete;
+    BatchTuning &operator=(const BatchTuning &) = delete;
+    ~BatchTuning();
+    bool done = false;
+    bool grid_phase = false;                           // tile sizes timed; now grid sizes
+    std::vector<std::pair<uint32_t, uint32_t>> cand;   // (tile shift, grid divisor)
+    std::vector<float> ms;                             // per candidate, -1 = not timed
+    uint32_t next = 0;                                 // next candidate to try
+    int pending = -1;                                  // candidate whose events are in flight
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    uint32_t shift = 4, grid_div = 1;                  // the choice once done
+};
+
 struct Op {
     enum Kind { Fill, Batch, Copy, Copy2D, Resolve } kind;
     // Fill: ptr = attachment base, rectangle in pixels, spp words per pixel
@@ -57,8 +109,9 @@ struct Op {
     int32_t x0 = 0, y0 = 0;
     uint32_t w = 0, h = 0;
     // Batch
-    const PipelineKernels *kernels = nullptr;
-    uint32_t tile_shift = 4;
+    PipelineVariants *variants = nullptr;
+    uint32_t tile_shift = 4;                // the pipeline's default tile size
+    std::shared_ptr<BatchTuning> tuning;    // created at the first submission
     KzFrame frame{};                // draws/scratch pointers filled at submit
     std::vector<KzDraw> draws;      // host copy, uploaded at end of recording
     KzDraw *d_draws = nullptr;      // device copy owned by the command buffer
@@ -74,7 +127,17 @@ struct Op {
 };
 
 // Kernels of a batch, in launch order, for per-kernel timing.
-enum KernelSlot { KS_CLEAR_COUNTS = 0, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_BIN_SORT, KS_RASTER, KS_COUNT };
+enum KernelSlot {
+    KS_CLEAR_COUNTS = 0,
+    KS_BIN_COUNT,
+    KS_BIN_HIST,
+    KS_BIN_SCAN,
+    KS_BIN_SCATTER,
+    KS_BIN_SORT,
+    KS_PLAN,
+    KS_RASTER,
+    KS_COUNT
+};
 
 struct SubmitStats {
     unsigned long long counters[KZ_STAT_COUNT] = {};
@@ -85,6 +148,19 @@ struct SubmitStats {
     uint32_t kernel_launches[KS_COUNT] = {};
 };
 
+// A command buffer's op list captured as a hipGraph, valid while the
+// engine's scratch buffers and settings are those it was captured with.
+struct GraphCache {
+    GraphCache() = default;
+    GraphCache(const GraphCache &) = delete;
+    GraphCache &operator=(const GraphCache &) = delete;
+    ~GraphCache() { reset(); }
+    void reset();
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+    uint32_t batches = 0, launches = 0;
+};
+
 class Engine {
   public:
     ~Engine();
@@ -93,9 +169,14 @@ class Engine {
     int device() const { return device_; }
 
     // Replays ops on the stream. Returns hipSuccess or the first error.
+    // With a cache (a command buffer's), once every batch is tuned the ops
+    // are captured into a hipGraph on first use and replayed as one graph
+    // launch afterwards (KAZAN_GRAPHS=0 turns this off).
     hipError_t begin_submit();
-    hipError_t execute(const std::vector<Op> &ops);
+    hipError_t execute(std::vector<Op> &ops, GraphCache *cache = nullptr);
     hipError_t end_submit(hipEvent_t fence_event);
+    uint32_t graph_launches() const { return graph_launches_; }
+    uint32_t graph_captures() const { return graph_captures_; }
     // After the submission's work is complete: read stats, adapt capacity.
     void collect_stats();
     const SubmitStats &last_stats() const { return last_; }
@@ -115,19 +196,70 @@ class Engine {
     // engine's stream. Off by default (it adds a little latency per launch).
     void set_kernel_timing(bool on) { kernel_timing_ = on; }
 
+    // Learned cost model of the balanced form. Loaded at init from
+    // $KAZAN_COST_MODEL, else cost_model.bin beside the driver library
+    // ("none" disables it; without a model the plan splits by binned
+    // triangle count). Returns false and leaves the model unchanged when
+    // `blob` is not a valid KzCostModel.
+    bool set_cost_model(const void *blob, size_t bytes);
+    bool has_cost_model() const { return cost_model_d_ != nullptr; }
+    const std::string &cost_model_source() const { return cost_model_src_; }
+
+    // Tile profiling (training data for the cost model): each batch also
+    // computes the per-tile features (and, with a model, its predictions),
+    // and the static form records every tile's fragments and raster-
+    // workgroup cycles. read_tile_profile copies the last batch's features
+    // [tiles * KZ_FEATURES], fragments, cycles and predicted fragments
+    // [tiles] (zeros where the batch did not produce them); any may be null.
+    void set_tile_profile(bool on) {
+        if (on != tile_profile_) ++generation_;
+        tile_profile_ = on;
+    }
+    hipError_t read_tile_profile(uint32_t *tiles, float *features, float *frags, float *cycles, float *pred);
+
+    // Autotuning of balanced batches (KAZAN_AUTOTUNE=0 turns it off: every
+    // batch then runs at the pipeline's tile size and the full grid).
+    bool autotune() const { return autotune_ && balanced_; }
+
   private:
     hipError_t ensure_scratch(uint32_t tiles, uint32_t tris);
-    hipError_t run_batch(const Op &op);
+    // the configuration a batch runs with this submission (tuning trial or
+    // the tuned choice), and its scratch allocated for it
+    hipError_t pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool &timed);
+    hipError_t prepare_batch(const Op &op, uint32_t shift);
+    hipError_t run_batch(const Op &op, uint32_t shift, uint32_t grid_div);
+    hipError_t execute_eager(std::vector<Op> &ops);
+    hipError_t launch_ops(std::vector<Op> &ops);  // tuned configurations, no tuning events
+    hipError_t launch_ops_one(const Op &op);      // a non-batch op
     hipError_t mark(int slot, bool end);
+    // reallocation of a buffer that kernels (and captured graphs) point at
+    template <class T>
+    hipError_t regrow(T *&p, uint64_t count);
 
     bool kernel_timing_ = false;
     bool balanced_ = true;
+    bool autotune_ = true, graphs_ = true;
+    uint64_t generation_ = 1;  // bumped whenever captured graphs go stale
+    uint32_t graph_launches_ = 0, graph_captures_ = 0;
     uint64_t list_limit_ = 0;
-    // balanced-mode scratch, sized for 16 work items per tile
-    uint32_t *tile_level_ = nullptr, *tile_item_base_ = nullptr, *items_ = nullptr, *item_counts_ = nullptr,
-             *item_offsets_ = nullptr, *item_cursor_ = nullptr, *queue_ = nullptr, *ctrl_ = nullptr;
-    uint32_t cap_items_tiles_ = 0;
-    hipError_t ensure_item_scratch(uint32_t tiles);
+    // balanced-mode scratch: up to 16 work items per tile; chunk tables
+    // sized from the list capacity; sub-tile lists grown on demand
+    uint32_t *tile_level_ = nullptr, *split_slot_ = nullptr, *split_info_ = nullptr, *queue_ = nullptr,
+             *ctrl_ = nullptr, *split_counts_ = nullptr, *split_offsets_ = nullptr, *chunk_tile_ = nullptr,
+             *chunk_counts_ = nullptr, *split_lists_ = nullptr;
+    uint32_t cap_split_tiles_ = 0, max_chunks_ = 0, split_capacity_ = 0;
+    uint32_t split_grid_ = 1024;  // workgroups of the chunk kernels: 4 per CU
+    // cost model and tile profile
+    KzCostModel *cost_model_d_ = nullptr;
+    float frag_weight_ = 0.0f;
+    std::string cost_model_src_;
+    bool tile_profile_ = false;
+    uint32_t *tile_acc_ = nullptr, *tile_cycles_ = nullptr, *tile_frags_ = nullptr, *tri_z_ = nullptr;
+    float *tile_feat_ = nullptr, *tile_pred_ = nullptr;
+    uint32_t cap_model_tiles_ = 0, cap_model_tris_ = 0, last_profile_tiles_ = 0;
+    bool last_profile_pred_ = false, last_profile_labels_ = false;
+    hipError_t ensure_model_scratch(uint32_t tiles, uint32_t tris);
+    hipError_t ensure_split_scratch(uint32_t tiles);
     std::vector<std::pair<int, hipEvent_t>> timing_events_;  // (slot, start) then end, alternating
     std::vector<hipEvent_t> event_pool_;
     size_t events_used_ = 0;

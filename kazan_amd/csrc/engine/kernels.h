@@ -8,19 +8,34 @@
 hipError_t kz_launch_fill_u32(hipStream_t st, uint32_t *dst, uint64_t n, uint32_t value);
 hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width, int32_t x0, int32_t y0, uint32_t w,
                                    uint32_t h, uint32_t spp, uint32_t value);
-// Exclusive scan of counts[0, n) (n read from n_dev when it is not null).
+// Exclusive scan of counts[0, n) into offsets and cursor; adds the total to
+// the list-entry statistic and flags an overflow beyond `capacity`.
 hipError_t kz_launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          const uint32_t *n_dev, uint32_t capacity, unsigned long long *stats);
+                          uint32_t capacity, unsigned long long *stats);
+// Binning from the per-triangle pixel boxes: pairs per tile, then the lists.
+hipError_t kz_launch_bin_hist(hipStream_t st, const KzFrame &f);
 hipError_t kz_launch_bin_scatter(hipStream_t st, const KzFrame &f);
-hipError_t kz_launch_sort_lists(hipStream_t st, const uint32_t *counts, const uint32_t *offsets, uint32_t n,
-                                const uint32_t *n_dev, uint32_t *lists, uint32_t *tmp, uint32_t capacity);
-// Balanced mode: split levels + item bases (item total -> ctrl[1]), item
-// geometry, per-item binning, and the most-expensive-first queue.
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *ctrl);
-hipError_t kz_launch_plan_emit(hipStream_t st, const KzFrame &f, uint32_t ntiles);
-hipError_t kz_launch_item_count(hipStream_t st, const KzFrame &f);
-hipError_t kz_launch_item_scatter(hipStream_t st, const KzFrame &f);
-hipError_t kz_launch_queue(hipStream_t st, const KzFrame &f, uint32_t *ctrl, uint32_t max_items);
+// Cost model: pairs per tile plus their statistics (f.tile_acc, cleared by
+// the caller), the per-tile features (f.tile_feat), the MLP (features ->
+// predicted fragments per tile).
+hipError_t kz_launch_bin_hist_acc(hipStream_t st, const KzFrame &f);
+hipError_t kz_launch_tile_features(hipStream_t st, const KzFrame &f, uint32_t ntiles);
+hipError_t kz_launch_cost_mlp(hipStream_t st, const KzCostModel *model, const float *feat, uint32_t ntiles,
+                              float *pred);
+// p.steps Adam steps on state = [W | M | V] (KZ_PARAMS each) against
+// targets y[n] from normalised features x[n][16]; loss_out[steps] or null.
+hipError_t kz_launch_cost_train(hipStream_t st, float *state, const float *x, const float *y, const KzTrainParams &p,
+                                float *loss_out);
+// Training data: the Gaussian-cluster scene of `seed` (nc of n triangles in
+// the cluster) as 3 * n pos4 + colour4 vertices.
+hipError_t kz_launch_gen_cluster(hipStream_t st, float *vertices, uint32_t n, uint32_t nc, uint32_t width,
+                                 uint32_t height, double sigma_frac, double radius, uint64_t seed);
+hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntiles);
+// Balanced mode: split levels, split slots and the ordered item queue
+// (one workgroup), then the sub-tile lists of split tiles on `grid`
+// workgroups.
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles);
+hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid);
 hipError_t kz_launch_stats_reduce(hipStream_t st, unsigned long long *stats);
 hipError_t kz_launch_resolve4(hipStream_t st, const uint32_t *src, uint32_t *dst, uint32_t width, int32_t x0,
                               int32_t y0, uint32_t w, uint32_t h);

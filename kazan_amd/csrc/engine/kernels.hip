@@ -3,16 +3,23 @@
 //   kz_fill_u32 / kz_fill_rect_u32  attachment clears and counter resets (a
 //                    fill kernel rather than a memset, so command buffers can
 //                    be captured into hipGraphs)
+//   kz_bin_hist      (triangle, tile) pairs per tile
 //   kz_scan          exclusive prefix sum of per-list counts
 //   kz_bin_scatter   writes triangle ids into the per-tile lists
 //   kz_sort_lists    sorts each list into primitive order (bitonic in LDS;
 //                    lists longer than one LDS load are sorted in LDS-sized
 //                    runs and merged with merge-path in HBM)
-//   kz_plan_*        load-balanced mode: split each tile into 1, 4 or 16
-//                    work items by predicted cost, bin triangles per item,
-//                    order the items most expensive first
+//   kz_plan, kz_split_*  load-balanced mode: split expensive tiles into 4 or
+//                    16 sub-tiles, cut their lists into ordered sub-tile
+//                    lists, order all work items most expensive first
 //   kz_resolve4      4x -> 1x box resolve, (s0+s1+s2+s3+2) >> 2 per channel
 //   kz_stats_reduce  folds the striped statistics counters
+//
+// Binning aggregates in LDS before it touches HBM counters: a workgroup
+// takes a chunk of consecutive triangles, counts its (triangle, tile) pairs
+// per tile in an LDS hash table, and then adds one total per distinct tile.
+// A skewed scene (thousands of triangles in a few tiles) thus costs one HBM
+// atomic per (workgroup, tile), not one per wave and tile.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -24,6 +31,16 @@ constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortCap = 4096;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+constexpr uint32_t kBinThreads = 512;
+constexpr uint32_t kHashLog = 12;
+constexpr uint32_t kHashSize = 1u << kHashLog;  // 32 KB of LDS: keys + values
+constexpr uint32_t kHashProbes = 32;
+constexpr uint32_t kHashLogAcc = 10;
+constexpr uint32_t kHashSizeAcc = 1u << kHashLogAcc;  // 44 KB: keys, values, KZ_ACC statistics
+
+constexpr uint32_t kPlanThreads = 1024;
+constexpr uint32_t kSplitThreads = 256;
 
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -41,34 +58,8 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// One atomic per distinct counter per wave.
-__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t idx, uint32_t lane) {
-    uint64_t pending = __ballot(idx != kNone);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)idx, leader);
-        const uint64_t peers = __ballot(idx == t) & pending;
-        if (lane == leader) atomicAdd(&counts[t], (uint32_t)__popcll(peers));
-        pending &= ~peers;
-    }
-}
-
-// Reserves one slot per lane with an index, one atomic per distinct index
-// per wave; returns this lane's slot (or kNone).
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t *cursor, uint32_t idx, uint32_t lane) {
-    uint64_t pending = __ballot(idx != kNone);
-    uint32_t my_slot = kNone;
-    while (pending) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)idx, leader);
-        const uint64_t peers = __ballot(idx == t) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[t], (uint32_t)__popcll(peers));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if ((peers >> lane) & 1ull) my_slot = base + rank_in(peers);
-        pending &= ~peers;
-    }
-    return my_slot;
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
 __device__ __forceinline__ void unpack_box(uint64_t r, uint32_t &x0, uint32_t &y0, uint32_t &x1, uint32_t &y1) {
@@ -76,6 +67,42 @@ __device__ __forceinline__ void unpack_box(uint64_t r, uint32_t &x0, uint32_t &y
     y0 = (uint32_t)((r >> 16) & 0xFFFF);
     x1 = (uint32_t)((r >> 32) & 0xFFFF);
     y1 = (uint32_t)(r >> 48);
+}
+
+// Exclusive scan of in[0, n) into out (in == out allowed) by one workgroup of
+// kScanThreads; returns the total to every thread.
+__device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *cursor, uint32_t n,
+                                         uint32_t *s_wave, uint32_t *s_carry) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) *s_carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < n; b += kScanThreads) {
+        const uint32_t i = b + tid;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t x = v;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= d) x += o;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        uint32_t wbase = 0, total = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            const uint32_t t = s_wave[w];
+            if (w < wave) wbase += t;
+            total += t;
+        }
+        const uint32_t carry = *s_carry;
+        if (i < n) {
+            const uint32_t excl = carry + wbase + x - v;
+            out[i] = excl;
+            if (cursor) cursor[i] = excl;
+        }
+        __syncthreads();
+        if (tid == 0) *s_carry = carry + total;
+        __syncthreads();
+    }
+    return *s_carry;
 }
 
 __global__ void kz_fill_u32(uint32_t *dst, uint64_t n, uint32_t value) {
@@ -95,77 +122,256 @@ __global__ void kz_fill_rect_u32(uint32_t *dst, uint32_t width, int32_t x0, int3
     }
 }
 
-// Exclusive prefix sum of counts[0, n) into offsets and cursor (n from
-// n_dev when given). Adds the total to *total_out and, when `capacity` is
-// exceeded, counts an overflow.
+// Exclusive prefix sum of counts[0, n) into offsets and cursor. Adds the
+// total to the list-entry statistic and, when `capacity` is exceeded,
+// counts an overflow.
 __global__ __launch_bounds__(kScanThreads) void kz_scan(const uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
-                                                        uint32_t n_host, const uint32_t *n_dev, uint32_t capacity,
-                                                        unsigned long long *stats, uint32_t *total_out) {
+                                                        uint32_t n, uint32_t capacity, unsigned long long *stats) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_carry;
-    const uint32_t n = n_dev ? *n_dev : n_host;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t b = 0; b < n; b += kScanThreads) {
-        const uint32_t i = b + tid;
-        const uint32_t v = i < n ? counts[i] : 0u;
-        uint32_t x = v;
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(x, d, 64);
-            if (lane >= d) x += o;
-        }
-        if (lane == 63) s_wave[wave] = x;
-        __syncthreads();
-        uint32_t wbase = 0, total = 0;
-        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
-            const uint32_t t = s_wave[w];
-            if (w < wave) wbase += t;
-            total += t;
-        }
-        const uint32_t carry = s_carry;
-        if (i < n) {
-            const uint32_t excl = carry + wbase + x - v;
-            offsets[i] = excl;
-            if (cursor) cursor[i] = excl;
-        }
-        __syncthreads();
-        if (tid == 0) s_carry = carry + total;
-        __syncthreads();
-    }
-    if (tid == 0 && total_out) *total_out = s_carry;
-    if (tid == 0 && stats) {
-        atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], (unsigned long long)s_carry);
-        if (s_carry > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
+    const uint32_t total = block_exclusive_scan(counts, offsets, cursor, n, s_wave, &s_carry);
+    if (threadIdx.x == 0 && stats) {
+        atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], (unsigned long long)total);
+        if (total > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
     }
 }
 
-// Static mode: one list per screen tile.
-__global__ __launch_bounds__(256) void kz_bin_scatter(KzFrame f) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t lane = lane_id();
-    const uint64_t r = g < f.total_tris ? f.tri_rect[g] : KZ_RECT_EMPTY;
-    uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;
-    if (r != KZ_RECT_EMPTY) {
-        uint32_t x0, y0, x1, y1;
-        unpack_box(r, x0, y0, x1, y1);
-        const uint32_t sh = f.tile_shift;
-        tx0 = x0 >> sh;
-        tx1 = x1 >> sh;
-        ty0 = y0 >> sh;
-        ty1 = y1 >> sh;
+// --- binning -----------------------------------------------------------
+
+// LDS open-addressing tables of 1 << LOG slots: keys[] (kNone = free) and
+// per-slot counters, linear probing over at most kHashProbes slots.
+template <uint32_t LOG>
+__device__ __forceinline__ uint32_t hash_home(uint32_t key) {
+    return (key * 0x9E3779B1u) >> (32 - LOG);
+}
+
+// Adds `add` to the counter of `key`; returns its slot, or kNone when the
+// key's probe window is full of other keys (the table is never emptied
+// while in use, so a key that did not fit never will).
+template <uint32_t LOG>
+__device__ __forceinline__ uint32_t hash_add(uint32_t *keys, uint32_t *vals, uint32_t key, uint32_t add) {
+    uint32_t h = hash_home<LOG>(key);
+    for (uint32_t p = 0; p < kHashProbes; ++p) {
+        uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == kNone) {
+            const uint32_t old = atomicCAS(&keys[h], kNone, key);
+            k = old == kNone ? key : old;
+        }
+        if (k == key) {
+            atomicAdd(&vals[h], add);
+            return h;
+        }
+        h = (h + 1) & ((1u << LOG) - 1);
     }
-    const uint32_t ntiles = tx0 <= tx1 ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    const uint32_t steps = wave_max_u32(ntiles);
-    uint32_t tx = tx0, ty = ty0;
-    for (uint32_t k = 0; k < steps; ++k) {
-        const bool act = k < ntiles;
-        const uint32_t slot = wave_reserve(f.tile_cursor, act ? ty * f.tiles_x + tx : kNone, lane);
-        if (act) {
-            if (slot < f.list_capacity) f.tile_lists[slot] = g;
-            if (++tx > tx1) {
-                tx = tx0;
-                ++ty;
+    return kNone;
+}
+
+// Slot of `key` in a table no longer being written, or kNone.
+template <uint32_t LOG>
+__device__ __forceinline__ uint32_t hash_find(const uint32_t *keys, uint32_t key) {
+    uint32_t h = hash_home<LOG>(key);
+    for (uint32_t p = 0; p < kHashProbes; ++p) {
+        const uint32_t k = keys[h];
+        if (k == key) return h;
+        if (k == kNone) return kNone;
+        h = (h + 1) & ((1u << LOG) - 1);
+    }
+    return kNone;
+}
+
+template <uint32_t LOG>
+__device__ __forceinline__ void hash_clear(uint32_t *keys, uint32_t *vals) {
+    for (uint32_t i = threadIdx.x; i < (1u << LOG); i += blockDim.x) {
+        keys[i] = kNone;
+        vals[i] = 0;
+    }
+}
+
+// The tiles of one pixel box, row-major, one per call to next(); (cx, cy)
+// is the tile next() returned last.
+struct TileWalk {
+    uint32_t x0, y0, x1, y1;
+    uint32_t tx0, tx1, tx, ty, n, k, cx, cy;
+    __device__ __forceinline__ void init(uint64_t r, uint32_t sh) {
+        n = k = 0;
+        tx0 = tx1 = tx = ty = cx = cy = 0;
+        x0 = y0 = x1 = y1 = 0;
+        if (r == KZ_RECT_EMPTY) return;
+        unpack_box(r, x0, y0, x1, y1);
+        tx0 = tx = x0 >> sh;
+        tx1 = x1 >> sh;
+        ty = y0 >> sh;
+        n = (tx1 - tx0 + 1) * ((y1 >> sh) - ty + 1);
+    }
+    __device__ __forceinline__ uint32_t next(uint32_t tiles_x) {
+        if (k >= n) return kNone;
+        cx = tx;
+        cy = ty;
+        ++k;
+        if (++tx > tx1) {
+            tx = tx0;
+            ++ty;
+        }
+        return cy * tiles_x + cx;
+    }
+    // pixels of the box inside tile (cx, cy)
+    __device__ __forceinline__ uint32_t overlap(uint32_t sh) const {
+        const uint32_t X0 = cx << sh, Y0 = cy << sh, X1 = X0 + (1u << sh) - 1, Y1 = Y0 + (1u << sh) - 1;
+        return (min(x1, X1) - max(x0, X0) + 1) * (min(y1, Y1) - max(y0, Y0) + 1);
+    }
+};
+
+// Runs of equal keys in adjacent lanes (consecutive triangles usually hit
+// the same tile): the first lane of a run is its leader and speaks for
+// `len` pairs; `head` is the leader's lane.
+struct Run {
+    bool leader;
+    uint32_t len, head;
+};
+
+__device__ __forceinline__ Run run_of(uint32_t key, uint32_t lane) {
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool boundary = lane == 0 || key != prev;
+    const uint64_t b = __ballot(boundary);
+    Run r;
+    r.leader = boundary && key != kNone;
+    const uint64_t after = lane == 63 ? 0ull : (b >> (lane + 1));
+    r.len = after ? (uint32_t)__builtin_ctzll(after) + 1u : 64u - lane;
+    r.head = 63u - (uint32_t)__builtin_clzll(b & ((2ull << lane) - 1ull));
+    return r;
+}
+
+// Pairs per tile. Workgroup b takes triangles [b * chunk, (b + 1) * chunk).
+__global__ __launch_bounds__(kBinThreads) void kz_bin_hist(const uint64_t *rect, uint32_t ntris, uint32_t chunk,
+                                                           uint32_t sh, uint32_t tiles_x, uint32_t *counts) {
+    __shared__ uint32_t s_key[kHashSize], s_val[kHashSize];
+    hash_clear<kHashLog>(s_key, s_val);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = blockIdx.x * chunk, t1 = min(t0 + chunk, ntris);
+    for (uint32_t b = t0; b < t1; b += kBinThreads) {
+        const uint32_t g = b + threadIdx.x;
+        TileWalk w;
+        w.init(g < t1 ? rect[g] : KZ_RECT_EMPTY, sh);
+        const uint32_t steps = wave_max_u32(w.n);
+        for (uint32_t s = 0; s < steps; ++s) {
+            const uint32_t key = w.next(tiles_x);
+            const Run r = run_of(key, lane);
+            if (r.leader && hash_add<kHashLog>(s_key, s_val, key, r.len) == kNone) atomicAdd(&counts[key], r.len);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHashSize; i += kBinThreads) {
+        const uint32_t v = s_val[i];
+        if (v) atomicAdd(&counts[s_key[i]], v);
+    }
+}
+
+// Adds one pair's statistics to counters c[j * stride] (LDS slot or HBM).
+__device__ __forceinline__ void acc_pair(uint32_t *c, uint32_t stride, uint32_t area, bool shared, uint32_t z) {
+    atomicAdd(&c[KZ_ACC_AREA * stride], area);
+    const uint32_t bin = area <= 2 ? 0u : area <= 8 ? 1u : area <= 32 ? 2u : area <= 128 ? 3u : 4u;
+    atomicAdd(&c[(KZ_ACC_HIST + bin) * stride], 1u);
+    if (shared) atomicAdd(&c[KZ_ACC_SHARED * stride], 1u);
+    atomicMax(&c[KZ_ACC_ZMIN * stride], 65535u - (z & 0xFFFFu));
+    atomicMax(&c[KZ_ACC_ZMAX * stride], z >> 16);
+}
+
+// kz_bin_hist plus the per-tile pair statistics of the cost model
+// (KZ_ACC_* in acc[tile * KZ_ACC + j], zeroed by the caller), on a smaller
+// table (KZ_ACC more counters per slot).
+__global__ __launch_bounds__(kBinThreads) void kz_bin_hist_acc(const uint64_t *rect, const uint32_t *tri_z,
+                                                               uint32_t ntris, uint32_t chunk, uint32_t sh,
+                                                               uint32_t tiles_x, uint32_t *counts, uint32_t *acc) {
+    __shared__ uint32_t s_key[kHashSizeAcc], s_val[kHashSizeAcc], s_acc[KZ_ACC * kHashSizeAcc];
+    hash_clear<kHashLogAcc>(s_key, s_val);
+    for (uint32_t i = threadIdx.x; i < KZ_ACC * kHashSizeAcc; i += kBinThreads) s_acc[i] = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = blockIdx.x * chunk, t1 = min(t0 + chunk, ntris);
+    for (uint32_t b = t0; b < t1; b += kBinThreads) {
+        const uint32_t g = b + threadIdx.x;
+        TileWalk w;
+        w.init(g < t1 ? rect[g] : KZ_RECT_EMPTY, sh);
+        const uint32_t z = w.n ? tri_z[g] : 0u;
+        const uint32_t steps = wave_max_u32(w.n);
+        for (uint32_t s = 0; s < steps; ++s) {
+            const uint32_t key = w.next(tiles_x);
+            const Run r = run_of(key, lane);
+            uint32_t slot = kNone;
+            if (r.leader) {
+                slot = hash_add<kHashLogAcc>(s_key, s_val, key, r.len);
+                if (slot == kNone) atomicAdd(&counts[key], r.len);
+            }
+            slot = (uint32_t)__shfl((int)slot, (int)r.head, 64);
+            if (key != kNone) {
+                if (slot != kNone)
+                    acc_pair(&s_acc[slot], kHashSizeAcc, w.overlap(sh), w.n > 1, z);
+                else
+                    acc_pair(&acc[(size_t)key * KZ_ACC], 1, w.overlap(sh), w.n > 1, z);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHashSizeAcc; i += kBinThreads) {
+        const uint32_t v = s_val[i];
+        if (v) {
+            uint32_t *dst = &acc[(size_t)s_key[i] * KZ_ACC];
+            atomicAdd(&counts[s_key[i]], v);
+            for (uint32_t j = 0; j < KZ_ACC_ZMIN; ++j)
+                if (s_acc[j * kHashSizeAcc + i]) atomicAdd(&dst[j], s_acc[j * kHashSizeAcc + i]);
+            atomicMax(&dst[KZ_ACC_ZMIN], s_acc[KZ_ACC_ZMIN * kHashSizeAcc + i]);
+            atomicMax(&dst[KZ_ACC_ZMAX], s_acc[KZ_ACC_ZMAX * kHashSizeAcc + i]);
+        }
+    }
+}
+
+// Writes triangle ids into the tile lists (positions within a list are in
+// no particular order; kz_sort_lists orders them). Same chunks as
+// kz_bin_hist: count per tile in LDS, reserve one range per (workgroup,
+// tile), then fill the ranges from LDS cursors.
+__global__ __launch_bounds__(kBinThreads) void kz_bin_scatter(const uint64_t *rect, uint32_t ntris, uint32_t chunk,
+                                                              uint32_t sh, uint32_t tiles_x, uint32_t *cursor,
+                                                              uint32_t *lists, uint32_t capacity) {
+    __shared__ uint32_t s_key[kHashSize], s_val[kHashSize];
+    hash_clear<kHashLog>(s_key, s_val);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = blockIdx.x * chunk, t1 = min(t0 + chunk, ntris);
+    for (uint32_t b = t0; b < t1; b += kBinThreads) {
+        const uint32_t g = b + threadIdx.x;
+        TileWalk w;
+        w.init(g < t1 ? rect[g] : KZ_RECT_EMPTY, sh);
+        const uint32_t steps = wave_max_u32(w.n);
+        for (uint32_t s = 0; s < steps; ++s) {
+            const uint32_t key = w.next(tiles_x);
+            const Run r = run_of(key, lane);
+            if (r.leader) hash_add<kHashLog>(s_key, s_val, key, r.len);  // keys that do not fit reserve in HBM below
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHashSize; i += kBinThreads) {
+        const uint32_t v = s_val[i];
+        if (v) s_val[i] = atomicAdd(&cursor[s_key[i]], v);
+    }
+    __syncthreads();
+    for (uint32_t b = t0; b < t1; b += kBinThreads) {
+        const uint32_t g = b + threadIdx.x;
+        TileWalk w;
+        w.init(g < t1 ? rect[g] : KZ_RECT_EMPTY, sh);
+        const uint32_t steps = wave_max_u32(w.n);
+        for (uint32_t s = 0; s < steps; ++s) {
+            const uint32_t key = w.next(tiles_x);
+            const Run r = run_of(key, lane);
+            uint32_t base = 0;
+            if (r.leader) {
+                const uint32_t slot = hash_find<kHashLog>(s_key, key);
+                base = slot != kNone ? atomicAdd(&s_val[slot], r.len) : atomicAdd(&cursor[key], r.len);
+            }
+            base = (uint32_t)__shfl((int)base, (int)r.head, 64);
+            if (key != kNone) {
+                const uint32_t pos = base + (lane - r.head);
+                if (pos < capacity) lists[pos] = g;
             }
         }
     }
@@ -227,12 +433,11 @@ __device__ void merge_pass(const uint32_t *src, uint32_t *dst, uint32_t n, uint3
     __syncthreads();
 }
 
-// Sorts lists [0, n) (n from n_dev when given), grid-stride over lists.
+// Sorts lists [0, n), grid-stride over lists.
 __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *counts, const uint32_t *offsets,
-                                                               uint32_t n_host, const uint32_t *n_dev,
-                                                               uint32_t *lists, uint32_t *tmp, uint32_t capacity) {
+                                                               uint32_t nl, uint32_t *lists, uint32_t *tmp,
+                                                               uint32_t capacity) {
     __shared__ uint32_t s[kSortCap];
-    const uint32_t nl = n_dev ? *n_dev : n_host;
     for (uint32_t l = blockIdx.x; l < nl; l += gridDim.x) {
         const uint32_t n = counts[l];
         const uint32_t off = offsets[l];
@@ -259,126 +464,607 @@ __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *co
 
 // --- load-balanced mode -------------------------------------------------
 
-// Split level and item count per tile, from the predicted cost (or, with no
-// cost model, the binned triangle count).
-__global__ void kz_plan_levels(KzFrame f, uint32_t ntiles) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    const float cost = f.tile_cost ? f.tile_cost[t] : (float)f.tile_counts[t];
+// Split level of a non-empty tile from its cost in list entries: the binned
+// triangle count, plus the model's predicted fragments weighted as entries
+// when a cost model ran. A tile whose list overflowed stays whole: its
+// triangles come from the in-order scan.
+__device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uint32_t n, float &cost) {
+    cost = (float)n + (f.tile_pred ? f.frag_weight * f.tile_pred[t] : 0.0f);
     uint32_t level = cost > KZ_SPLIT_16 ? 2u : cost > KZ_SPLIT_4 ? 1u : 0u;
-    if (f.tile_counts[t] == 0) level = 0;
-    while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px
-    f.tile_level[t] = level;
-    f.tile_item_base[t] = f.tile_counts[t] ? (1u << (2 * level)) : 0u;  // items per tile, scanned next
+    while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px wide
+    if ((uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
+    return level;
 }
 
-// Writes the item geometry table; items of empty tiles are not created.
-__global__ void kz_plan_emit(KzFrame f, uint32_t ntiles) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles || f.tile_counts[t] == 0) return;
-    const uint32_t level = f.tile_level[t], base = f.tile_item_base[t];
-    for (uint32_t s = 0; s < (1u << (2 * level)); ++s) {
-        f.items[base + s] = KZ_ITEM_PACK(t, level, s);
-        f.item_counts[base + s] = 0;
-    }
+// floor(log2(cost + 1)), the LPT bucket of an item
+__device__ __forceinline__ uint32_t cost_bucket(float cost) {
+    const float c = fminf(fmaxf(cost, 0.0f), 2.0e9f);
+    return 31u - __clz((uint32_t)c + 1u);
 }
 
-// Walks the work items a triangle's pixel box overlaps, one per call.
-struct ItemWalk {
-    uint32_t px0, py0, px1, py1;
-    uint32_t tx0, tx1, ty1, tx, ty;
-    uint32_t sx, sy, sx0, sx1, sy1, base, level;
-    bool done;
-
-    __device__ void enter(const KzFrame &f) {
-        const uint32_t t = ty * f.tiles_x + tx;
-        level = f.tile_level[t];
-        base = f.tile_item_base[t];
-        const uint32_t sh = f.tile_shift - level;  // sub-tile side = 1 << sh
-        const uint32_t X0 = tx << f.tile_shift, Y0 = ty << f.tile_shift;
-        const uint32_t last = (1u << level) - 1;
-        sx0 = px0 > X0 ? (px0 - X0) >> sh : 0;
-        sx1 = min((px1 - X0) >> sh, last);
-        sy = py0 > Y0 ? (py0 - Y0) >> sh : 0;
-        sy1 = min((py1 - Y0) >> sh, last);
-        sx = sx0;
-    }
-    __device__ void init(const KzFrame &f, uint64_t r) {
-        done = r == KZ_RECT_EMPTY;
-        if (done) return;
-        unpack_box(r, px0, py0, px1, py1);
-        tx0 = px0 >> f.tile_shift;
-        tx1 = px1 >> f.tile_shift;
-        ty1 = py1 >> f.tile_shift;
-        tx = tx0;
-        ty = py0 >> f.tile_shift;
-        enter(f);
-    }
-    __device__ uint32_t next(const KzFrame &f) {
-        if (done) return kNone;
-        const uint32_t item = base + (sy << level) + sx;
-        if (++sx > sx1) {
-            sx = sx0;
-            if (++sy > sy1) {
-                if (++tx > tx1) {
-                    tx = tx0;
-                    if (++ty > ty1) done = true;
-                }
-                if (!done) enter(f);
-            }
+// One workgroup: split levels, split slots and their chunk ranges, and the
+// queue of all items ordered by bucket, most expensive first (order within
+// a bucket is arbitrary; pixels never depend on it).
+__global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntiles) {
+    __shared__ uint32_t s_bucket[32], s_cursor[32], s_split, s_chunks;
+    if (threadIdx.x < 32) s_bucket[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_split = s_chunks = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
+        const uint32_t n = f.tile_counts[t];
+        if (n == 0) continue;
+        float cost;
+        const uint32_t level = plan_level(f, t, n, cost);
+        f.tile_level[t] = level;
+        const uint32_t items = 1u << (2 * level);
+        atomicAdd(&s_bucket[cost_bucket(cost / (float)items)], items);
+        if (level) {
+            const uint32_t slot = atomicAdd(&s_split, 1u);
+            f.split_slot[t] = slot;
+            f.split_info[2 * slot] = atomicAdd(&s_chunks, (n + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
+            f.split_info[2 * slot + 1] = t;
         }
-        return item;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int b = 31; b >= 0; --b) {
+            s_cursor[b] = s;
+            s += s_bucket[b];
+        }
+        f.ctrl[KZ_CTRL_HEAD] = 0;
+        f.ctrl[KZ_CTRL_ITEMS] = s;
+        f.ctrl[KZ_CTRL_SPLITS] = s_split;
+        f.ctrl[KZ_CTRL_CHUNKS] = min(s_chunks, f.max_chunks);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
+        const uint32_t n = f.tile_counts[t];
+        if (n == 0) continue;
+        float cost;
+        const uint32_t level = plan_level(f, t, n, cost);
+        const uint32_t items = 1u << (2 * level);
+        const uint32_t pos = atomicAdd(&s_cursor[cost_bucket(cost / (float)items)], items);
+        for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t, level, s);
+        if (level) {
+            const uint32_t c0 = f.split_info[2 * f.split_slot[t]];
+            const uint32_t nc = (n + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+            for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) f.chunk_tile[c0 + j] = t;
+        }
+    }
+}
+
+// A KZ_SPLIT_CHUNK-entry piece of a split tile's (sorted) list.
+struct Chunk {
+    const uint32_t *ids;
+    uint32_t n, nsub, level, ssh;
+    int32_t X0, Y0;
 };
 
-__global__ __launch_bounds__(256) void kz_item_count(KzFrame f) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ Chunk chunk_at(const KzFrame &f, uint32_t c) {
+    Chunk k;
+    const uint32_t t = f.chunk_tile[c];
+    const uint32_t j = c - f.split_info[2 * f.split_slot[t]];
+    const uint32_t n = f.tile_counts[t];
+    k.ids = f.tile_lists + f.tile_offsets[t] + j * KZ_SPLIT_CHUNK;
+    k.n = min(KZ_SPLIT_CHUNK, n - j * KZ_SPLIT_CHUNK);
+    k.level = f.tile_level[t];
+    k.nsub = 1u << (2 * k.level);
+    k.ssh = f.tile_shift - k.level;
+    k.X0 = (int32_t)((t % f.tiles_x) << f.tile_shift);
+    k.Y0 = (int32_t)((t / f.tiles_x) << f.tile_shift);
+    return k;
+}
+
+// Sub-tiles (bit sy << level | sx) of a split tile that a pixel box overlaps.
+__device__ __forceinline__ uint32_t sub_mask(uint64_t r, const Chunk &k) {
+    uint32_t x0, y0, x1, y1;
+    unpack_box(r, x0, y0, x1, y1);
+    const int32_t last = (1 << k.level) - 1;
+    const int32_t sx0 = max((int32_t)x0 - k.X0, 0) >> k.ssh, sx1 = min(((int32_t)x1 - k.X0) >> k.ssh, last);
+    const int32_t sy0 = max((int32_t)y0 - k.Y0, 0) >> k.ssh, sy1 = min(((int32_t)y1 - k.Y0) >> k.ssh, last);
+    const uint32_t row = (2u << sx1) - (1u << sx0);
+    uint32_t m = 0;
+    for (int32_t sy = sy0; sy <= sy1; ++sy) m |= row << (sy << k.level);
+    return m;
+}
+
+// Entries per sub-tile of every chunk (one wave per chunk).
+__global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     const uint32_t lane = lane_id();
-    ItemWalk w;
-    w.init(f, g < f.total_tris ? f.tri_rect[g] : KZ_RECT_EMPTY);
-    while (__ballot(!w.done)) wave_count(f.item_counts, w.next(f), lane);
+    const uint32_t nch = f.ctrl[KZ_CTRL_CHUNKS];
+    const uint32_t waves = gridDim.x * (kSplitThreads / 64);
+    for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
+        const Chunk k = chunk_at(f, c);
+        uint32_t cnt = 0;  // lane s < 16: entries of sub-tile s
+        for (uint32_t e0 = 0; e0 < k.n; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const uint32_t m = e < k.n ? sub_mask(f.tri_rect[k.ids[e]], k) : 0u;
+            for (uint32_t s = 0; s < k.nsub; ++s) {
+                const uint32_t c_s = (uint32_t)__popcll(__ballot((m >> s) & 1u));
+                if (lane == s) cnt += c_s;
+            }
+        }
+        if (lane < 16) f.chunk_counts[c * 16 + lane] = cnt;
+    }
 }
 
-__global__ __launch_bounds__(256) void kz_item_scatter(KzFrame f) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup: sub-tile list sizes and offsets, then each chunk's write
+// base per sub-tile (in place of its counts).
+__global__ __launch_bounds__(kScanThreads) void kz_split_offsets(KzFrame f) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_carry;
+    const uint32_t n_items = f.ctrl[KZ_CTRL_SPLITS] * 16;
+    for (uint32_t i = threadIdx.x; i < n_items; i += kScanThreads) {
+        const uint32_t slot = i >> 4, s = i & 15;
+        const uint32_t c0 = f.split_info[2 * slot], t = f.split_info[2 * slot + 1];
+        const uint32_t nc = (f.tile_counts[t] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+        uint32_t total = 0;
+        if (s < (1u << (2 * f.tile_level[t])))
+            for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) total += f.chunk_counts[(c0 + j) * 16 + s];
+        f.split_counts[i] = total;
+    }
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(f.split_counts, f.split_offsets, nullptr, n_items, s_wave, &s_carry);
+    if (threadIdx.x == 0) {
+        atomicAdd(&f.stats[KZ_STAT_SPLIT_ENTRIES], (unsigned long long)total);
+        atomicMax(&f.ctrl[KZ_CTRL_SPLIT_MAX], total);
+    }
+    for (uint32_t i = threadIdx.x; i < n_items; i += kScanThreads) {
+        const uint32_t slot = i >> 4, s = i & 15;
+        const uint32_t c0 = f.split_info[2 * slot], t = f.split_info[2 * slot + 1];
+        const uint32_t nc = (f.tile_counts[t] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+        if (s >= (1u << (2 * f.tile_level[t]))) continue;
+        uint32_t run = f.split_offsets[i];
+        for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) {
+            uint32_t *p = &f.chunk_counts[(c0 + j) * 16 + s];
+            const uint32_t v = *p;
+            *p = run;
+            run += v;
+        }
+    }
+}
+
+// Writes the sub-tile lists: entries keep their list order, so each
+// sub-tile list is in primitive order.
+__global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
     const uint32_t lane = lane_id();
-    ItemWalk w;
-    w.init(f, g < f.total_tris ? f.tri_rect[g] : KZ_RECT_EMPTY);
-    while (__ballot(!w.done)) {
-        const uint32_t slot = wave_reserve(f.item_cursor, w.next(f), lane);
-        if (slot != kNone && slot < f.list_capacity) f.tile_lists[slot] = g;
+    const uint32_t nch = f.ctrl[KZ_CTRL_CHUNKS];
+    const uint32_t waves = gridDim.x * (kSplitThreads / 64);
+    for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
+        const Chunk k = chunk_at(f, c);
+        uint32_t base = lane < 16 ? f.chunk_counts[c * 16 + lane] : 0u;  // lane s: next slot of sub-tile s
+        for (uint32_t e0 = 0; e0 < k.n; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            uint32_t id = 0, m = 0;
+            if (e < k.n) {
+                id = k.ids[e];
+                m = sub_mask(f.tri_rect[id], k);
+            }
+            for (uint32_t s = 0; s < k.nsub; ++s) {
+                const uint64_t bal = __ballot((m >> s) & 1u);
+                const uint32_t b = readlane(base, s);
+                if ((m >> s) & 1u) {
+                    const uint32_t pos = b + rank_in(bal);
+                    if (pos < f.split_capacity) f.split_lists[pos] = id;
+                }
+                if (lane == s) base += (uint32_t)__popcll(bal);
+            }
+        }
     }
 }
 
-// LPT queue: items bucketed by floor(log2(count + 1)), largest bucket first.
-// ctrl layout: [0] queue head, [1] item count, [2, 34) bucket counts,
-// [34, 66) bucket cursors.
-__device__ __forceinline__ uint32_t cost_bucket(uint32_t count) { return 31u - (31u - __clz(count + 1)); }
+// --- learned cost model -------------------------------------------------
 
-__global__ __launch_bounds__(256) void kz_queue_count(KzFrame f, uint32_t *ctrl) {
-    const uint32_t n = ctrl[1], lane = lane_id();
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        wave_count(ctrl + 2, i < n ? cost_bucket(f.item_counts[i]) : kNone, lane);
+// The 16 features of a tile, from its binned-triangle count, its pair
+// statistics and the pipeline (the model normalises them):
+//   0     log2(1 + binned triangles)
+//   1     log2(1 + summed box-tile overlap pixels)
+//   2..6  overlap-area histogram (<= 2, 8, 32, 128, more pixels), as
+//         fractions of the pairs
+//   7     fraction of pairs whose triangle spans several tiles
+//   8     z range of the tile's triangles (max z - min z, [0, 1])
+//   9..14 pipeline: log2(samples) / 2, blend, depth test, varyings / 16,
+//         textures / 4, uniform buffers / 4
+//   15    fraction of the tile inside the framebuffer
+__global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint32_t cnt = f.tile_counts[t];
+    const float n = (float)cnt;
+    const uint32_t *a = f.tile_acc + (size_t)t * KZ_ACC;
+    const float inv = cnt ? 1.0f / n : 0.0f;
+    const uint32_t ts = 1u << f.tile_shift, p = f.pipe_info;
+    const uint32_t X0 = (t % f.tiles_x) * ts, Y0 = (t / f.tiles_x) * ts;
+    const float inside = (float)(min(ts, f.width - X0) * min(ts, f.height - Y0)) / (float)(ts * ts);
+    float *o = f.tile_feat + (size_t)t * KZ_FEATURES;
+    o[0] = log2f(1.0f + n);
+    o[1] = log2f(1.0f + (float)a[KZ_ACC_AREA]);
+    for (uint32_t b = 0; b < 5; ++b) o[2 + b] = (float)a[KZ_ACC_HIST + b] * inv;
+    o[7] = (float)a[KZ_ACC_SHARED] * inv;
+    const float zmin = (float)(65535u - a[KZ_ACC_ZMIN]), zmax = (float)a[KZ_ACC_ZMAX];
+    o[8] = cnt ? fmaxf(zmax - zmin, 0.0f) * (1.0f / 65535.0f) : 0.0f;
+    o[9] = 0.5f * (float)KZ_PIPE_LOG2_SAMPLES(p);
+    o[10] = (float)KZ_PIPE_BLEND(p);
+    o[11] = (float)KZ_PIPE_DEPTH(p);
+    o[12] = (float)KZ_PIPE_VARYINGS(p) * (1.0f / 16.0f);
+    o[13] = (float)KZ_PIPE_TEXTURES(p) * 0.25f;
+    o[14] = (float)KZ_PIPE_UBOS(p) * 0.25f;
+    o[15] = inside;
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// f32 -> bf16 bits, round to nearest even (inputs are finite)
+__device__ __forceinline__ short to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+constexpr uint32_t kMlpThreads = 256;
+
+// One 64-unit layer for a wave's 16 rows (tiles or samples): in (bf16
+// [row][K], row stride in_stride) x W (bf16 [64][K], transposed) on MFMA
+// 16x16x16 bf16 with fp32 accumulation, + bias, ReLU, to bf16 out ([row][64];
+// may alias `in`: every read precedes every write). Operand layouts of
+// v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l % 16][4 (l / 16) + j] and
+// B[4 (l / 16) + j][l % 16]; it receives D[4 (l / 16) + i][l % 16].
+template <uint32_t K>
+__device__ __forceinline__ void mlp_layer(const short *in, uint32_t in_stride, short *out, const short *w,
+                                          const float *bias, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+    f32x4 d[4];
+#pragma unroll
+    for (uint32_t nb = 0; nb < 4; ++nb) {
+        d[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t kb = 0; kb < K / 16; ++kb) {
+            bf16x4 a, b;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                a[j] = in[row * in_stride + kb * 16 + quad * 4 + j];
+                b[j] = w[(nb * 16 + row) * K + kb * 16 + quad * 4 + j];
+            }
+            d[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d[nb], 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t unit = nb * 16 + row;
+            out[(quad * 4 + i) * KZ_HIDDEN + unit] = to_bf16(fmaxf(d[nb][i] + bias[unit], 0.0f));
+        }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Predicted fragments per tile: pred[t] = max(exp2(MLP(features)) - 1, 0).
+// Each wave evaluates 16 tiles at a time; weights are staged in LDS once per
+// workgroup.
+__global__ __launch_bounds__(kMlpThreads) void kz_cost_mlp(const KzCostModel *m, const float *feat, uint32_t ntiles,
+                                                           float *pred) {
+    __shared__ short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN], s_w3[KZ_HIDDEN * KZ_HIDDEN];
+    __shared__ float s_b[3][KZ_HIDDEN], s_w4[KZ_HIDDEN];
+    __shared__ short s_act[kMlpThreads / 64][16 * KZ_HIDDEN];
+    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_FEATURES; i += kMlpThreads) s_w1[i] = (short)m->w1[i];
+    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_HIDDEN; i += kMlpThreads) {
+        s_w2[i] = (short)m->w2[i];
+        s_w3[i] = (short)m->w3[i];
+    }
+    if (threadIdx.x < KZ_HIDDEN) {
+        s_b[0][threadIdx.x] = m->b1[threadIdx.x];
+        s_b[1][threadIdx.x] = m->b2[threadIdx.x];
+        s_b[2][threadIdx.x] = m->b3[threadIdx.x];
+        s_w4[threadIdx.x] = m->w4[threadIdx.x];
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t row = lane & 15, quad = lane >> 4;
+    short *act = s_act[wave];
+    const uint32_t stride = gridDim.x * (kMlpThreads / 64) * 16;
+    for (uint32_t base = (blockIdx.x * (kMlpThreads / 64) + wave) * 16; base < ntiles; base += stride) {
+        // normalised features of tile base + row, 4 per lane, as layer-1 input
+        const uint32_t t = base + row;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t k = quad * 4 + j;
+            const float x = t < ntiles ? (feat[(size_t)t * KZ_FEATURES + k] - m->feat_mean[k]) * m->feat_scale[k]
+                                       : 0.0f;
+            act[row * KZ_HIDDEN + k] = to_bf16(x);
+        }
+        __builtin_amdgcn_wave_barrier();
+        mlp_layer<KZ_FEATURES>(act, KZ_HIDDEN, act, s_w1, s_b[0], lane);
+        mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, s_w2, s_b[1], lane);
+        mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, s_w3, s_b[2], lane);
+        // output unit: lanes row, row + 16, row + 32, row + 48 share tile `row`
+        float y = 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t u = quad * 16 + j;
+            y += __uint_as_float((uint32_t)(unsigned short)act[row * KZ_HIDDEN + u] << 16) * s_w4[u];
+        }
+        y += __shfl_xor(y, 16, 64);
+        y += __shfl_xor(y, 32, 64);
+        if (quad == 0 && t < ntiles) pred[t] = fmaxf(exp2f(y + m->b4) - 1.0f, 0.0f);
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-__global__ void kz_queue_scan(uint32_t *ctrl) {
-    if (threadIdx.x != 0) return;
-    uint32_t s = 0;
-    for (uint32_t b = 0; b < 32; ++b) {
-        ctrl[34 + b] = s;
-        s += ctrl[2 + b];
-    }
-    ctrl[0] = 0;
+// --- cost model training (one workgroup, MFMA forward and backward) ------
+
+constexpr uint32_t kTrainThreads = 256;
+
+__device__ __forceinline__ float from_bf16(short h) { return __uint_as_float((uint32_t)(unsigned short)h << 16); }
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(256) void kz_queue_emit(KzFrame f, uint32_t *ctrl) {
-    const uint32_t n = ctrl[1], lane = lane_id();
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        const uint32_t slot = wave_reserve(ctrl + 34, i < n ? cost_bucket(f.item_counts[i]) : kNone, lane);
-        if (i < n) f.queue[slot] = i;
+// g[mb][nb] += dZ^T x H over the wave's 16 rows: dW[u][i] = sum_s dz[s][u]
+// h[s][i] for 64 output units u and KIN inputs i. Block (mb, nb) lane l
+// holds dW[mb * 16 + 4 (l / 16) + r][nb * 16 + l % 16], r = 0..3.
+template <uint32_t KIN>
+__device__ __forceinline__ void accum_wgrad(const short *dz, const short *h, f32x4 *g, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+#pragma unroll
+    for (uint32_t mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (uint32_t nb = 0; nb < KIN / 16; ++nb) {
+            bf16x4 a, b;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                a[j] = dz[(quad * 4 + j) * KZ_HIDDEN + mb * 16 + row];
+                b[j] = h[(quad * 4 + j) * KIN + nb * 16 + row];
+            }
+            g[mb * (KIN / 16) + nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, g[mb * (KIN / 16) + nb], 0, 0, 0);
+        }
+}
+
+// dH = dZ x W for the wave's 16 rows (W bf16 [64][64]): dh[nb] lane l holds
+// dH[4 (l / 16) + r][nb * 16 + l % 16].
+__device__ __forceinline__ void back_act(const short *dz, const short *w, f32x4 *dh, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+#pragma unroll
+    for (uint32_t nb = 0; nb < 4; ++nb) {
+        dh[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t kb = 0; kb < 4; ++kb) {
+            bf16x4 a, b;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                a[j] = dz[row * KZ_HIDDEN + kb * 16 + quad * 4 + j];
+                b[j] = w[(kb * 16 + quad * 4 + j) * KZ_HIDDEN + nb * 16 + row];
+            }
+            dh[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, dh[nb], 0, 0, 0);
+        }
+    }
+}
+
+// dZ = dH * relu'(H) in the D layout of back_act, to bf16 dz [row][64];
+// adds the bias gradient partials (unit nb * 16 + l % 16) to gb[nb].
+__device__ __forceinline__ void relu_back(const f32x4 *dh, const short *h, short *dz, float *gb, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+#pragma unroll
+    for (uint32_t nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t s = quad * 4 + r, u = nb * 16 + row;
+            const float d = h[s * KZ_HIDDEN + u] > 0 ? dh[nb][r] : 0.0f;
+            gb[nb] += d;
+            dz[s * KZ_HIDDEN + u] = to_bf16(d);
+        }
+}
+
+// Adds a weight-gradient register block set to the LDS gradient sums.
+template <uint32_t KIN>
+__device__ __forceinline__ void flush_wgrad(const f32x4 *g, float *s_g, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+#pragma unroll
+    for (uint32_t mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (uint32_t nb = 0; nb < KIN / 16; ++nb)
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r)
+                atomicAdd(&s_g[(mb * 16 + quad * 4 + r) * KIN + nb * 16 + row], g[mb * (KIN / 16) + nb][r]);
+}
+
+// Minibatch Adam on the squared error of the MLP's output against y, the
+// forward pass in the same bf16/fp32 arithmetic as kz_cost_mlp. state holds
+// [W | M | V] (KZ_PARAMS each, layout KZ_P_*); x: normalised features
+// [n][16]; loss_out[step] = the step's mean squared error.
+__global__ __launch_bounds__(kTrainThreads) void kz_cost_train(float *state, const float *x, const float *y,
+                                                               KzTrainParams p, float *loss_out) {
+    float *W = state, *M = state + KZ_PARAMS, *V = state + 2 * KZ_PARAMS;
+    __shared__ short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN], s_w3[KZ_HIDDEN * KZ_HIDDEN];
+    __shared__ float s_b[3][KZ_HIDDEN], s_w4[KZ_HIDDEN], s_b4, s_loss;
+    __shared__ float s_g[KZ_PARAMS];
+    __shared__ short s_h0[kTrainThreads / 64][16 * KZ_FEATURES];
+    __shared__ short s_h[kTrainThreads / 64][3][16 * KZ_HIDDEN];
+    __shared__ short s_dz[kTrainThreads / 64][16 * KZ_HIDDEN];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t row = lane & 15, quad = lane >> 4;
+    for (uint32_t i = tid; i < KZ_PARAMS; i += kTrainThreads) {
+        const float w = W[i];
+        if (i < KZ_P_W2) s_w1[i - KZ_P_W1] = to_bf16(w);
+        else if (i < KZ_P_W3) s_w2[i - KZ_P_W2] = to_bf16(w);
+        else if (i < KZ_P_B1) s_w3[i - KZ_P_W3] = to_bf16(w);
+        else if (i < KZ_P_W4) s_b[(i - KZ_P_B1) / KZ_HIDDEN][(i - KZ_P_B1) % KZ_HIDDEN] = w;
+        else if (i < KZ_P_B4) s_w4[i - KZ_P_W4] = w;
+        else s_b4 = w;
+        s_g[i] = 0.0f;
+    }
+    if (tid == 0) s_loss = 0.0f;
+    __syncthreads();
+    short *h0 = s_h0[wave], *h1 = s_h[wave][0], *h2 = s_h[wave][1], *h3 = s_h[wave][2], *dz = s_dz[wave];
+
+    for (uint32_t step = 0; step < p.steps; ++step) {
+        f32x4 g3[16], g2[16], g1[4];
+        for (uint32_t i = 0; i < 16; ++i) g3[i] = g2[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t i = 0; i < 4; ++i) g1[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        float gb3[16], gw4[16], gb2[4] = {0, 0, 0, 0}, gb1[4] = {0, 0, 0, 0}, gb4 = 0.0f, loss = 0.0f;
+        for (uint32_t j = 0; j < 16; ++j) gb3[j] = gw4[j] = 0.0f;
+
+        for (uint32_t tile = 0; tile < p.batch / 64; ++tile) {  // 16 samples per wave per tile
+            // this lane's sample (row), drawn with replacement
+            const uint64_t k = ((uint64_t)(p.step0 + step) * p.batch) + (tile * 4 + wave) * 16 + row;
+            const uint32_t idx = (uint32_t)(splitmix64(p.seed ^ (k * 0x632BE59BD9B4E019ull)) % p.n);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                h0[row * KZ_FEATURES + quad * 4 + j] = to_bf16(x[(size_t)idx * KZ_FEATURES + quad * 4 + j]);
+            const float target = y[idx];
+            __builtin_amdgcn_wave_barrier();
+            mlp_layer<KZ_FEATURES>(h0, KZ_FEATURES, h1, s_w1, s_b[0], lane);
+            mlp_layer<KZ_HIDDEN>(h1, KZ_HIDDEN, h2, s_w2, s_b[1], lane);
+            mlp_layer<KZ_HIDDEN>(h2, KZ_HIDDEN, h3, s_w3, s_b[2], lane);
+            // output, error and dL/dy for sample `row` (lanes row + 16 q share it)
+            float part = 0.0f;
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) part += from_bf16(h3[row * KZ_HIDDEN + quad * 16 + j]) * s_w4[quad * 16 + j];
+            part += __shfl_xor(part, 16, 64);
+            part += __shfl_xor(part, 32, 64);
+            const float err = part + s_b4 - target;
+            const float dy = 2.0f * err / (float)p.batch;
+            if (quad == 0) {
+                loss += err * err;
+                gb4 += dy;
+            }
+            // layer 4 and dZ3 (lane: sample row, units quad * 16 + j)
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t u = quad * 16 + j;
+                const float h = from_bf16(h3[row * KZ_HIDDEN + u]);
+                gw4[j] += dy * h;
+                const float d = h > 0.0f ? dy * s_w4[u] : 0.0f;
+                gb3[j] += d;
+                dz[row * KZ_HIDDEN + u] = to_bf16(d);
+            }
+            __builtin_amdgcn_wave_barrier();
+            f32x4 dh[4];
+            accum_wgrad<KZ_HIDDEN>(dz, h2, g3, lane);
+            back_act(dz, s_w3, dh, lane);
+            __builtin_amdgcn_wave_barrier();
+            relu_back(dh, h2, dz, gb2, lane);
+            __builtin_amdgcn_wave_barrier();
+            accum_wgrad<KZ_HIDDEN>(dz, h1, g2, lane);
+            back_act(dz, s_w2, dh, lane);
+            __builtin_amdgcn_wave_barrier();
+            relu_back(dh, h1, dz, gb1, lane);
+            __builtin_amdgcn_wave_barrier();
+            accum_wgrad<KZ_FEATURES>(dz, h0, g1, lane);
+            __builtin_amdgcn_wave_barrier();
+        }
+
+        // the wave's gradient sums into LDS
+        flush_wgrad<KZ_HIDDEN>(g3, s_g + KZ_P_W3, lane);
+        flush_wgrad<KZ_HIDDEN>(g2, s_g + KZ_P_W2, lane);
+        flush_wgrad<KZ_FEATURES>(g1, s_g + KZ_P_W1, lane);
+#pragma unroll
+        for (uint32_t j = 0; j < 16; ++j) {
+            float a = gb3[j], b = gw4[j];
+            for (uint32_t d = 1; d < 16; d <<= 1) {
+                a += __shfl_xor(a, d, 64);
+                b += __shfl_xor(b, d, 64);
+            }
+            if (row == 0) {
+                atomicAdd(&s_g[KZ_P_B3 + quad * 16 + j], a);
+                atomicAdd(&s_g[KZ_P_W4 + quad * 16 + j], b);
+            }
+        }
+#pragma unroll
+        for (uint32_t nb = 0; nb < 4; ++nb) {
+            float a = gb2[nb], b = gb1[nb];
+            a += __shfl_xor(a, 16, 64);
+            a += __shfl_xor(a, 32, 64);
+            b += __shfl_xor(b, 16, 64);
+            b += __shfl_xor(b, 32, 64);
+            if (quad == 0) {
+                atomicAdd(&s_g[KZ_P_B2 + nb * 16 + row], a);
+                atomicAdd(&s_g[KZ_P_B1 + nb * 16 + row], b);
+            }
+        }
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            gb4 += __shfl_xor(gb4, d, 64);
+            loss += __shfl_xor(loss, d, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&s_g[KZ_P_B4], gb4);
+            atomicAdd(&s_loss, loss);
+        }
+        __syncthreads();
+
+        // Adam, then the bf16 / fp32 copies the next step computes with
+        const float t = (float)(p.step0 + step + 1);
+        const float c1 = 1.0f / (1.0f - powf(p.beta1, t)), c2 = 1.0f / (1.0f - powf(p.beta2, t));
+        for (uint32_t i = tid; i < KZ_PARAMS; i += kTrainThreads) {
+            const float g = s_g[i];
+            const float m = p.beta1 * M[i] + (1.0f - p.beta1) * g;
+            const float v = p.beta2 * V[i] + (1.0f - p.beta2) * g * g;
+            const float w = W[i] - p.lr * (m * c1) / (sqrtf(v * c2) + p.eps);
+            M[i] = m;
+            V[i] = v;
+            W[i] = w;
+            s_g[i] = 0.0f;
+            if (i < KZ_P_W2) s_w1[i - KZ_P_W1] = to_bf16(w);
+            else if (i < KZ_P_W3) s_w2[i - KZ_P_W2] = to_bf16(w);
+            else if (i < KZ_P_B1) s_w3[i - KZ_P_W3] = to_bf16(w);
+            else if (i < KZ_P_W4) s_b[(i - KZ_P_B1) / KZ_HIDDEN][(i - KZ_P_B1) % KZ_HIDDEN] = w;
+            else if (i < KZ_P_B4) s_w4[i - KZ_P_W4] = w;
+            else s_b4 = w;
+        }
+        if (tid == 0) {
+            if (loss_out) loss_out[step] = s_loss / (float)p.batch;
+            s_loss = 0.0f;
+        }
+        __syncthreads();
+    }
+}
+
+// --- training-data generator ----------------------------------------------
+
+// Output k of the seeded SplitMix64 stream `seed` (state seed + (k + 1) *
+// gamma), as a float in [0, 1): (u >> 40) * 2^-24.
+__device__ __forceinline__ double stream_f32(uint64_t seed, uint64_t k) {
+    const uint64_t u = splitmix64(seed + k * 0x9E3779B97F4A7C15ull);  // splitmix64 adds one more gamma
+    return (double)(float)((double)(u >> 40) * (1.0 / 16777216.0));
+}
+
+// The Gaussian-cluster scene (triangles 0 .. nc-1 around the centre with
+// sigma = sigma_frac * (W, H), the rest uniform; vertices at offsets
+// U[-radius, radius]^2 px; z and RGBA uniform), as pos4 + colour4 vertices,
+// in double precision like the host generator of the same seed.
+__global__ void kz_gen_cluster(float *v, uint32_t n, uint32_t nc, uint32_t width, uint32_t height, double sigma_frac,
+                               double radius, uint64_t seed) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double W = (double)width, H = (double)height;
+    const double u0 = stream_f32(seed, 4ull * t), u1 = stream_f32(seed, 4ull * t + 1);
+    const double u2 = stream_f32(seed, 4ull * t + 2), u3 = stream_f32(seed, 4ull * t + 3);
+    double cx, cy;
+    if (t < nc) {
+        const double r = sqrt(-2.0 * log(fmax(u0, 1e-12)));
+        const double a = 2.0 * 3.141592653589793 * u1;
+        cx = W / 2 + sigma_frac * W * r * cos(a);
+        cy = H / 2 + sigma_frac * H * r * sin(a);
+    } else {
+        cx = u2 * W;
+        cy = u3 * H;
+    }
+    for (uint32_t j = 0; j < 3; ++j) {
+        const uint64_t vi = 3ull * t + j;
+        const double wx = stream_f32(seed + 3, 6ull * t + 2 * j) * 2 - 1;
+        const double wy = stream_f32(seed + 3, 6ull * t + 2 * j + 1) * 2 - 1;
+        float *o = v + vi * 8;
+        o[0] = (float)((cx + wx * radius) * (2.0 / W) - 1.0);
+        o[1] = (float)((cy + wy * radius) * (2.0 / H) - 1.0);
+        o[2] = (float)stream_f32(seed + 5, vi);
+        o[3] = 1.0f;
+        for (uint32_t c = 0; c < 4; ++c) o[4 + c] = (float)stream_f32(seed + 6, 4 * vi + c);
     }
 }
 
@@ -416,6 +1102,16 @@ uint32_t grid_for(uint64_t n, uint32_t block) {
     return (uint32_t)(g == 0 ? 1 : g);
 }
 
+// Triangles per binning workgroup: enough workgroups to fill the device
+// twice over, at most 4096 triangles each (LDS table occupancy), at least
+// one full round of the workgroup.
+uint32_t bin_chunk(uint32_t ntris) {
+    uint32_t c = (ntris / 1024 + kBinThreads - 1) / kBinThreads * kBinThreads;
+    if (c < kBinThreads) c = kBinThreads;
+    if (c > 4096) c = 4096;
+    return c;
+}
+
 }  // namespace
 
 hipError_t kz_launch_fill_u32(hipStream_t st, uint32_t *dst, uint64_t n, uint32_t value) {
@@ -433,55 +1129,80 @@ hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width
 }
 
 hipError_t kz_launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          const uint32_t *n_dev, uint32_t capacity, unsigned long long *stats) {
-    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, n_dev, capacity, stats, nullptr);
+                          uint32_t capacity, unsigned long long *stats) {
+    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_bin_hist(hipStream_t st, const KzFrame &f) {
+    if (f.total_tris == 0) return hipSuccess;
+    const uint32_t chunk = bin_chunk(f.total_tris);
+    kz_bin_hist<<<(f.total_tris + chunk - 1) / chunk, kBinThreads, 0, st>>>(f.tri_rect, f.total_tris, chunk,
+                                                                             f.tile_shift, f.tiles_x, f.tile_counts);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_bin_hist_acc(hipStream_t st, const KzFrame &f) {
+    if (f.total_tris == 0) return hipSuccess;
+    const uint32_t chunk = min(bin_chunk(f.total_tris), 1024u);
+    kz_bin_hist_acc<<<(f.total_tris + chunk - 1) / chunk, kBinThreads, 0, st>>>(
+        f.tri_rect, f.tri_z, f.total_tris, chunk, f.tile_shift, f.tiles_x, f.tile_counts, f.tile_acc);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_tile_features(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
+    kz_tile_features<<<(ntiles + 255) / 256, 256, 0, st>>>(f, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_cost_mlp(hipStream_t st, const KzCostModel *model, const float *feat, uint32_t ntiles,
+                              float *pred) {
+    if (ntiles == 0) return hipSuccess;
+    const uint32_t per_block = (kMlpThreads / 64) * 16;
+    uint32_t grid = (ntiles + per_block - 1) / per_block;
+    if (grid > 1024) grid = 1024;
+    kz_cost_mlp<<<grid, kMlpThreads, 0, st>>>(model, feat, ntiles, pred);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_gen_cluster(hipStream_t st, float *vertices, uint32_t n, uint32_t nc, uint32_t width,
+                                 uint32_t height, double sigma_frac, double radius, uint64_t seed) {
+    if (n == 0) return hipSuccess;
+    kz_gen_cluster<<<(n + 255) / 256, 256, 0, st>>>(vertices, n, nc, width, height, sigma_frac, radius, seed);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_cost_train(hipStream_t st, float *state, const float *x, const float *y, const KzTrainParams &p,
+                                float *loss_out) {
+    if (p.n == 0 || p.batch == 0 || p.batch % 64 != 0) return hipErrorInvalidValue;
+    kz_cost_train<<<1, kTrainThreads, 0, st>>>(state, x, y, p, loss_out);
     return hipGetLastError();
 }
 
 hipError_t kz_launch_bin_scatter(hipStream_t st, const KzFrame &f) {
     if (f.total_tris == 0) return hipSuccess;
-    kz_bin_scatter<<<(f.total_tris + 255) / 256, 256, 0, st>>>(f);
+    const uint32_t chunk = bin_chunk(f.total_tris);
+    kz_bin_scatter<<<(f.total_tris + chunk - 1) / chunk, kBinThreads, 0, st>>>(
+        f.tri_rect, f.total_tris, chunk, f.tile_shift, f.tiles_x, f.tile_cursor, f.tile_lists, f.list_capacity);
     return hipGetLastError();
 }
 
-hipError_t kz_launch_sort_lists(hipStream_t st, const uint32_t *counts, const uint32_t *offsets, uint32_t n,
-                                const uint32_t *n_dev, uint32_t *lists, uint32_t *tmp, uint32_t capacity) {
-    const uint32_t grid = n < 8192u ? (n ? n : 1u) : 8192u;
-    kz_sort_lists<<<grid, kSortThreads, 0, st>>>(counts, offsets, n, n_dev, lists, tmp, capacity);
+hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
+    const uint32_t grid = ntiles < 8192u ? (ntiles ? ntiles : 1u) : 8192u;
+    kz_sort_lists<<<grid, kSortThreads, 0, st>>>(f.tile_counts, f.tile_offsets, ntiles, f.tile_lists, f.sort_tmp,
+                                                 f.list_capacity);
     return hipGetLastError();
 }
 
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *ctrl) {
-    const uint32_t g = (ntiles + 255) / 256;
-    kz_plan_levels<<<g, 256, 0, st>>>(f, ntiles);
-    // items per tile -> first item of each tile; the total lands in ctrl[1]
-    kz_scan<<<1, kScanThreads, 0, st>>>(f.tile_item_base, f.tile_item_base, nullptr, ntiles, nullptr, 0, nullptr,
-                                        ctrl + 1);
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
+    kz_plan<<<1, kPlanThreads, 0, st>>>(f, ntiles);
     return hipGetLastError();
 }
 
-hipError_t kz_launch_plan_emit(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
-    kz_plan_emit<<<(ntiles + 255) / 256, 256, 0, st>>>(f, ntiles);
-    return hipGetLastError();
-}
-
-hipError_t kz_launch_item_count(hipStream_t st, const KzFrame &f) {
-    if (f.total_tris == 0) return hipSuccess;
-    kz_item_count<<<(f.total_tris + 255) / 256, 256, 0, st>>>(f);
-    return hipGetLastError();
-}
-
-hipError_t kz_launch_item_scatter(hipStream_t st, const KzFrame &f) {
-    if (f.total_tris == 0) return hipSuccess;
-    kz_item_scatter<<<(f.total_tris + 255) / 256, 256, 0, st>>>(f);
-    return hipGetLastError();
-}
-
-hipError_t kz_launch_queue(hipStream_t st, const KzFrame &f, uint32_t *ctrl, uint32_t max_items) {
-    const uint32_t g = max_items / 256 + 1 < 1024 ? max_items / 256 + 1 : 1024;
-    kz_queue_count<<<g, 256, 0, st>>>(f, ctrl);
-    kz_queue_scan<<<1, 64, 0, st>>>(ctrl);
-    kz_queue_emit<<<g, 256, 0, st>>>(f, ctrl);
+hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid) {
+    kz_split_count<<<grid, kSplitThreads, 0, st>>>(f);
+    kz_split_offsets<<<1, kScanThreads, 0, st>>>(f);
+    kz_split_scatter<<<grid, kSplitThreads, 0, st>>>(f);
     return hipGetLastError();
 }
 

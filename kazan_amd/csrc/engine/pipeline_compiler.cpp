@@ -168,4 +168,32 @@ bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, 
     return true;
 }
 
+PipelineVariants::~PipelineVariants() {
+    for (PipelineKernels &k : k_)
+        if (k.module) hipModuleUnload(k.module);
+}
+
+const PipelineKernels *PipelineVariants::get(uint32_t tile_shift, std::string *err) {
+    if (!fits(tile_shift)) {
+        if (err) *err = "tile size does not fit the raster workgroup's LDS";
+        return nullptr;
+    }
+    if (!tried_[tile_shift]) {
+        tried_[tile_shift] = true;
+        FixedState fs = fixed;
+        fs.tile_shift = tile_shift;
+        std::vector<char> code;
+        std::string log;
+        if (!compile_pipeline_code(fs, shader_source, code, log, nullptr))
+            err_[tile_shift] = "kernel compilation failed:\n" + log;
+        else if (!load_pipeline_kernels(code, k_[tile_shift], err_[tile_shift]))
+            k_[tile_shift] = PipelineKernels{};
+    }
+    if (!k_[tile_shift].module) {
+        if (err) *err = err_[tile_shift];
+        return nullptr;
+    }
+    return &k_[tile_shift];
+}
+
 }  // namespace kz

@@ -252,21 +252,8 @@ __device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, 
 }
 
 // ---------------------------------------------------------------------------
-// binning, pass 1
-
-// One atomic per distinct tile per wave: lanes that hit the same tile in
-// this step elect a leader that adds their number. `tile` is ~0u for lanes
-// without a tile this step.
-__device__ __forceinline__ void kz_wave_count_tiles(uint32_t *counts, uint32_t tile, uint32_t lane) {
-    uint64_t pending = kz_ballot(tile != 0xFFFFFFFFu);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t t = (uint32_t)kz_readlane_i((int32_t)tile, leader);
-        const uint64_t peers = kz_ballot(tile == t) & pending;
-        if (lane == leader) atomicAdd(&counts[t], (uint32_t)__popcll(peers));
-        pending &= ~peers;
-    }
-}
+// binning, pass 1: vertex shader + setup of every triangle, to its pixel box
+// (the tile lists are built from the boxes by the prebuilt binning kernels)
 
 __device__ __forceinline__ unsigned long long *kz_stats_stripe(const KzFrame &f) {
     return f.stats + (size_t)(blockIdx.x & (KZ_STAT_STRIPES - 1)) * KZ_STAT_COUNT;
@@ -276,7 +263,6 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = kz_lane();
     uint32_t n_setup = 0, n_unsup = 0;
-    uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;
     if (g < f.total_tris) {
         float p[3][4], vary[3][KZ_NVA];
         kz_fetch_triangle(f, g, p, vary);
@@ -285,31 +271,19 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
         uint64_t rect = KZ_RECT_EMPTY;
         if (st == 1) {
             n_setup = 1;
-            if (t.px0 <= t.px1 && t.py0 <= t.py1) {
-                tx0 = (uint32_t)t.px0 >> KZ_TILE_SHIFT;
-                tx1 = (uint32_t)t.px1 >> KZ_TILE_SHIFT;
-                ty0 = (uint32_t)t.py0 >> KZ_TILE_SHIFT;
-                ty1 = (uint32_t)t.py1 >> KZ_TILE_SHIFT;
+            if (t.px0 <= t.px1 && t.py0 <= t.py1)
                 rect = (uint64_t)(uint32_t)t.px0 | ((uint64_t)(uint32_t)t.py0 << 16) |
                        ((uint64_t)(uint32_t)t.px1 << 32) | ((uint64_t)(uint32_t)t.py1 << 48);
+            if (f.tri_z) {  // cost-model feature: the triangle's depth range
+                const float z1 = t.z0 + t.dz1, z2 = t.z0 + t.dz2;
+                const float lo = fminf(fmaxf(fminf(t.z0, fminf(z1, z2)), 0.0f), 1.0f);
+                const float hi = fminf(fmaxf(fmaxf(t.z0, fmaxf(z1, z2)), 0.0f), 1.0f);
+                f.tri_z[g] = (uint32_t)(lo * 65535.0f + 0.5f) | ((uint32_t)(hi * 65535.0f + 0.5f) << 16);
             }
         } else if (st == 2) {
             n_unsup = 1;
         }
         f.tri_rect[g] = rect;
-    }
-    // walk every lane's tile rectangle in lock step, row-major
-    const bool has = tx0 <= tx1;
-    const uint32_t ntiles = has ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    const int32_t steps = kz_wave_max((int32_t)ntiles, lane);
-    uint32_t tx = tx0, ty = ty0;
-    for (int32_t k = 0; k < steps; ++k) {
-        const bool act = (uint32_t)k < ntiles;
-        kz_wave_count_tiles(f.tile_counts, act ? ty * f.tiles_x + tx : 0xFFFFFFFFu, lane);
-        if (act && ++tx > tx1) {
-            tx = tx0;
-            ++ty;
-        }
     }
     const uint64_t ms = kz_ballot(n_setup != 0), mu = kz_ballot(n_unsup != 0), mi = kz_ballot(g < f.total_tris);
     if (lane == 0) {
@@ -466,15 +440,17 @@ __device__ __forceinline__ uint32_t kz_pack_frag(uint32_t pix, uint32_t seq, uin
     return (pix << 10) | (seq << 4) | mask;
 }
 
-// Rasterises every triangle of one list into the square of side 1 << RS at
-// (X0, Y0) (a whole tile, or a sub-tile of the balanced mode). `list` holds
-// the n triangle ids overlapping the square, in primitive order; when it
-// overflowed, the triangles come from an in-order scan of all pixel boxes.
-__device__ void kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_t RS, uint32_t list_base,
-                               uint32_t n, uint32_t stats_tile, uint32_t *s_color, float *s_depth, uint32_t *s_claim,
-                               uint32_t *s_frag) {
+// Rasterises n triangles into the square of side 1 << RS at (X0, Y0) (a
+// whole tile, or a sub-tile of the balanced mode), in primitive order. The
+// triangles are ids[0, n) (ids = null: triangles 0 .. n-1); with `filter`,
+// only those whose pixel box overlaps the square (the fallback when a list
+// did not fit its storage: the parent tile's list, or every triangle).
+// Returns the square's fragment count (to every lane).
+__device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_t RS,
+                                             const uint32_t *ids, uint32_t n, bool filter, uint32_t *s_color,
+                                             float *s_depth, uint32_t *s_claim, uint32_t *s_frag) {
     const uint32_t lane = kz_lane();
-    if (n == 0) return;
+    if (n == 0) return 0;
     const int32_t RW = 1 << RS;
     const uint32_t *gcolor = (const uint32_t *)f.color;
 
@@ -493,24 +469,15 @@ __device__ void kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_
     }
     __syncthreads();
 
-    const uint32_t base = list_base;
-    // Entries past list_capacity were dropped by the scatter kernels; such a
-    // list takes its triangles from an in-order scan of all pixel boxes.
-    const bool overflow = (uint64_t)base + n > (uint64_t)f.list_capacity;
     unsigned long long nfrag = 0, npass = 0;
     uint32_t k = 0;
 
-    while (true) {
+    while (k < n) {
         uint32_t g = 0;
-        bool valid;
-        if (!overflow) {
-            if (k >= n) break;
-            valid = k + lane < n;
-            if (valid) g = f.tile_lists[base + k + lane];
-        } else {
-            if (k >= f.total_tris) break;
-            g = k + lane;
-            valid = g < f.total_tris && kz_box_overlaps(f.tri_rect[g], X0, Y0, RW);
+        bool valid = k + lane < n;
+        if (valid) g = ids ? ids[k + lane] : k + lane;
+        if (filter) {
+            valid = valid && kz_box_overlaps(f.tri_rect[g], X0, Y0, RW);
             if (kz_ballot(valid) == 0) {
                 k += 64;
                 continue;
@@ -787,9 +754,9 @@ __device__ void kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_
         unsigned long long *st = kz_stats_stripe(f);
         atomicAdd(&st[KZ_STAT_FRAGMENTS], nfrag);
         atomicAdd(&st[KZ_STAT_DEPTH_PASSED], npass);
-        if (f.tile_frags) atomicAdd(&f.tile_frags[stats_tile], (uint32_t)nfrag);
     }
     __syncthreads();
+    return nfrag;
 }
 
 // Static mode: one workgroup per screen tile, in tile order.
@@ -801,31 +768,65 @@ extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
     const uint32_t tile = blockIdx.x;
     const int32_t X0 = (int32_t)((tile % f.tiles_x) << KZ_TILE_SHIFT);
     const int32_t Y0 = (int32_t)((tile / f.tiles_x) << KZ_TILE_SHIFT);
-    kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, f.tile_offsets[tile], f.tile_counts[tile], tile, s_color, s_depth,
-                   s_claim, s_frag);
+    const uint32_t off = f.tile_offsets[tile], n = f.tile_counts[tile];
+    const uint64_t t0 = f.tile_cycles ? clock64() : 0;
+    unsigned long long frags = 0;
+    // entries past list_capacity were dropped by the scatter: such a tile
+    // takes its triangles from an in-order scan of all pixel boxes
+    if (n == 0)
+        frags = 0;
+    else if ((uint64_t)off + n <= (uint64_t)f.list_capacity)
+        frags = kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, f.tile_lists + off, n, false, s_color, s_depth, s_claim,
+                               s_frag);
+    else
+        frags = kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, nullptr, f.total_tris, true, s_color, s_depth, s_claim,
+                               s_frag);
+    // tile profile: the cost model's label (fragments) and raster cycles
+    if (f.tile_cycles && kz_lane() == 0) {
+        const uint64_t dt = n ? clock64() - t0 : 0;
+        f.tile_cycles[tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+        f.tile_frags[tile] = frags > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)frags;
+    }
 }
 
-// Balanced mode: persistent workgroups pop work items (tiles or sub-tiles)
-// from the queue, most expensive first.
+// Balanced mode: persistent workgroups take work items (tiles or
+// sub-tiles) from the queue, most expensive first: item b in the first
+// round, then the next untaken item from a counter.
 extern "C" __global__ __launch_bounds__(64) void kz_raster_persistent(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];
     __shared__ uint32_t s_frag[KZ_FRAG_CAP];
     const uint32_t lane = kz_lane();
-    const uint32_t n_items = f.queue_head[1];
-    while (true) {
-        uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(&f.queue_head[0], 1u);
-        q = (uint32_t)kz_readlane_i((int32_t)q, 0);
-        if (q >= n_items) break;
-        const uint32_t item = f.queue[q];
-        const uint32_t geo = f.items[item];
+    const uint32_t n_items = f.ctrl[KZ_CTRL_ITEMS];
+    uint32_t q = blockIdx.x;
+    while (q < n_items) {
+        const uint32_t geo = f.queue[q];
         const uint32_t tile = KZ_ITEM_TILE(geo), level = KZ_ITEM_LEVEL(geo), sub = KZ_ITEM_SUB(geo);
         const uint32_t RS = KZ_TILE_SHIFT - level;
         const int32_t X0 = (int32_t)(((tile % f.tiles_x) << KZ_TILE_SHIFT) + ((sub & ((1u << level) - 1)) << RS));
         const int32_t Y0 = (int32_t)(((tile / f.tiles_x) << KZ_TILE_SHIFT) + ((sub >> level) << RS));
-        kz_raster_rect(f, X0, Y0, RS, f.item_offsets[item], f.item_counts[item], tile, s_color, s_depth, s_claim,
-                       s_frag);
+        const uint32_t toff = f.tile_offsets[tile], tn = f.tile_counts[tile];
+        if (level == 0) {
+            if ((uint64_t)toff + tn <= (uint64_t)f.list_capacity)
+                kz_raster_rect(f, X0, Y0, RS, f.tile_lists + toff, tn, false, s_color, s_depth, s_claim,
+                               s_frag);
+            else
+                kz_raster_rect(f, X0, Y0, RS, nullptr, f.total_tris, true, s_color, s_depth, s_claim, s_frag);
+        } else {
+            // a split tile's list always fits; its sub-tile lists may not,
+            // and then the sub-tile filters the tile's list
+            const uint32_t i = f.split_slot[tile] * 16 + sub;
+            const uint32_t soff = f.split_offsets[i], sn = f.split_counts[i];
+            if ((uint64_t)soff + sn <= (uint64_t)f.split_capacity)
+                kz_raster_rect(f, X0, Y0, RS, f.split_lists + soff, sn, false, s_color, s_depth, s_claim,
+                               s_frag);
+            else
+                kz_raster_rect(f, X0, Y0, RS, f.tile_lists + toff, tn, true, s_color, s_depth, s_claim,
+                               s_frag);
+        }
+        uint32_t next = 0;
+        if (lane == 0) next = atomicAdd(&f.ctrl[KZ_CTRL_HEAD], 1u);
+        q = gridDim.x + (uint32_t)kz_readlane_i((int32_t)next, 0);
     }
 }

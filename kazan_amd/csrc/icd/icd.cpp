@@ -16,6 +16,7 @@
 #include <unordered_map>
 
 #include "kazan/kazan_icd.h"
+#include "kernels.h"
 #include "objects.h"
 
 namespace kz {
@@ -389,6 +390,52 @@ VKAPI_ATTR VkResult VKAPI_CALL vkGetKernelTimesKAZAN(VkDevice device, float *pMs
         if (pMs) pMs[i] = s.kernel_ms[i];
         if (pLaunches) pLaunches[i] = s.kernel_launches[i];
     }
+    return VK_SUCCESS;
+}
+
+VKAPI_ATTR void VKAPI_CALL vkSetTileProfileKAZAN(VkDevice device, VkBool32 enable) {
+    from<Device>(device)->engine->set_tile_profile(enable != 0);
+}
+
+VKAPI_ATTR VkResult VKAPI_CALL vkGetTileProfileKAZAN(VkDevice device, uint32_t *pTileCount, float *pFeatures,
+                                                     float *pFragments, float *pCycles, float *pPredicted) {
+    Device *d = from<Device>(device);
+    VkResult r = vkDeviceWaitIdle(device);
+    if (r != VK_SUCCESS) return r;
+    return d->engine->read_tile_profile(pTileCount, pFeatures, pFragments, pCycles, pPredicted) == hipSuccess
+               ? VK_SUCCESS
+               : VK_ERROR_DEVICE_LOST;
+}
+
+VKAPI_ATTR VkResult VKAPI_CALL vkSetCostModelKAZAN(VkDevice device, const void *pBlob, size_t size) {
+    return from<Device>(device)->engine->set_cost_model(pBlob, size) ? VK_SUCCESS
+                                                                       : VK_ERROR_INITIALIZATION_FAILED;
+}
+
+VKAPI_ATTR VkBool32 VKAPI_CALL vkHasCostModelKAZAN(VkDevice device) {
+    return from<Device>(device)->engine->has_cost_model() ? VK_TRUE : VK_FALSE;
+}
+
+VKAPI_ATTR void VKAPI_CALL vkGetGraphStatsKAZAN(VkDevice device, uint32_t *pCaptures, uint32_t *pLaunches) {
+    const Engine &e = *from<Device>(device)->engine;
+    if (pCaptures) *pCaptures = e.graph_captures();
+    if (pLaunches) *pLaunches = e.graph_launches();
+}
+
+VKAPI_ATTR VkResult VKAPI_CALL vkFillClusterSceneKAZAN(VkDevice device, VkBuffer buffer, VkDeviceSize offset,
+                                                       const KazanClusterScene *p) {
+    Device *d = from<Device>(device);
+    Buffer *b = from<Buffer>(buffer);
+    if (!p || !b || !b->address() || p->width == 0 || p->height == 0) return VK_ERROR_INITIALIZATION_FAILED;
+    const VkDeviceSize bytes = (VkDeviceSize)p->triangles * 3 * 8 * sizeof(float);
+    if (offset % 4 != 0 || offset > b->size || bytes > b->size - offset) return VK_ERROR_INITIALIZATION_FAILED;
+    const uint32_t nc = (uint32_t)std::llround((double)p->triangles * std::min(std::max(p->cluster_frac, 0.0), 1.0));
+    hipSetDevice(d->engine->device());
+    hipStream_t st = d->engine->stream();
+    if (kz_launch_gen_cluster(st, (float *)(b->address() + offset), p->triangles, nc, p->width, p->height,
+                              p->sigma_frac, p->radius, p->seed) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return VK_ERROR_DEVICE_LOST;
     return VK_SUCCESS;
 }
 
@@ -777,11 +824,12 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateCommandPool(VkDevice, const VkCommandPool
 }
 
 static void reset_cb(CommandBuffer *cb) {
-    if (!cb->device_allocs.empty()) {
+    if (!cb->device_allocs.empty() || cb->graph.exec) {
         set_device(cb->device);
         hipStreamSynchronize(cb->device->engine->stream());
         for (void *p : cb->device_allocs) hipFree(p);
     }
+    cb->graph.reset();
     cb->device_allocs.clear();
     cb->ops.clear();
     cb->st = RecordState();
@@ -999,6 +1047,11 @@ static void record_draw(CommandBuffer *cb, uint32_t count, uint32_t instances, u
     const uint32_t ts = 1u << p->fixed.tile_shift;
     f.tiles_x = (f.width + ts - 1) / ts;
     f.tiles_y = (f.height + ts - 1) / ts;
+    const FixedState &fx = p->fixed;
+    f.pipe_info = KZ_PIPE_PACK(fx.samples == 4 ? 2u : 0u, fx.blend ? 1u : 0u, fx.depth_test ? 1u : 0u,
+                               std::min<uint32_t>(fx.num_varyings, 31u),
+                               std::min<uint32_t>((uint32_t)p->tex_slot.size(), 7u),
+                               std::min<uint32_t>((uint32_t)p->ubo_slot.size(), 7u));
     const VkViewport &v = st.viewport;
     f.vp_hw = v.width * 0.5f;
     f.vp_hh = v.height * 0.5f;
@@ -1040,7 +1093,7 @@ static void record_draw(CommandBuffer *cb, uint32_t count, uint32_t instances, u
     Op *batch = nullptr;
     if (st.batch_open && cb->device->engine->merge_draws() && !cb->ops.empty() && cb->ops.back().kind == Op::Batch) {
         Op &last = cb->ops.back();
-        if (last.kernels == &p->kernels && std::memcmp(&last.frame.vp_hw, &f.vp_hw, sizeof(float) * 8) == 0 &&
+        if (last.variants == &p->variants && std::memcmp(&last.frame.vp_hw, &f.vp_hw, sizeof(float) * 8) == 0 &&
             last.frame.sc_x0 == f.sc_x0 && last.frame.sc_y0 == f.sc_y0 && last.frame.sc_x1 == f.sc_x1 &&
             last.frame.sc_y1 == f.sc_y1 && last.frame.color == f.color && last.frame.depth == f.depth &&
             same_textures(last.frame, f))
@@ -1049,7 +1102,7 @@ static void record_draw(CommandBuffer *cb, uint32_t count, uint32_t instances, u
     if (!batch) {
         Op op;
         op.kind = Op::Batch;
-        op.kernels = &p->kernels;
+        op.variants = &p->variants;
         op.tile_shift = p->fixed.tile_shift;
         op.frame = f;
         cb->ops.push_back(std::move(op));
@@ -1232,7 +1285,7 @@ VKAPI_ATTR VkResult VKAPI_CALL vkQueueSubmit(VkQueue queue, uint32_t submitCount
     for (uint32_t s = 0; s < submitCount; ++s)
         for (uint32_t i = 0; i < submits[s].commandBufferCount; ++i) {
             CommandBuffer *cb = from<CommandBuffer>(submits[s].pCommandBuffers[i]);
-            hipError_t e = eng.execute(cb->ops);
+            hipError_t e = eng.execute(cb->ops, &cb->graph);
             if (e != hipSuccess) {
                 std::fprintf(stderr, "kazan: vkQueueSubmit: %s\n", hipGetErrorString(e));
                 return VK_ERROR_DEVICE_LOST;
@@ -1377,6 +1430,12 @@ std::unordered_map<std::string, Entry> &entries() {
         KZ_D(vkGetDrawStatsKAZAN);
         KZ_D(vkSetKernelTimingKAZAN);
         KZ_D(vkGetKernelTimesKAZAN);
+        KZ_D(vkSetTileProfileKAZAN);
+        KZ_D(vkGetTileProfileKAZAN);
+        KZ_D(vkSetCostModelKAZAN);
+        KZ_D(vkHasCostModelKAZAN);
+        KZ_D(vkFillClusterSceneKAZAN);
+        KZ_D(vkGetGraphStatsKAZAN);
 #undef KZ_I
 #undef KZ_P
 #undef KZ_D

@@ -139,7 +139,7 @@ struct Framebuffer {
 
 struct Pipeline {
     FixedState fixed;
-    PipelineKernels kernels;
+    PipelineVariants variants;  // compiled kernels per tile size
     bool dynamic_viewport = false, dynamic_scissor = false;
     VkViewport viewport{};
     VkRect2D scissor{};
@@ -176,6 +176,7 @@ struct CommandBuffer : Dispatchable {
     CommandPool *pool = nullptr;
     std::vector<Op> ops;
     std::vector<void *> device_allocs;
+    GraphCache graph;  // the ops as a hipGraph, once captured
     RecordState st;
     bool recording = false;
 };

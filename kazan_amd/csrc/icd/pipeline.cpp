@@ -99,6 +99,9 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
         err = "sample count must be 1 or 4";
         return false;
     }
+    // the largest tile whose raster workgroup fits LDS at this sample count
+    while (fx.tile_shift > kMinTileShift && raster_lds_bytes(fx.tile_shift, fx.samples) > 160u * 1024u)
+        --fx.tile_shift;
     if (const VkPipelineDepthStencilStateCreateInfo *ds = ci->pDepthStencilState) {
         fx.depth_test = ds->depthTestEnable ? 1u : 0u;
         fx.depth_write = ds->depthWriteEnable ? 1u : 0u;
@@ -179,20 +182,14 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
     }
     fx.num_varyings = co.num_varyings;
     p->shader_source = co.source;
-    std::vector<char> code;
-    std::string log;
-    if (!compile_pipeline_code(fx, co.source, code, log, nullptr)) {
-        err = "kernel compilation failed:\n" + log;
-        return false;
-    }
-    if (!load_pipeline_kernels(code, p->kernels, err)) return false;
+    p->variants.fixed = fx;
+    p->variants.shader_source = co.source;
+    // the default tile size now; others when the autotuner asks for them
+    if (!p->variants.get(fx.tile_shift, &err)) return false;
     *out = p.release();
     return true;
 }
 
-void destroy_pipeline(Pipeline *p) {
-    if (p->kernels.module) hipModuleUnload(p->kernels.module);
-    delete p;
-}
+void destroy_pipeline(Pipeline *p) { delete p; }
 
 }  // namespace kz

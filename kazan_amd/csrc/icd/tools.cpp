@@ -1,10 +1,15 @@
 // tools.cpp — the shader-compiler entry points of kazan_tools.h.
 #include "kazan/kazan_tools.h"
 
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../engine/engine.h"
+#include "../engine/kernels.h"
 #include "../shader/backend_hip.h"
 #include "../shader/spirv_reader.h"
 #include "../shader/translate.h"
@@ -30,7 +35,11 @@ kz::ir::Stage stage_of(uint32_t s) {
 extern "C" int kazan_spirv_disassemble(const uint32_t *words, size_t nwords, char *out, size_t cap, size_t *out_len) {
     kz::SpirvModule m;
     std::string err;
-    if (!kz::spirv_parse(words, nwords, m, err)) return put(err, out, cap, out_len, -1);
+    // as the module's bytes (either byte order), like vkCreateShaderModule
+    std::vector<uint32_t> w;
+    if (!kz::spirv_words_from_bytes((const uint8_t *)words, nwords * 4, w, err) ||
+        !kz::spirv_parse(w.data(), w.size(), m, err))
+        return put(err, out, cap, out_len, -1);
     return put(kz::spirv_disassemble(m), out, cap, out_len, 0);
 }
 
@@ -88,4 +97,77 @@ extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t
     return put("code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
                    " varyings=" + std::to_string(co.num_varyings),
                out, cap, out_len, 0);
+}
+
+// --- the learned tile-cost model's kernels, outside a Vulkan device ------
+
+static_assert(KAZAN_COST_PARAMS == KZ_PARAMS, "parameter count of kazan_tools.h");
+static_assert(sizeof(KzCostModel) == 19616, "cost_model.bin layout (kazan_amd/balancer writes it)");
+
+namespace {
+
+struct DeviceBuffers {
+    std::vector<void *> ptrs;
+    ~DeviceBuffers() {
+        for (void *p : ptrs) hipFree(p);
+    }
+    template <class T>
+    T *alloc(size_t count) {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+};
+
+}  // namespace
+
+extern "C" int kazan_cost_model_eval(const void *blob, size_t bytes, const float *features, uint32_t n, float *out) {
+    KzCostModel m;
+    if (!blob || bytes != sizeof m || (!features && n) || (!out && n)) return -1;
+    std::memcpy(&m, blob, sizeof m);
+    if (m.magic != KZ_COST_MODEL_MAGIC || m.version != 1 || m.bytes != sizeof m) return -2;
+    if (n == 0) return 0;
+    DeviceBuffers b;
+    KzCostModel *dm = b.alloc<KzCostModel>(1);
+    float *df = b.alloc<float>((size_t)n * KZ_FEATURES), *dp = b.alloc<float>(n);
+    if (!dm || !df || !dp) return -3;
+    if (hipMemcpy(dm, &m, sizeof m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(df, features, (size_t)n * KZ_FEATURES * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        kz_launch_cost_mlp(nullptr, dm, df, n, dp) != hipSuccess ||
+        hipMemcpy(out, dp, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return -4;
+    return 0;
+}
+
+extern "C" int kazan_cost_model_train(const float *x, const float *y, uint32_t n, float *state, uint32_t steps,
+                                      uint32_t step0, uint32_t batch, float lr, uint64_t seed, float *loss_out) {
+    if (!x || !y || !state || n == 0 || batch == 0 || batch % 64 != 0) return -1;
+    DeviceBuffers b;
+    float *dx = b.alloc<float>((size_t)n * KZ_FEATURES), *dy = b.alloc<float>(n);
+    float *ds = b.alloc<float>(3 * (size_t)KZ_PARAMS), *dl = b.alloc<float>(steps);
+    if (!dx || !dy || !ds || !dl) return -3;
+    if (hipMemcpy(dx, x, (size_t)n * KZ_FEATURES * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, y, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ds, state, 3 * (size_t)KZ_PARAMS * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return -4;
+    // runs of at most 1000 steps per launch
+    for (uint32_t done = 0; done < steps;) {
+        KzTrainParams p{};
+        p.lr = lr;
+        p.beta1 = 0.9f;
+        p.beta2 = 0.999f;
+        p.eps = 1e-8f;
+        p.steps = std::min<uint32_t>(steps - done, 1000u);
+        p.step0 = step0 + done;
+        p.batch = batch;
+        p.n = n;
+        p.seed = seed;
+        if (kz_launch_cost_train(nullptr, ds, dx, dy, p, dl + done) != hipSuccess) return -4;
+        done += p.steps;
+    }
+    if (hipMemcpy(state, ds, 3 * (size_t)KZ_PARAMS * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return -4;
+    if (loss_out && steps && hipMemcpy(loss_out, dl, steps * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return -4;
+    return 0;
 }

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 
 namespace kz {
 
@@ -368,10 +369,15 @@ std::string spirv_disassemble(const SpirvModule &m) {
                   (m.version >> 16) & 0xFF, (m.version >> 8) & 0xFF, m.generator, m.bound, m.schema);
     out += buf;
     const auto &tab = op_table();
+    std::unordered_map<uint32_t, uint32_t> int_width, type_of;  // integer type -> bits; value -> type
     for (const auto &ins : m.instructions) {
         auto it = tab.find(ins.opcode);
         std::string result, text;
         const auto &ops = ins.operands;
+        if (ins.opcode == 21 && ops.size() >= 2) int_width[ops[0]] = ops[1];  // OpTypeInt
+        if (it != tab.end() && it->second.kinds.size() >= 2 && it->second.kinds[0] == K_RTYPE &&
+            it->second.kinds[1] == K_RESULT && ops.size() >= 2)
+            type_of[ops[1]] = ops[0];
         size_t i = 0;
         auto id = [&](uint32_t v) { return "%" + std::to_string(v); };
         if (it == tab.end()) {
@@ -447,11 +453,21 @@ std::string spirv_disassemble(const SpirvModule &m) {
                     for (; i < ops.size(); ++i) text += " " + id(ops[i]);
                     break;
                 }
-                case K_SWITCH:
+                case K_SWITCH: {
+                    // case literals have the selector's width (one or two words)
+                    const uint32_t sel = ops[i];
+                    auto ty = type_of.find(sel);
+                    const bool wide = ty != type_of.end() && int_width.count(ty->second) &&
+                                      int_width[ty->second] == 64;
                     text += " " + id(ops[i++]);
                     if (i < ops.size()) text += " " + id(ops[i++]);
-                    for (; i + 1 < ops.size(); i += 2) text += " " + std::to_string(ops[i]) + " " + id(ops[i + 1]);
+                    const size_t step = wide ? 3 : 2;
+                    for (; i + step - 1 < ops.size(); i += step) {
+                        const uint64_t v = wide ? ((uint64_t)ops[i + 1] << 32) | ops[i] : ops[i];
+                        text += " " + std::to_string(v) + " " + id(ops[i + step - 1]);
+                    }
                     break;
+                }
                 default:
                     text += " " + num_or_enum(k, ops[i++]);
                     break;

@@ -152,8 +152,8 @@ class Context:
 
     def kernel_times(self):
         """{kernel: (summed ms, launches)} of the submissions since the last read."""
-        ms = (ctypes.c_float * 6)()
-        n = (ctypes.c_uint32 * 6)()
+        ms = (ctypes.c_float * len(vk.KERNEL_NAMES))()
+        n = (ctypes.c_uint32 * len(vk.KERNEL_NAMES))()
         self.d.check("vkGetKernelTimesKAZAN", self.device, ms, n)
         return {k: (ms[i], n[i]) for i, k in enumerate(vk.KERNEL_NAMES)}
 
@@ -390,3 +390,37 @@ class SceneRenderer:
         depth = outs[1].view(np.float32).reshape(H, W, S)
         resolved = outs[2].reshape(H, W, 4) if self.resolve else color[:, :, 0, :].copy()
         return color, depth, resolved
+
+
+def pipeline_desc(s, tile_shift=4):
+    """The KazanPipelineDesc of the pipeline SceneRenderer creates for scene
+    `s` (same shaders, vertex input and fixed-function state), for the
+    device-less compiler entry point. Returns (desc, keep-alive buffers)."""
+    vs_fn, attrs = _VS[s.vs]
+    vs = np.frombuffer(vs_fn(), dtype=np.uint32).copy()
+    fs = np.frombuffer(_FS[s.fs](), dtype=np.uint32).copy()
+    d = vk.KazanPipelineDesc()
+    d.vs_words, d.vs_nwords = vs.ctypes.data, vs.size
+    d.fs_words, d.fs_nwords = fs.ctypes.data, fs.size
+    d.num_bindings = 1
+    d.binding_stride[0] = s.stride * 4
+    d.num_attributes = len(attrs)
+    for i, (loc, fmt, off) in enumerate(attrs):
+        d.attr_location[i], d.attr_binding[i], d.attr_format[i], d.attr_offset[i] = loc, 0, fmt, off
+    d.tile_shift, d.samples, d.cull_mode, d.front_face = tile_shift, s.samples, s.cull_mode, s.front_face
+    d.depth_test, d.depth_write, d.depth_op = s.depth_test, s.depth_write, s.depth_op
+    (d.blend, d.src_color, d.dst_color, d.color_op, d.src_alpha, d.dst_alpha, d.alpha_op) = s.blend
+    d.write_mask = s.write_mask
+    if s.fs == g.FS_TEXTURE:
+        d.num_textures, d.texture_key[0] = 1, 0
+    if s.fs == g.FS_PHONG:
+        d.num_ubos, d.ubo_key[0] = 1, 0
+    return d, (vs, fs)
+
+
+def compile_offline(driver, s, what=2, tile_shift=4):
+    """Runs pipeline compilation for scene `s` without a device: what = 0
+    generated shader source, 1 full kernel source, 2 code object (through the
+    on-disk cache; the text reports its size). Returns (rc, text)."""
+    d, keep = pipeline_desc(s, tile_shift)
+    return vk.tool_call(driver, "kazan_compile_pipeline_spirv", ctypes.byref(d), ctypes.c_uint32(what))

@@ -583,12 +583,18 @@ _SIGS = {
     "vkGetDrawStatsKAZAN": (c_int32, [Handle, P(c_uint64), P(c_float), P(c_uint32), P(c_uint32)]),
     "vkSetKernelTimingKAZAN": (None, [Handle, c_uint32]),
     "vkGetKernelTimesKAZAN": (c_int32, [Handle, P(c_float), P(c_uint32)]),
+    "vkSetTileProfileKAZAN": (None, [Handle, c_uint32]),
+    "vkGetTileProfileKAZAN": (c_int32, [Handle, P(c_uint32), P(c_float), P(c_float), P(c_float), P(c_float)]),
+    "vkSetCostModelKAZAN": (c_int32, [Handle, c_void_p, c_size_t]),
+    "vkHasCostModelKAZAN": (c_uint32, [Handle]),
+    "vkFillClusterSceneKAZAN": (c_int32, [Handle, Handle, c_uint64, c_void_p]),
+    "vkGetGraphStatsKAZAN": (None, [Handle, P(c_uint32), P(c_uint32)]),
 }
 
-KERNEL_NAMES = ["clear_counts", "bin_count", "bin_scan", "bin_scatter", "bin_sort", "raster"]
+KERNEL_NAMES = ["clear_counts", "bin_count", "bin_hist", "bin_scan", "bin_scatter", "bin_sort", "plan", "raster"]
 
 STAT_NAMES = ["triangles_in", "triangles_setup", "triangles_unsupported", "fragments", "depth_passed",
-              "list_entries", "list_overflow", "reserved"]
+              "list_entries", "list_overflow", "split_entries"]
 
 
 class VulkanError(RuntimeError):
@@ -636,3 +642,33 @@ class Driver:
         if r != VK_SUCCESS:
             raise VulkanError(name, r)
         return r
+
+
+# --- shader-compiler tools exported beside the ICD (include/kazan/kazan_tools.h)
+
+class KazanPipelineDesc(ctypes.Structure):
+    _fields_ = [("vs_words", c_void_p), ("vs_nwords", ctypes.c_size_t), ("fs_words", c_void_p),
+                ("fs_nwords", ctypes.c_size_t), ("num_bindings", c_uint32), ("binding_stride", c_uint32 * 8),
+                ("binding_rate", c_uint32 * 8), ("num_attributes", c_uint32), ("attr_location", c_uint32 * 16),
+                ("attr_binding", c_uint32 * 16), ("attr_format", c_uint32 * 16), ("attr_offset", c_uint32 * 16),
+                ("tile_shift", c_uint32), ("samples", c_uint32), ("cull_mode", c_uint32), ("front_face", c_uint32),
+                ("depth_test", c_uint32), ("depth_write", c_uint32), ("depth_op", c_uint32), ("blend", c_uint32),
+                ("src_color", c_uint32), ("dst_color", c_uint32), ("color_op", c_uint32), ("src_alpha", c_uint32),
+                ("dst_alpha", c_uint32), ("alpha_op", c_uint32), ("write_mask", c_uint32), ("num_ubos", c_uint32),
+                ("ubo_key", c_uint32 * 4), ("num_textures", c_uint32), ("texture_key", c_uint32 * 4)]
+
+
+def tool_call(driver, name, *args, cap=1 << 22):
+    """Calls one of the kazan_* text-returning tools; returns (rc, text)."""
+    fn = getattr(driver.lib, name)
+    fn.restype = ctypes.c_int
+    out = ctypes.create_string_buffer(cap)
+    n = ctypes.c_size_t(0)
+    rc = fn(*args, out, ctypes.c_size_t(cap), ctypes.byref(n))
+    return rc, out.value.decode(errors="replace")
+
+
+class KazanClusterScene(ctypes.Structure):
+    _fields_ = [("seed", c_uint64), ("triangles", c_uint32), ("width", c_uint32), ("height", c_uint32),
+                ("pad", c_uint32), ("cluster_frac", ctypes.c_double), ("sigma_frac", ctypes.c_double),
+                ("radius", ctypes.c_double)]

@@ -135,6 +135,32 @@ def test_huge_triangles_list_overflow(small_list_ctx, oracle):
     assert st["list_overflow"] >= 1
 
 
+@pytest.fixture
+def tiny_list_ctx(driver, monkeypatch):
+    # balanced form; tile lists and sub-tile lists hold 10,000 entries
+    from kazan_amd.scenes.renderer import Context
+    monkeypatch.setenv("KAZAN_LIST_CAPACITY_LIMIT", "10000")
+    monkeypatch.setenv("KAZAN_PIPELINE", "balanced")
+    return Context(driver)
+
+
+def test_split_tile_sub_list_overflow(tiny_list_ctx, oracle):
+    # 7,000 translucent triangles inside tile (1, 1): its list (7,000
+    # entries) fits, the tile is split into 16 sub-tiles, and their lists
+    # (several entries per triangle) do not, so the sub-tiles past the
+    # storage filter the tile's list instead
+    r = np.random.default_rng(15)
+    n = 7000
+    tris = np.round(r.uniform(16.1, 31.9, (n, 3, 2)) * 16) / 16
+    s = scene_from_pixels(tris, colors=r.uniform(0, 1, (n, 3, 4)), z=r.uniform(0, 1, (n, 3)), width=64, height=64)
+    s.blend = (1, g.BLEND_SRC_ALPHA, g.BLEND_ONE_MINUS_SRC_ALPHA, 0, g.BLEND_ONE, g.BLEND_ONE_MINUS_SRC_ALPHA, 0)
+    s.depth_test = 0
+    st, ost, _ = check_parity(tiny_list_ctx, oracle, s)
+    assert st["list_overflow"] == 0
+    assert st["split_entries"] > 10000
+    assert ost["fragments"] > 100000
+
+
 @pytest.mark.parametrize("cull,front", [(1, 0), (2, 0), (2, 1), (3, 0)])
 def test_cull_modes(mctx, oracle, cull, front):
     s = g.random_triangles(200, 64, 64, seed=5)
