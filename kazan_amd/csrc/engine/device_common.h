@@ -80,6 +80,7 @@ struct KzFrame {
     uint32_t list_capacity;
     uint32_t pad0;
     uint64_t *tri_rect;      // [total_tris] packed pixel bounding box or ~0
+    uint32_t *tri_rec;       // [total_tris * KZ_REC_WORDS_FOR(varyings)] setup records, or null
     unsigned long long *stats;  // KZ_STAT_* counters
 
     // Load-balanced mode. A work item is a square sub-tile of a screen
@@ -218,6 +219,10 @@ enum {
 // Per-triangle pixel bounding box (inclusive, clipped to the scissor),
 // packed as px0 | py0 << 16 | px1 << 32 | py1 << 48; ~0 = no pixels.
 #define KZ_RECT_EMPTY (~0ULL)
+
+// 32-bit words of a setup record with nv varyings: 16 words of edge and
+// plane constants, 3 per varying (base, two deltas), rounded up to 16 B.
+#define KZ_REC_WORDS_FOR(nv) ((16u + 3u * (nv) + 3u) & ~3u)
 
 #define KZ_ITEM_PACK(tile, level, sub) (((tile) << 6) | ((level) << 4) | (sub))
 #define KZ_ITEM_TILE(it) ((it) >> 6)

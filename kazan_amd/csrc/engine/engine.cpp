@@ -46,6 +46,7 @@ Engine::~Engine() {
     hipFree(tile_lists_);
     hipFree(sort_tmp_);
     hipFree(tri_rect_);
+    hipFree(tri_rec_);
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
                         chunk_tile_, chunk_counts_, split_lists_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
         hipFree(p);
@@ -77,6 +78,7 @@ bool Engine::init(int hip_device, std::string &err) {
     if (const char *e = std::getenv("KAZAN_PIPELINE")) balanced_ = std::strcmp(e, "plain") != 0;
     if (const char *e = std::getenv("KAZAN_AUTOTUNE")) autotune_ = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("KAZAN_GRAPHS")) graphs_ = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("KAZAN_SETUP_RECORDS")) setup_records_ = std::strcmp(e, "0") != 0;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) == hipSuccess && cus > 0)
         split_grid_ = 4u * (uint32_t)cus;
@@ -258,6 +260,15 @@ hipError_t Engine::prepare_batch(const Op &op, uint32_t shift) {
     const uint32_t ts = 1u << shift;
     const uint32_t ntiles = ((f.width + ts - 1) >> shift) * ((f.height + ts - 1) >> shift);
     KZ_TRY(ensure_scratch(ntiles, f.total_tris));
+    if (setup_records_) {
+        const uint64_t words = (uint64_t)f.total_tris * KZ_REC_WORDS_FOR(op.variants->fixed.num_varyings);
+        if (words > cap_rec_words_) {
+            KZ_TRY(hipStreamSynchronize(stream_));
+            const uint64_t w = std::max(words, cap_rec_words_ + cap_rec_words_ / 2);
+            KZ_TRY(regrow(tri_rec_, w));
+            cap_rec_words_ = w;
+        }
+    }
     if (balanced_) KZ_TRY(ensure_split_scratch(ntiles));
     if ((cost_model_d_ != nullptr && balanced_) || tile_profile_) KZ_TRY(ensure_model_scratch(ntiles, f.total_tris));
     return hipSuccess;
@@ -285,6 +296,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     f.sort_tmp = sort_tmp_;
     f.list_capacity = list_capacity_;
     f.tri_rect = tri_rect_;
+    f.tri_rec = setup_records_ ? tri_rec_ : nullptr;
     f.stats = stats_d_;
     f.tile_shift = shift;
     if (balanced_) {
@@ -346,7 +358,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     KZ_TRY(mark(KS_BIN_SORT, false));
     KZ_TRY(kz_launch_sort_lists(stream_, f, ntiles));
     KZ_TRY(mark(KS_BIN_SORT, true));
-    launches += acc ? 7 : 6;
+    launches += acc ? 8 : 7;
 
     if (!balanced_) {
         if (acc) {
@@ -482,10 +494,19 @@ hipError_t Engine::pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool
 }
 
 hipError_t Engine::execute(std::vector<Op> &ops, GraphCache *cache) {
+    // Graphs pay off for command buffers that draw and are submitted again:
+    // the first submission (and every tuning trial) runs eagerly.
     bool graphable = cache != nullptr && graphs_ && !kernel_timing_;
+    bool draws = false;
     for (Op &op : ops)
-        if (op.kind == Op::Batch && autotune() && !(op.tuning && op.tuning->done)) graphable = false;
-    if (!graphable) return execute_eager(ops);
+        if (op.kind == Op::Batch) {
+            draws = true;
+            if (autotune() && !(op.tuning && op.tuning->done)) graphable = false;
+        }
+    if (!graphable || !draws || (cache->eager_runs == 0 && !cache->exec)) {
+        if (cache) ++cache->eager_runs;
+        return execute_eager(ops);
+    }
     // every allocation before the capture
     for (Op &op : ops)
         if (op.kind == Op::Batch) {

@@ -159,6 +159,7 @@ struct GraphCache {
     hipGraphExec_t exec = nullptr;
     uint64_t generation = 0;
     uint32_t batches = 0, launches = 0;
+    uint32_t eager_runs = 0;  // a command buffer is captured from its second submission on
 };
 
 class Engine {
@@ -270,6 +271,10 @@ class Engine {
     uint32_t *tile_counts_ = nullptr, *tile_offsets_ = nullptr, *tile_cursor_ = nullptr;
     uint32_t *tile_lists_ = nullptr, *sort_tmp_ = nullptr;
     uint64_t *tri_rect_ = nullptr;
+    // setup records (KAZAN_SETUP_RECORDS=0: the raster recomputes the setup)
+    bool setup_records_ = true;
+    uint32_t *tri_rec_ = nullptr;
+    uint64_t cap_rec_words_ = 0;
     unsigned long long *stats_d_ = nullptr, *stats_h_ = nullptr;
     uint32_t cap_tiles_ = 0, cap_tris_ = 0, list_capacity_ = 0;
     uint32_t pending_batches_ = 0, pending_launches_ = 0;

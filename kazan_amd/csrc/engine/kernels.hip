@@ -22,6 +22,8 @@
 // atomic per (workgroup, tile), not one per wave and tile.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -433,7 +435,57 @@ __device__ void merge_pass(const uint32_t *src, uint32_t *dst, uint32_t n, uint3
     __syncthreads();
 }
 
-// Sorts lists [0, n), grid-stride over lists.
+// Sorts the lists of at most kShortList ids, one wave per list, in
+// registers: a bitonic network over 256 keys, 4 per lane (key 4 * lane + j;
+// partners 1 and 2 apart are in the same lane, farther ones in lane ^ d / 4).
+constexpr uint32_t kShortList = 256;
+
+__global__ __launch_bounds__(256) void kz_sort_short(const uint32_t *counts, const uint32_t *offsets, uint32_t nl,
+                                                     uint32_t *lists, uint32_t capacity) {
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t l = blockIdx.x * 4 + (threadIdx.x >> 6); l < nl; l += waves) {
+        const uint32_t n = counts[l];
+        const uint32_t off = offsets[l];
+        if (n <= 1 || n > kShortList || (uint64_t)off + n > (uint64_t)capacity) continue;
+        uint32_t v[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = 4 * lane + j;
+            v[j] = i < n ? lists[off + i] : kNone;
+        }
+        for (uint32_t k = 2; k <= kShortList; k <<= 1)
+            for (uint32_t d = k >> 1; d > 0; d >>= 1) {
+                if (d >= 4) {
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) {
+                        const uint32_t i = 4 * lane + j;
+                        const uint32_t o = (uint32_t)__shfl_xor((int)v[j], (int)(d >> 2), 64);
+                        const bool keep_min = ((i & d) == 0) == ((i & k) == 0);
+                        v[j] = keep_min ? min(v[j], o) : max(v[j], o);
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) {
+                        if (j & d) continue;
+                        const uint32_t i = 4 * lane + j;
+                        const uint32_t a = v[j], b = v[j | d];
+                        if ((a > b) == ((i & k) == 0)) {
+                            v[j] = b;
+                            v[j | d] = a;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = 4 * lane + j;
+            if (i < n) lists[off + i] = v[j];
+        }
+    }
+}
+
+// Sorts the lists [0, n) longer than kShortList, grid-stride over lists.
 __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *counts, const uint32_t *offsets,
                                                                uint32_t nl, uint32_t *lists, uint32_t *tmp,
                                                                uint32_t capacity) {
@@ -441,7 +493,7 @@ __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *co
     for (uint32_t l = blockIdx.x; l < nl; l += gridDim.x) {
         const uint32_t n = counts[l];
         const uint32_t off = offsets[l];
-        if (n <= 1 || (uint64_t)off + n > (uint64_t)capacity) continue;
+        if (n <= kShortList || (uint64_t)off + n > (uint64_t)capacity) continue;
         uint32_t *list = lists + off;
         if (n <= kSortCap) {
             sort_run_lds(list, n, s);
@@ -1188,9 +1240,13 @@ hipError_t kz_launch_bin_scatter(hipStream_t st, const KzFrame &f) {
 }
 
 hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
-    const uint32_t grid = ntiles < 8192u ? (ntiles ? ntiles : 1u) : 8192u;
-    kz_sort_lists<<<grid, kSortThreads, 0, st>>>(f.tile_counts, f.tile_offsets, ntiles, f.tile_lists, f.sort_tmp,
-                                                 f.list_capacity);
+    if (ntiles == 0) return hipSuccess;
+    // short lists: one wave each; long lists (few, skewed scenes): a workgroup each
+    const uint32_t short_grid = std::min((ntiles + 3) / 4, 16384u);
+    kz_sort_short<<<short_grid, 256, 0, st>>>(f.tile_counts, f.tile_offsets, ntiles, f.tile_lists, f.list_capacity);
+    const uint32_t long_grid = std::min(ntiles, 1024u);
+    kz_sort_lists<<<long_grid, kSortThreads, 0, st>>>(f.tile_counts, f.tile_offsets, ntiles, f.tile_lists,
+                                                      f.sort_tmp, f.list_capacity);
     return hipGetLastError();
 }
 

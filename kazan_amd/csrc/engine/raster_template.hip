@@ -191,12 +191,19 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
 }
 
 // Coverage constants: the interior-positive orientation and the top-left
-// thresholds (0 on a top-left edge, 1 otherwise) of E01, E12, E20.
+// thresholds (0 on a top-left edge, 1 otherwise) of E01, E12, E20; `small`:
+// every edge delta is under 8192 (512 px), so that for any sample in the
+// triangle's pixel box (|r| < 2^14) the edge products stay under 2^27 and
+// E01, E20, E12 and S fit 32 bits (24-bit multiplies, same integers).
 struct KzCov {
-    int32_t flip, th01, th12, th20;
+    int32_t flip, th01, th12, th20, small;
 };
 
 __device__ __forceinline__ int32_t kz_tl_threshold(int64_t A, int64_t B) { return (A > 0 || (A == 0 && B > 0)) ? 0 : 1; }
+
+__device__ __forceinline__ bool kz_small_deltas(int32_t d1x, int32_t d1y, int32_t d2x, int32_t d2y) {
+    return max(max(abs(d1x), abs(d1y)), max(abs(d2x), abs(d2y))) < 8192;
+}
 
 __device__ __forceinline__ KzCov kz_cov_setup(const KzTri &t) {
     KzCov c;
@@ -205,14 +212,24 @@ __device__ __forceinline__ KzCov kz_cov_setup(const KzTri &t) {
     c.th01 = kz_tl_threshold(-(int64_t)t.d1y * sg, (int64_t)t.d1x * sg);
     c.th20 = kz_tl_threshold((int64_t)t.d2y * sg, -(int64_t)t.d2x * sg);
     c.th12 = kz_tl_threshold(-((int64_t)t.d2y - t.d1y) * sg, ((int64_t)t.d2x - t.d1x) * sg);
+    c.small = kz_small_deltas(t.d1x, t.d1y, t.d2x, t.d2y) ? 1 : 0;
     return c;
 }
 
 // Signed (unflipped) edge values at absolute sample position (sx, sy) in
-// 1/16 px: E01, E20 and E12 = S - E01 - E20 (the three sum to S).
+// 1/16 px: E01, E20 and E12 = S - E01 - E20 (the three sum to S). The
+// sample lies in the triangle's pixel box.
 __device__ __forceinline__ void kz_edges(int32_t x0, int32_t y0, int32_t d1x, int32_t d1y, int32_t d2x, int32_t d2y,
                                          int64_t S, int32_t sx, int32_t sy, int64_t &e01, int64_t &e20, int64_t &e12) {
     const int32_t rx = sx - x0, ry = sy - y0;
+    if (kz_small_deltas(d1x, d1y, d2x, d2y)) {
+        const int32_t a = __mul24(d1x, ry) - __mul24(d1y, rx);
+        const int32_t b = __mul24(d2y, rx) - __mul24(d2x, ry);
+        e01 = a;
+        e20 = b;
+        e12 = (int64_t)((int32_t)S - a - b);
+        return;
+    }
     e01 = (int64_t)d1x * ry - (int64_t)d1y * rx;
     e20 = (int64_t)d2y * rx - (int64_t)d2x * ry;
     e12 = S - e01 - e20;
@@ -233,6 +250,23 @@ __constant__ const int32_t kz_sample_y[4] = {KZ_SAMPLES == 4 ? 2 : 8, 6, 10, 14}
 
 __device__ __forceinline__ uint32_t kz_cover_mask(const KzTri &t, const KzCov &c, int32_t px, int32_t py) {
     uint32_t m = 0;
+    if (c.small) {
+        const int32_t S = (int32_t)t.S;
+#pragma unroll
+        for (int s = 0; s < KZ_SAMPLES; ++s) {
+            const int32_t rx = px * 16 + kz_sample_x[s] - t.x0, ry = py * 16 + kz_sample_y[s] - t.y0;
+            int32_t e01 = __mul24(t.d1x, ry) - __mul24(t.d1y, rx);
+            int32_t e20 = __mul24(t.d2y, rx) - __mul24(t.d2x, ry);
+            int32_t e12 = S - e01 - e20;
+            if (c.flip) {
+                e01 = -e01;
+                e20 = -e20;
+                e12 = -e12;
+            }
+            if (e01 >= c.th01 && e12 >= c.th12 && e20 >= c.th20) m |= 1u << s;
+        }
+        return m;
+    }
 #pragma unroll
     for (int s = 0; s < KZ_SAMPLES; ++s) {
         int64_t e01, e20, e12;
@@ -259,6 +293,85 @@ __device__ __forceinline__ unsigned long long *kz_stats_stripe(const KzFrame &f)
     return f.stats + (size_t)(blockIdx.x & (KZ_STAT_STRIPES - 1)) * KZ_STAT_COUNT;
 }
 
+// Setup records: what the raster needs of a set-up triangle (edge and
+// plane constants, varyings' base and deltas), KZ_REC_WORDS 32-bit words,
+// written once per triangle so that a tile's raster does one gather per
+// list entry instead of index -> vertex -> setup again.
+#define KZ_REC_WORDS KZ_REC_WORDS_FOR(KZ_NV)
+
+__device__ __forceinline__ void kz_store_rec(const KzFrame &f, uint32_t g, const KzTri &t,
+                                             const float vary[3][KZ_NVA]) {
+    uint32_t w[KZ_REC_WORDS];
+    w[0] = (uint32_t)t.x0;
+    w[1] = (uint32_t)t.y0;
+    w[2] = (uint32_t)t.d1x;
+    w[3] = (uint32_t)t.d1y;
+    w[4] = (uint32_t)t.d2x;
+    w[5] = (uint32_t)t.d2y;
+    w[6] = (uint32_t)(uint64_t)t.S;
+    w[7] = (uint32_t)((uint64_t)t.S >> 32);
+    w[8] = __float_as_uint(t.invS);
+    w[9] = __float_as_uint(t.z0);
+    w[10] = __float_as_uint(t.dz1);
+    w[11] = __float_as_uint(t.dz2);
+    w[12] = __float_as_uint(t.iw0);
+    w[13] = __float_as_uint(t.iw1);
+    w[14] = __float_as_uint(t.iw2);
+    w[15] = 0u;
+#pragma unroll
+    for (int i = 0; i < KZ_NV; ++i) {
+        w[16 + 3 * i] = __float_as_uint(vary[0][i]);
+        w[17 + 3 * i] = __float_as_uint(vary[1][i] - vary[0][i]);
+        w[18 + 3 * i] = __float_as_uint(vary[2][i] - vary[0][i]);
+    }
+#pragma unroll
+    for (int i = 16 + 3 * KZ_NV; i < KZ_REC_WORDS; ++i) w[i] = 0u;
+    uint4 *dst = (uint4 *)(f.tri_rec + (size_t)g * KZ_REC_WORDS);
+#pragma unroll
+    for (int q = 0; q < KZ_REC_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+// The record of triangle g and its pixel box: the values kz_setup and the
+// varying differences gave when the record was written.
+__device__ __forceinline__ void kz_load_rec(const KzFrame &f, uint32_t g, KzTri &t, float *a0, float *da1,
+                                            float *da2) {
+    const uint4 *src = (const uint4 *)(f.tri_rec + (size_t)g * KZ_REC_WORDS);
+    uint32_t w[KZ_REC_WORDS];
+#pragma unroll
+    for (int q = 0; q < KZ_REC_WORDS / 4; ++q) {
+        const uint4 v = src[q];
+        w[4 * q] = v.x;
+        w[4 * q + 1] = v.y;
+        w[4 * q + 2] = v.z;
+        w[4 * q + 3] = v.w;
+    }
+    const uint64_t r = f.tri_rect[g];
+    t.x0 = (int32_t)w[0];
+    t.y0 = (int32_t)w[1];
+    t.d1x = (int32_t)w[2];
+    t.d1y = (int32_t)w[3];
+    t.d2x = (int32_t)w[4];
+    t.d2y = (int32_t)w[5];
+    t.S = (int64_t)((uint64_t)w[6] | ((uint64_t)w[7] << 32));
+    t.invS = __uint_as_float(w[8]);
+    t.z0 = __uint_as_float(w[9]);
+    t.dz1 = __uint_as_float(w[10]);
+    t.dz2 = __uint_as_float(w[11]);
+    t.iw0 = __uint_as_float(w[12]);
+    t.iw1 = __uint_as_float(w[13]);
+    t.iw2 = __uint_as_float(w[14]);
+    t.px0 = (int32_t)(r & 0xFFFF);
+    t.py0 = (int32_t)((r >> 16) & 0xFFFF);
+    t.px1 = (int32_t)((r >> 32) & 0xFFFF);
+    t.py1 = (int32_t)(r >> 48);
+#pragma unroll
+    for (int i = 0; i < KZ_NV; ++i) {
+        a0[i] = __uint_as_float(w[16 + 3 * i]);
+        da1[i] = __uint_as_float(w[17 + 3 * i]);
+        da2[i] = __uint_as_float(w[18 + 3 * i]);
+    }
+}
+
 extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = kz_lane();
@@ -271,9 +384,11 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
         uint64_t rect = KZ_RECT_EMPTY;
         if (st == 1) {
             n_setup = 1;
-            if (t.px0 <= t.px1 && t.py0 <= t.py1)
+            if (t.px0 <= t.px1 && t.py0 <= t.py1) {
                 rect = (uint64_t)(uint32_t)t.px0 | ((uint64_t)(uint32_t)t.py0 << 16) |
                        ((uint64_t)(uint32_t)t.px1 << 32) | ((uint64_t)(uint32_t)t.py1 << 48);
+                if (f.tri_rec) kz_store_rec(f, g, t, vary);
+            }
             if (f.tri_z) {  // cost-model feature: the triangle's depth range
                 const float z1 = t.z0 + t.dz1, z2 = t.z0 + t.dz2;
                 const float lo = fminf(fmaxf(fminf(t.z0, fminf(z1, z2)), 0.0f), 1.0f);
@@ -491,6 +606,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         L.cx1 = L.cy1 = -1;
         L.c.flip = 0;
         L.c.th01 = L.c.th12 = L.c.th20 = 0;
+        L.c.small = 1;
         L.t.x0 = L.t.y0 = L.t.d1x = L.t.d1y = L.t.d2x = L.t.d2y = 0;
         L.t.S = 0;
         L.t.invS = L.t.z0 = L.t.dz1 = L.t.dz2 = 0.0f;
@@ -498,21 +614,29 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 #pragma unroll
         for (int i = 0; i < KZ_NVA; ++i) L.a0[i] = L.da1[i] = L.da2[i] = 0.0f;
         if (valid) {
-            float p[3][4], vary[3][KZ_NVA];
-            kz_fetch_triangle(f, g, p, vary);
-            if (kz_setup(f, p, L.t) == 1) {
-                L.cx0 = max(L.t.px0, X0);
-                L.cx1 = min(L.t.px1, X0 + RW - 1);
-                L.cy0 = max(L.t.py0, Y0);
-                L.cy1 = min(L.t.py1, Y0 + RW - 1);
-                L.live = L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
-                L.c = kz_cov_setup(L.t);
+            // every listed triangle was set up by kz_bin_count: its record,
+            // or (without records) the vertex shader and setup once more
+            bool ok = true;
+            if (f.tri_rec) {
+                kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
+            } else {
+                float p[3][4], vary[3][KZ_NVA];
+                kz_fetch_triangle(f, g, p, vary);
+                ok = kz_setup(f, p, L.t) == 1;
 #pragma unroll
                 for (int i = 0; i < KZ_NV; ++i) {
                     L.a0[i] = vary[0][i];
                     L.da1[i] = vary[1][i] - vary[0][i];
                     L.da2[i] = vary[2][i] - vary[0][i];
                 }
+            }
+            if (ok) {
+                L.cx0 = max(L.t.px0, X0);
+                L.cx1 = min(L.t.px1, X0 + RW - 1);
+                L.cy0 = max(L.t.py0, Y0);
+                L.cy1 = min(L.t.py1, Y0 + RW - 1);
+                L.live = L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
+                L.c = kz_cov_setup(L.t);
             }
         }
         const int32_t cw = L.cx1 - L.cx0 + 1;
@@ -552,6 +676,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             cb.th01 = kz_readlane_i(L.c.th01, b);
             cb.th12 = kz_readlane_i(L.c.th12, b);
             cb.th20 = kz_readlane_i(L.c.th20, b);
+            cb.small = kz_readlane_i(L.c.small, b);
             const int32_t bx0 = kz_readlane_i(L.cx0, b), by0 = kz_readlane_i(L.cy0, b);
             const int32_t bw = kz_readlane_i(cw, b), bc = kz_readlane_i(cand, b);
             uint32_t c = 0;
@@ -602,6 +727,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             cb.th01 = kz_readlane_i(L.c.th01, b);
             cb.th12 = kz_readlane_i(L.c.th12, b);
             cb.th20 = kz_readlane_i(L.c.th20, b);
+            cb.small = kz_readlane_i(L.c.small, b);
             const int32_t bx0 = kz_readlane_i(L.cx0, b), by0 = kz_readlane_i(L.cy0, b);
             const int32_t bw = kz_readlane_i(cw, b), bc = kz_readlane_i(cand, b);
             uint32_t w = (uint32_t)kz_readlane_i((int32_t)excl, b);

@@ -830,6 +830,7 @@ static void reset_cb(CommandBuffer *cb) {
         for (void *p : cb->device_allocs) hipFree(p);
     }
     cb->graph.reset();
+    cb->graph.eager_runs = 0;
     cb->device_allocs.clear();
     cb->ops.clear();
     cb->st = RecordState();

@@ -141,6 +141,7 @@ def tiny_list_ctx(driver, monkeypatch):
     from kazan_amd.scenes.renderer import Context
     monkeypatch.setenv("KAZAN_LIST_CAPACITY_LIMIT", "10000")
     monkeypatch.setenv("KAZAN_PIPELINE", "balanced")
+    monkeypatch.setenv("KAZAN_AUTOTUNE", "0")  # 16x16 tiles, as the scene below assumes
     return Context(driver)
 
 
