@@ -14,6 +14,7 @@ this host's cores on the same input buffers).
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload grid_1080p]
 """
 import argparse
+import ctypes
 import json
 import os
 import statistics
@@ -99,8 +100,10 @@ def measure(scene, form, args):
     r.render(n_prof)
     ktimes = ctx.kernel_times()
     ctx.set_kernel_timing(False)
+    ts, gd = ctypes.c_uint32(), ctypes.c_uint32()
+    ctx.d.vkGetTuningKAZAN(ctx.device, ctypes.byref(ts), ctypes.byref(gd))
     return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r,
-            "cost_model": has_model}
+            "cost_model": has_model, "tuned": {"tile": ts.value, "grid_divisor": gd.value}}
 
 
 def main():
@@ -176,7 +179,9 @@ def main():
                           "gpu_ms_per_frame": v["stats"]["gpu_ms"] / args.steps,
                           "launches_per_frame": v["stats"]["launches"] / args.steps,
                           "split_entries_per_frame": v["stats"]["split_entries"] / args.steps,
-                          "cost_model": v["cost_model"]}
+                          "cost_model": v["cost_model"],
+                          "tuned": v["tuned"],
+                          "kernel_ms_per_frame": {k: t[0] / v["n_prof"] for k, t in v["ktimes"].items()}}
                       for f, v in forms.items()},
         },
         "roofline": {

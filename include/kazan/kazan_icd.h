@@ -154,8 +154,11 @@ typedef VkResult(VKAPI_PTR *PFN_vkFillClusterSceneKAZAN)(VkDevice device, VkBuff
  * hipGraph on its next submission and replayed as one graph launch
  * (KAZAN_GRAPHS=0 turns this off; re-recording, or any reallocation of the
  * device's scratch buffers, makes it capture again).
- * vkGetGraphStatsKAZAN: graph captures and graph launches so far. */
+ * vkGetGraphStatsKAZAN: graph captures and graph launches so far.
+ * vkGetTuningKAZAN: the tile size (8..64, 0 before any tuning finished) and
+ * persistent-grid divisor the most recently finished tuning chose. */
 typedef void(VKAPI_PTR *PFN_vkGetGraphStatsKAZAN)(VkDevice device, uint32_t *pCaptures, uint32_t *pLaunches);
+typedef void(VKAPI_PTR *PFN_vkGetTuningKAZAN)(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor);
 
 #ifdef __cplusplus
 }

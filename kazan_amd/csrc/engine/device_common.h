@@ -49,6 +49,7 @@ struct KzTexture {
 // One batch of draws that share a pipeline, dynamic state and descriptors.
 struct KzFrame {
     const KzDraw *draws;
+    const uint32_t *draw_begin;  // [ndraws] draws[i].tri_begin, densely
     uint32_t ndraws;
     uint32_t total_tris;
 
@@ -117,7 +118,7 @@ struct KzFrame {
     uint32_t *tile_frags;       // [tiles]
     uint32_t *tile_cycles;      // [tiles]
     float frag_weight;
-    uint32_t pad3;
+    uint32_t raster_slots;      // workgroups of the persistent raster (split threshold)
 };
 
 // Per-tile pair statistics (tile_acc) over the (triangle, tile) pairs; the
@@ -229,8 +230,6 @@ enum {
 #define KZ_ITEM_LEVEL(it) (((it) >> 4) & 3u)
 #define KZ_ITEM_SUB(it) ((it) & 15u)
 
-// Split thresholds of the balanced mode, in predicted cost units
-// (triangle-equivalents of raster work): a tile becomes 4 items above
-// KZ_SPLIT_4 and 16 items above KZ_SPLIT_16.
-#define KZ_SPLIT_4 1536.0f
-#define KZ_SPLIT_16 6144.0f
+// Balanced mode: a tile is split when its cost (list entries) exceeds the
+// frame's cost per resident raster workgroup, and never below this.
+#define KZ_SPLIT_MIN 512.0f

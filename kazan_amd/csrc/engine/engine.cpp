@@ -289,6 +289,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     const uint32_t ntiles = f.tiles_x * f.tiles_y;
     KZ_TRY(prepare_batch(op, shift));
     f.draws = op.d_draws;
+    f.draw_begin = op.d_draw_begin;
     f.tile_counts = tile_counts_;
     f.tile_offsets = tile_offsets_;
     f.tile_cursor = tile_cursor_;
@@ -313,6 +314,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
         f.split_offsets = split_offsets_;
         f.split_lists = split_lists_;
         f.max_chunks = max_chunks_;
+        f.raster_slots = std::max(1u, kern->persistent_blocks / std::max(grid_div, 1u));
     }
     // the cost model's inputs (and, in the static form, its labels) when a
     // model will run or tiles are being profiled
@@ -389,7 +391,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
         KZ_TRY(kz_launch_split(stream_, f, split_grid_));
         KZ_TRY(mark(KS_PLAN, true));
         KZ_TRY(mark(KS_RASTER, false));
-        const uint32_t grid = std::max(1u, std::min(kern->persistent_blocks / std::max(grid_div, 1u), 16 * ntiles));
+        const uint32_t grid = std::min(f.raster_slots, 16 * ntiles);
         KZ_TRY(hipModuleLaunchKernel(kern->raster_persistent, grid, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
         KZ_TRY(mark(KS_RASTER, true));
         launches += 5;
@@ -490,6 +492,8 @@ hipError_t Engine::pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool
     t.grid_div = best >= 0 ? t.cand[best].second : 1u;
     shift = t.shift;
     grid_div = t.grid_div;
+    last_tuned_shift_ = shift;
+    last_tuned_grid_div_ = grid_div;
     return hipSuccess;
 }
 

@@ -115,6 +115,7 @@ struct Op {
     KzFrame frame{};                // draws/scratch pointers filled at submit
     std::vector<KzDraw> draws;      // host copy, uploaded at end of recording
     KzDraw *d_draws = nullptr;      // device copy owned by the command buffer
+    uint32_t *d_draw_begin = nullptr;  // the draws' first triangles, in the same allocation
     // Copy: linear bytes; Copy2D: pitched rows
     const void *src = nullptr;
     void *dst = nullptr;
@@ -178,6 +179,9 @@ class Engine {
     hipError_t end_submit(hipEvent_t fence_event);
     uint32_t graph_launches() const { return graph_launches_; }
     uint32_t graph_captures() const { return graph_captures_; }
+    // the configuration the most recently finished tuning chose (0 = none yet)
+    uint32_t last_tuned_shift() const { return last_tuned_shift_; }
+    uint32_t last_tuned_grid_div() const { return last_tuned_grid_div_; }
     // After the submission's work is complete: read stats, adapt capacity.
     void collect_stats();
     const SubmitStats &last_stats() const { return last_; }
@@ -242,6 +246,7 @@ class Engine {
     bool autotune_ = true, graphs_ = true;
     uint64_t generation_ = 1;  // bumped whenever captured graphs go stale
     uint32_t graph_launches_ = 0, graph_captures_ = 0;
+    uint32_t last_tuned_shift_ = 0, last_tuned_grid_div_ = 0;
     uint64_t list_limit_ = 0;
     // balanced-mode scratch: up to 16 work items per tile; chunk tables
     // sized from the list capacity; sub-tile lists grown on demand

@@ -6,9 +6,13 @@
 //   kz_bin_hist      (triangle, tile) pairs per tile
 //   kz_scan          exclusive prefix sum of per-list counts
 //   kz_bin_scatter   writes triangle ids into the per-tile lists
-//   kz_sort_lists    sorts each list into primitive order (bitonic in LDS;
-//                    lists longer than one LDS load are sorted in LDS-sized
-//                    runs and merged with merge-path in HBM)
+//   kz_sort_short / kz_sort_lists  sort each list into primitive order:
+//                    up to 256 ids in one wave's registers; longer ones by a
+//                    workgroup, bitonic in LDS (16K ids), beyond that in
+//                    LDS-sized runs merged with merge-path in HBM
+//   kz_bin_hist_acc, kz_tile_features, kz_cost_mlp, kz_cost_train  the
+//                    learned cost model: statistics, features, bf16 MFMA
+//                    inference and training
 //   kz_plan, kz_split_*  load-balanced mode: split expensive tiles into 4 or
 //                    16 sub-tiles, cut their lists into ordered sub-tile
 //                    lists, order all work items most expensive first
@@ -30,16 +34,17 @@
 namespace {
 
 constexpr uint32_t kScanThreads = 1024;
-constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortCap = 4096;
+constexpr uint32_t kSortThreads = 1024;
+constexpr uint32_t kSortCap = 16384;  // ids one workgroup sorts in LDS (64 KB)
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kBinThreads = 512;
 constexpr uint32_t kHashLog = 12;
 constexpr uint32_t kHashSize = 1u << kHashLog;  // 32 KB of LDS: keys + values
 constexpr uint32_t kHashProbes = 32;
-constexpr uint32_t kHashLogAcc = 10;
-constexpr uint32_t kHashSizeAcc = 1u << kHashLogAcc;  // 44 KB: keys, values, KZ_ACC statistics
+constexpr uint32_t kAccThreads = 1024;
+constexpr uint32_t kHashLogAcc = 11;
+constexpr uint32_t kHashSizeAcc = 1u << kHashLogAcc;  // 64 KB: keys, values, 6 packed statistics words
 
 constexpr uint32_t kPlanThreads = 1024;
 constexpr uint32_t kSplitThreads = 256;
@@ -269,29 +274,45 @@ __global__ __launch_bounds__(kBinThreads) void kz_bin_hist(const uint64_t *rect,
     }
 }
 
-// Adds one pair's statistics to counters c[j * stride] (LDS slot or HBM).
-__device__ __forceinline__ void acc_pair(uint32_t *c, uint32_t stride, uint32_t area, bool shared, uint32_t z) {
-    atomicAdd(&c[KZ_ACC_AREA * stride], area);
+// Adds one pair's statistics to a tile's KZ_ACC counters in HBM.
+__device__ __forceinline__ void acc_pair_global(uint32_t *c, uint32_t area, bool shared, uint32_t z) {
+    atomicAdd(&c[KZ_ACC_AREA], area);
     const uint32_t bin = area <= 2 ? 0u : area <= 8 ? 1u : area <= 32 ? 2u : area <= 128 ? 3u : 4u;
-    atomicAdd(&c[(KZ_ACC_HIST + bin) * stride], 1u);
-    if (shared) atomicAdd(&c[KZ_ACC_SHARED * stride], 1u);
-    atomicMax(&c[KZ_ACC_ZMIN * stride], 65535u - (z & 0xFFFFu));
-    atomicMax(&c[KZ_ACC_ZMAX * stride], z >> 16);
+    atomicAdd(&c[KZ_ACC_HIST + bin], 1u);
+    if (shared) atomicAdd(&c[KZ_ACC_SHARED], 1u);
+    atomicMax(&c[KZ_ACC_ZMIN], 65535u - (z & 0xFFFFu));
+    atomicMax(&c[KZ_ACC_ZMAX], z >> 16);
+}
+
+// The same in an LDS slot of kAccWords words with the six small counters
+// packed in halves (a workgroup's chunk of at most 65535 triangles gives a
+// tile at most that many pairs): [area, h0 | h1 << 16, h2 | h3 << 16,
+// h4 | shared << 16, max(65535 - zmin), max(zmax)].
+constexpr uint32_t kAccWords = 6;
+
+__device__ __forceinline__ void acc_pair_lds(uint32_t *c, uint32_t stride, uint32_t area, bool shared, uint32_t z) {
+    atomicAdd(&c[0], area);
+    const uint32_t bin = area <= 2 ? 0u : area <= 8 ? 1u : area <= 32 ? 2u : area <= 128 ? 3u : 4u;
+    atomicAdd(&c[(1 + bin / 2) * stride], (1u << (16 * (bin & 1))) + (shared && bin == 4 ? 1u << 16 : 0u));
+    if (shared && bin != 4) atomicAdd(&c[3 * stride], 1u << 16);
+    atomicMax(&c[4 * stride], 65535u - (z & 0xFFFFu));
+    atomicMax(&c[5 * stride], z >> 16);
 }
 
 // kz_bin_hist plus the per-tile pair statistics of the cost model
-// (KZ_ACC_* in acc[tile * KZ_ACC + j], zeroed by the caller), on a smaller
-// table (KZ_ACC more counters per slot).
-__global__ __launch_bounds__(kBinThreads) void kz_bin_hist_acc(const uint64_t *rect, const uint32_t *tri_z,
+// (KZ_ACC_* in acc[tile * KZ_ACC + j], zeroed by the caller). Large chunks
+// (up to 8192 triangles on 1024 threads) keep the HBM flush, one set of
+// counters per (workgroup, tile), to a few hundred workgroups.
+__global__ __launch_bounds__(kAccThreads) void kz_bin_hist_acc(const uint64_t *rect, const uint32_t *tri_z,
                                                                uint32_t ntris, uint32_t chunk, uint32_t sh,
                                                                uint32_t tiles_x, uint32_t *counts, uint32_t *acc) {
-    __shared__ uint32_t s_key[kHashSizeAcc], s_val[kHashSizeAcc], s_acc[KZ_ACC * kHashSizeAcc];
+    __shared__ uint32_t s_key[kHashSizeAcc], s_val[kHashSizeAcc], s_acc[kAccWords * kHashSizeAcc];
     hash_clear<kHashLogAcc>(s_key, s_val);
-    for (uint32_t i = threadIdx.x; i < KZ_ACC * kHashSizeAcc; i += kBinThreads) s_acc[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kAccWords * kHashSizeAcc; i += kAccThreads) s_acc[i] = 0;
     __syncthreads();
     const uint32_t lane = lane_id();
     const uint32_t t0 = blockIdx.x * chunk, t1 = min(t0 + chunk, ntris);
-    for (uint32_t b = t0; b < t1; b += kBinThreads) {
+    for (uint32_t b = t0; b < t1; b += kAccThreads) {
         const uint32_t g = b + threadIdx.x;
         TileWalk w;
         w.init(g < t1 ? rect[g] : KZ_RECT_EMPTY, sh);
@@ -308,23 +329,27 @@ __global__ __launch_bounds__(kBinThreads) void kz_bin_hist_acc(const uint64_t *r
             slot = (uint32_t)__shfl((int)slot, (int)r.head, 64);
             if (key != kNone) {
                 if (slot != kNone)
-                    acc_pair(&s_acc[slot], kHashSizeAcc, w.overlap(sh), w.n > 1, z);
+                    acc_pair_lds(&s_acc[slot], kHashSizeAcc, w.overlap(sh), w.n > 1, z);
                 else
-                    acc_pair(&acc[(size_t)key * KZ_ACC], 1, w.overlap(sh), w.n > 1, z);
+                    acc_pair_global(&acc[(size_t)key * KZ_ACC], w.overlap(sh), w.n > 1, z);
             }
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kHashSizeAcc; i += kBinThreads) {
+    for (uint32_t i = threadIdx.x; i < kHashSizeAcc; i += kAccThreads) {
         const uint32_t v = s_val[i];
-        if (v) {
-            uint32_t *dst = &acc[(size_t)s_key[i] * KZ_ACC];
-            atomicAdd(&counts[s_key[i]], v);
-            for (uint32_t j = 0; j < KZ_ACC_ZMIN; ++j)
-                if (s_acc[j * kHashSizeAcc + i]) atomicAdd(&dst[j], s_acc[j * kHashSizeAcc + i]);
-            atomicMax(&dst[KZ_ACC_ZMIN], s_acc[KZ_ACC_ZMIN * kHashSizeAcc + i]);
-            atomicMax(&dst[KZ_ACC_ZMAX], s_acc[KZ_ACC_ZMAX * kHashSizeAcc + i]);
-        }
+        if (!v) continue;
+        uint32_t *dst = &acc[(size_t)s_key[i] * KZ_ACC];
+        atomicAdd(&counts[s_key[i]], v);
+        const uint32_t w[kAccWords] = {s_acc[i], s_acc[kHashSizeAcc + i], s_acc[2 * kHashSizeAcc + i],
+                                       s_acc[3 * kHashSizeAcc + i], s_acc[4 * kHashSizeAcc + i],
+                                       s_acc[5 * kHashSizeAcc + i]};
+        const uint32_t unpacked[KZ_ACC_ZMIN] = {w[0], w[1] & 0xFFFFu, w[1] >> 16, w[2] & 0xFFFFu, w[2] >> 16,
+                                               w[3] & 0xFFFFu, w[3] >> 16};
+        for (uint32_t j = 0; j < KZ_ACC_ZMIN; ++j)
+            if (unpacked[j]) atomicAdd(&dst[j], unpacked[j]);
+        atomicMax(&dst[KZ_ACC_ZMIN], w[4]);
+        atomicMax(&dst[KZ_ACC_ZMAX], w[5]);
     }
 }
 
@@ -516,13 +541,20 @@ __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *co
 
 // --- load-balanced mode -------------------------------------------------
 
-// Split level of a non-empty tile from its cost in list entries: the binned
-// triangle count, plus the model's predicted fragments weighted as entries
-// when a cost model ran. A tile whose list overflowed stays whole: its
-// triangles come from the in-order scan.
-__device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uint32_t n, float &cost) {
-    cost = (float)n + (f.tile_pred ? f.frag_weight * f.tile_pred[t] : 0.0f);
-    uint32_t level = cost > KZ_SPLIT_16 ? 2u : cost > KZ_SPLIT_4 ? 1u : 0u;
+// A tile's cost in list entries: the binned triangle count, plus the
+// model's predicted fragments weighted as entries when a cost model ran.
+__device__ __forceinline__ float tile_cost(const KzFrame &f, uint32_t t, uint32_t n) {
+    return (float)n + (f.tile_pred ? f.frag_weight * f.tile_pred[t] : 0.0f);
+}
+
+// Split level of a non-empty tile: a tile above the fair share `share`
+// (the frame's cost over the raster's resident workgroups) would hold up
+// the frame's end, so it becomes 4 items, or 16 above 4 shares. A tile
+// whose list overflowed stays whole: its triangles come from the in-order
+// scan.
+__device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uint32_t n, float share, float &cost) {
+    cost = tile_cost(f, t, n);
+    uint32_t level = cost > 4.0f * share ? 2u : cost > share ? 1u : 0u;
     while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px wide
     if ((uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
     return level;
@@ -539,14 +571,26 @@ __device__ __forceinline__ uint32_t cost_bucket(float cost) {
 // a bucket is arbitrary; pixels never depend on it).
 __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntiles) {
     __shared__ uint32_t s_bucket[32], s_cursor[32], s_split, s_chunks;
+    __shared__ float s_total[kPlanThreads / 64];
     if (threadIdx.x < 32) s_bucket[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_split = s_chunks = 0;
+    // the frame's total cost -> the fair share of one resident workgroup
+    float part = 0.0f;
+    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
+        const uint32_t n = f.tile_counts[t];
+        if (n) part += tile_cost(f, t, n);
+    }
+    for (uint32_t d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+    if ((threadIdx.x & 63) == 0) s_total[threadIdx.x >> 6] = part;
     __syncthreads();
+    float total = 0.0f;
+    for (uint32_t w = 0; w < kPlanThreads / 64; ++w) total += s_total[w];
+    const float share = fmaxf(total / (float)max(f.raster_slots, 1u), KZ_SPLIT_MIN);
     for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
         const uint32_t n = f.tile_counts[t];
         if (n == 0) continue;
         float cost;
-        const uint32_t level = plan_level(f, t, n, cost);
+        const uint32_t level = plan_level(f, t, n, share, cost);
         f.tile_level[t] = level;
         const uint32_t items = 1u << (2 * level);
         atomicAdd(&s_bucket[cost_bucket(cost / (float)items)], items);
@@ -574,7 +618,7 @@ __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntil
         const uint32_t n = f.tile_counts[t];
         if (n == 0) continue;
         float cost;
-        const uint32_t level = plan_level(f, t, n, cost);
+        const uint32_t level = plan_level(f, t, n, share, cost);
         const uint32_t items = 1u << (2 * level);
         const uint32_t pos = atomicAdd(&s_cursor[cost_bucket(cost / (float)items)], items);
         for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t, level, s);
@@ -1196,8 +1240,10 @@ hipError_t kz_launch_bin_hist(hipStream_t st, const KzFrame &f) {
 
 hipError_t kz_launch_bin_hist_acc(hipStream_t st, const KzFrame &f) {
     if (f.total_tris == 0) return hipSuccess;
-    const uint32_t chunk = min(bin_chunk(f.total_tris), 1024u);
-    kz_bin_hist_acc<<<(f.total_tris + chunk - 1) / chunk, kBinThreads, 0, st>>>(
+    // a few hundred workgroups: 1024..8192 triangles each
+    uint32_t chunk = (f.total_tris / 512 + kAccThreads - 1) / kAccThreads * kAccThreads;
+    chunk = std::min(std::max(chunk, kAccThreads), 8192u);
+    kz_bin_hist_acc<<<(f.total_tris + chunk - 1) / chunk, kAccThreads, 0, st>>>(
         f.tri_rect, f.tri_z, f.total_tris, chunk, f.tile_shift, f.tiles_x, f.tile_counts, f.tile_acc);
     return hipGetLastError();
 }

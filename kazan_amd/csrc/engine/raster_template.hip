@@ -94,11 +94,13 @@ __device__ __forceinline__ unsigned long long kz_wave_sum_u64(unsigned long long
 // ---------------------------------------------------------------------------
 // primitive assembly
 
+// The draw record of batch triangle g: binary search over the draws' first
+// triangles (a dense 4-byte array, so the probes of a wave share cache lines).
 __device__ __forceinline__ uint32_t kz_find_draw(const KzFrame &f, uint32_t g) {
     uint32_t lo = 0, hi = f.ndraws - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        if (f.draws[mid].tri_begin <= g) lo = mid;
+        if (f.draw_begin[mid] <= g) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -288,6 +290,16 @@ __device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, 
 // ---------------------------------------------------------------------------
 // binning, pass 1: vertex shader + setup of every triangle, to its pixel box
 // (the tile lists are built from the boxes by the prebuilt binning kernels)
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2. Renumbering block b (on XCD b % 8) into a contiguous range per XCD
+// gives every XCD a compact band of tiles, so that triangles and texels
+// shared by neighbouring tiles are fetched into one L2, not eight. A
+// bijection of [0, n) for any n.
+__device__ __forceinline__ uint32_t kz_xcd_remap(uint32_t b, uint32_t n) {
+    const uint32_t xcd = b & 7u, k = b >> 3, q = n >> 3, r = n & 7u;
+    return xcd * q + min(xcd, r) + k;
+}
 
 __device__ __forceinline__ unsigned long long *kz_stats_stripe(const KzFrame &f) {
     return f.stats + (size_t)(blockIdx.x & (KZ_STAT_STRIPES - 1)) * KZ_STAT_COUNT;
@@ -644,13 +656,20 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         const bool small = cand <= KZ_SMALL_CAND;
 
         // --- phase 1: covered pixels per triangle ------------------------
+        // (small triangles keep their sample masks, KZ_SAMPLES bits per
+        // candidate pixel, for phase 2: at most 16 x 4 bits)
         uint32_t cnt = 0;
+        uint64_t cmask = 0;
         {
             const int32_t maxc = kz_wave_max(small ? cand : 0, lane);
             int32_t cx = L.cx0, cy = L.cy0;
             for (int32_t j = 0; j < maxc; ++j) {
                 if (small && j < cand) {
-                    if (kz_cover_mask(L.t, L.c, cx, cy)) ++cnt;
+                    const uint32_t m = kz_cover_mask(L.t, L.c, cx, cy);
+                    if (m) {
+                        ++cnt;
+                        cmask |= (uint64_t)m << (j * KZ_SAMPLES);
+                    }
                     if (++cx > L.cx1) {
                         cx = L.cx0;
                         ++cy;
@@ -697,11 +716,11 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         const uint32_t excl = incl - cnt;
 
         // --- phase 2: write the fragment queue in primitive order ---------
-        if (taken && small && cand > 0) {
+        if (taken && small && cmask) {
             uint32_t w = excl;
             int32_t cx = L.cx0, cy = L.cy0;
             for (int32_t j = 0; j < cand; ++j) {
-                const uint32_t m = kz_cover_mask(L.t, L.c, cx, cy);
+                const uint32_t m = (uint32_t)(cmask >> (j * KZ_SAMPLES)) & ((1u << KZ_SAMPLES) - 1u);
                 if (m) s_frag[w++] = kz_pack_frag((uint32_t)((cy - Y0) * KZ_TS + (cx - X0)), lane, m);
                 if (++cx > L.cx1) {
                     cx = L.cx0;
@@ -891,7 +910,7 @@ extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];
     __shared__ uint32_t s_frag[KZ_FRAG_CAP];
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = kz_xcd_remap(blockIdx.x, gridDim.x);
     const int32_t X0 = (int32_t)((tile % f.tiles_x) << KZ_TILE_SHIFT);
     const int32_t Y0 = (int32_t)((tile / f.tiles_x) << KZ_TILE_SHIFT);
     const uint32_t off = f.tile_offsets[tile], n = f.tile_counts[tile];
@@ -925,7 +944,7 @@ extern "C" __global__ __launch_bounds__(64) void kz_raster_persistent(KzFrame f)
     __shared__ uint32_t s_frag[KZ_FRAG_CAP];
     const uint32_t lane = kz_lane();
     const uint32_t n_items = f.ctrl[KZ_CTRL_ITEMS];
-    uint32_t q = blockIdx.x;
+    uint32_t q = kz_xcd_remap(blockIdx.x, gridDim.x);  // first round: one item per workgroup
     while (q < n_items) {
         const uint32_t geo = f.queue[q];
         const uint32_t tile = KZ_ITEM_TILE(geo), level = KZ_ITEM_LEVEL(geo), sub = KZ_ITEM_SUB(geo);

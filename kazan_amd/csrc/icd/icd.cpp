@@ -422,6 +422,12 @@ VKAPI_ATTR void VKAPI_CALL vkGetGraphStatsKAZAN(VkDevice device, uint32_t *pCapt
     if (pLaunches) *pLaunches = e.graph_launches();
 }
 
+VKAPI_ATTR void VKAPI_CALL vkGetTuningKAZAN(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor) {
+    const Engine &e = *from<Device>(device)->engine;
+    if (pTileSize) *pTileSize = e.last_tuned_shift() ? 1u << e.last_tuned_shift() : 0u;
+    if (pGridDivisor) *pGridDivisor = e.last_tuned_grid_div();
+}
+
 VKAPI_ATTR VkResult VKAPI_CALL vkFillClusterSceneKAZAN(VkDevice device, VkBuffer buffer, VkDeviceSize offset,
                                                        const KazanClusterScene *p) {
     Device *d = from<Device>(device);
@@ -893,13 +899,19 @@ VKAPI_ATTR VkResult VKAPI_CALL vkEndCommandBuffer(VkCommandBuffer commandBuffer)
     // upload each batch's draw records once; every submission reuses them
     for (Op &op : cb->ops) {
         if (op.kind != Op::Batch || op.draws.empty()) continue;
+        // the records, then their first triangles as a dense array
+        const size_t n = op.draws.size(), rec_bytes = n * sizeof(KzDraw);
+        std::vector<char> host(rec_bytes + n * sizeof(uint32_t));
+        std::memcpy(host.data(), op.draws.data(), rec_bytes);
+        uint32_t *begin = (uint32_t *)(host.data() + rec_bytes);
+        for (size_t i = 0; i < n; ++i) begin[i] = op.draws[i].tri_begin;
         void *p = nullptr;
-        const size_t bytes = op.draws.size() * sizeof(KzDraw);
-        if (hipMalloc(&p, bytes) != hipSuccess) return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+        if (hipMalloc(&p, host.size()) != hipSuccess) return VK_ERROR_OUT_OF_DEVICE_MEMORY;
         cb->device_allocs.push_back(p);
-        if (hipMemcpy(p, op.draws.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return VK_ERROR_DEVICE_LOST;
+        if (hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return VK_ERROR_DEVICE_LOST;
         op.d_draws = (KzDraw *)p;
-        op.frame.ndraws = (uint32_t)op.draws.size();
+        op.d_draw_begin = (uint32_t *)((char *)p + rec_bytes);
+        op.frame.ndraws = (uint32_t)n;
     }
     cb->recording = false;
     return VK_SUCCESS;
@@ -1437,6 +1449,7 @@ std::unordered_map<std::string, Entry> &entries() {
         KZ_D(vkHasCostModelKAZAN);
         KZ_D(vkFillClusterSceneKAZAN);
         KZ_D(vkGetGraphStatsKAZAN);
+        KZ_D(vkGetTuningKAZAN);
 #undef KZ_I
 #undef KZ_P
 #undef KZ_D
