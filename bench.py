@@ -11,7 +11,15 @@ frame time, a roofline object for the dominant kernel (HIP-event timed on the
 queue's stream) and a cpu_baseline object (the C++ CPU restatement, run on
 this host's cores on the same input buffers).
 
-  python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload grid_1080p]
+Each pipeline form in --forms is measured on its own device: "learned"
+(balanced form planned by the cost model), "balanced_static" (balanced form,
+split by binned counts only) and "plain" (one workgroup per tile). The first
+form is the reported value; config.forms holds all of them and
+config.<form>_vs_plain the throughput ratios.
+
+  python bench.py [--gpus 1] [--steps 50] [--warmup 10] [--workload grid_1080p]
+                  [--forms learned,balanced_static,plain]
+  python bench.py --copy-probe 1073741824   # HBM counter calibration only
 """
 import argparse
 import ctypes
@@ -106,6 +114,21 @@ def measure(scene, form, args):
             "cost_model": has_model, "tuned": {"tile": ts.value, "grid_divisor": gd.value}}
 
 
+def copy_probe(nbytes, reps=3):
+    """A plain device-to-device copy of known size, `reps` times: the scale
+    for rocprofv3's FETCH_SIZE / WRITE_SIZE counters (run it under the same
+    counter as the frame and divide by reps x nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4, "reps": reps}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--forms", default="learned,balanced_static,plain",
@@ -118,7 +141,11 @@ def main():
     ap.add_argument("--workload", default="grid_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 3 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU: the draw path runs on a single MI355X (multi-GPU sharding is a "
               "design note, not a measured configuration); run with --gpus 1", file=sys.stderr)

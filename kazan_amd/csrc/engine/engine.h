@@ -276,8 +276,11 @@ class Engine {
     uint32_t *tile_counts_ = nullptr, *tile_offsets_ = nullptr, *tile_cursor_ = nullptr;
     uint32_t *tile_lists_ = nullptr, *sort_tmp_ = nullptr;
     uint64_t *tri_rect_ = nullptr;
-    // setup records (KAZAN_SETUP_RECORDS=0: the raster recomputes the setup)
-    bool setup_records_ = true;
+    // setup records written by binning for the raster (KAZAN_SETUP_RECORDS=1);
+    // off by default: a record (96-112 B) costs as many HBM bytes as the
+    // vertices it replaces, and writing it slows binning more than the
+    // skipped setup speeds up the raster
+    bool setup_records_ = false;
     uint32_t *tri_rec_ = nullptr;
     uint64_t cap_rec_words_ = 0;
     unsigned long long *stats_d_ = nullptr, *stats_h_ = nullptr;

@@ -543,8 +543,13 @@ __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *co
 
 // A tile's cost in list entries: the binned triangle count, plus the
 // model's predicted fragments weighted as entries when a cost model ran.
+// The prediction is capped at what n triangles can cover (every sample of
+// the tile, at most 4 per pixel, once each), so an input far outside the
+// training range cannot make a tile's cost, or the frame's total, infinite.
 __device__ __forceinline__ float tile_cost(const KzFrame &f, uint32_t t, uint32_t n) {
-    return (float)n + (f.tile_pred ? f.frag_weight * f.tile_pred[t] : 0.0f);
+    if (!f.tile_pred) return (float)n;
+    const float cap = (float)n * (float)(4u << (2 * f.tile_shift));
+    return (float)n + f.frag_weight * fminf(f.tile_pred[t], cap);
 }
 
 // Split level of a non-empty tile: a tile above the fair share `share`

@@ -189,11 +189,13 @@ def test_tile_profile_counts(driver, monkeypatch, oracle):
     assert (feat[:, 8] >= 0).all() and (feat[:, 8] <= 1).all()
 
 
-@pytest.mark.parametrize("b4", [20.0, -20.0])
+@pytest.mark.parametrize("b4", [20.0, 200.0, -20.0])
 def test_balanced_pixels_do_not_depend_on_the_model(driver, monkeypatch, oracle, b4):
     # b4 = 20: every tile predicted at a million fragments, so every tile is
-    # cut into 16 sub-tiles; b4 = -20: no fragments predicted, so tiles split
-    # by their binned-triangle count alone, as without a model
+    # cut into 16 sub-tiles; b4 = 200: a prediction beyond fp32 (exp2 -> inf),
+    # which the plan caps at the tile's samples x triangles, so tiles still
+    # split; b4 = -20: no fragments predicted, so tiles split by their
+    # binned-triangle count alone, as without a model
     ctx = _ctx(driver, monkeypatch, "balanced")
     blob = _random_blob(8, b4=b4, frag_weight=1.0)
     ctx.d.check("vkSetCostModelKAZAN", ctx.device, blob, len(blob))

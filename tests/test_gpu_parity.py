@@ -247,6 +247,20 @@ def test_phong_msaa4_stress_small(mctx, oracle):
     assert ost["fragments"] > 0
 
 
+@pytest.mark.parametrize("form", ["balanced", "plain"])
+def test_setup_records(driver, monkeypatch, oracle, form):
+    # the opt-in path where binning stores each triangle's setup and the
+    # raster loads it instead of running the vertex shader and setup again
+    from kazan_amd.scenes.renderer import Context
+    monkeypatch.setenv("KAZAN_PIPELINE", form)
+    monkeypatch.setenv("KAZAN_SETUP_RECORDS", "1")
+    ctx = Context(driver)
+    phong = g.stress_4k(16, 4000, 160, 100, 4)
+    phong.vertices = phong.vertices.copy()
+    for s in (g.jittered_grid(32, 64, 160, 120, 64), g.gaussian_cluster(60_000, 160, 128), phong):
+        check_parity(ctx, oracle, s)
+
+
 def test_phong_big_triangles(mctx, oracle):
     from kazan_amd.scenes import rng
     counts = g.zipf_counts(16, 800)
