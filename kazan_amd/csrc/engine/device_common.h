@@ -82,6 +82,7 @@ struct KzFrame {
     uint32_t pad0;
     uint64_t *tri_rect;      // [total_tris] packed pixel bounding box or ~0
     uint32_t *tri_rec;       // [total_tris * KZ_REC_WORDS_FOR(varyings)] setup records, or null
+    uint32_t *block_pairs;   // [ceil(total_tris / 256)] (triangle, tile) pairs per binning block, or null
     unsigned long long *stats;  // KZ_STAT_* counters
 
     // Load-balanced mode. A work item is a square sub-tile of a screen
@@ -211,6 +212,17 @@ enum {
 
 // Entries of a split tile's list that one wave cuts into sub-tile lists.
 #define KZ_SPLIT_CHUNK 1024u
+
+// Radix binning: kz_bin_count adds the (triangle, tile) pairs of each block
+// of KZ_BIN_BLOCK triangles; pairs are generated KZ_PAIR_CHUNK triangles
+// (KZ_PAIR_CHUNK / KZ_BIN_BLOCK blocks) per workgroup.
+#define KZ_BIN_BLOCK 256u
+#define KZ_PAIR_CHUNK 4096u
+enum {
+    KZ_BIN_PAIRS = 0,  // pairs generated and sorted this batch (<= list capacity)
+    KZ_BIN_CUT = 1,    // first pair chunk whose pairs did not fit (chunks when all fit)
+    KZ_BIN_WORDS = 4
+};
 
 // Statistics are accumulated in KZ_STAT_STRIPES copies (a workgroup adds to
 // copy blockIdx % KZ_STAT_STRIPES) so that no single address takes every

@@ -47,6 +47,9 @@ Engine::~Engine() {
     hipFree(sort_tmp_);
     hipFree(tri_rect_);
     hipFree(tri_rec_);
+    for (uint32_t *p : {block_pairs_, chunk_offsets_, bin_ctrl_, tile_flag_, pair_keys_[0], pair_keys_[1],
+                        pair_vals_[0], pair_vals_[1], radix_table_, radix_totals_})
+        hipFree(p);
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
                         chunk_tile_, chunk_counts_, split_lists_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
         hipFree(p);
@@ -79,6 +82,13 @@ bool Engine::init(int hip_device, std::string &err) {
     if (const char *e = std::getenv("KAZAN_AUTOTUNE")) autotune_ = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("KAZAN_GRAPHS")) graphs_ = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("KAZAN_SETUP_RECORDS")) setup_records_ = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("KAZAN_BINNING")) radix_ = std::strcmp(e, "sort") != 0;
+    if (radix_ &&
+        ((e = hipMalloc((void **)&bin_ctrl_, KZ_BIN_WORDS * sizeof(uint32_t))) != hipSuccess ||
+         (e = hipMalloc((void **)&radix_totals_, kz_radix_digits_max() * sizeof(uint32_t))) != hipSuccess)) {
+        err = hipGetErrorString(e);
+        return false;
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) == hipSuccess && cus > 0)
         split_grid_ = 4u * (uint32_t)cus;
@@ -220,21 +230,44 @@ hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
         const uint32_t t = std::max(tiles, cap_tiles_ * 2);
         KZ_TRY(regrow(tile_counts_, t));
         KZ_TRY(regrow(tile_offsets_, t));
-        KZ_TRY(regrow(tile_cursor_, t));
+        if (radix_) {
+            KZ_TRY(regrow(tile_flag_, t));
+            KZ_TRY(kz_launch_fill_u32(stream_, tile_flag_, t, 0u));  // kz_scan clears what it reads
+        } else {
+            KZ_TRY(regrow(tile_cursor_, t));
+        }
         cap_tiles_ = t;
     }
     if (tris > cap_tris_) {
         const uint32_t t = std::max(tris, cap_tris_ + cap_tris_ / 2);
         KZ_TRY(regrow(tri_rect_, t));
+        if (radix_) {
+            KZ_TRY(regrow(block_pairs_, (t + KZ_BIN_BLOCK - 1) / KZ_BIN_BLOCK));
+            KZ_TRY(regrow(chunk_offsets_, (t + KZ_PAIR_CHUNK - 1) / KZ_PAIR_CHUNK));
+        }
         cap_tris_ = t;
     }
     if (want_list > list_capacity_) {
         const uint64_t c = std::min<uint64_t>(std::max<uint64_t>(want_list, (uint64_t)list_capacity_ * 3 / 2),
                                               0xFFFFFFF0ull);
-        KZ_TRY(regrow(tile_lists_, c));
-        KZ_TRY(regrow(sort_tmp_, c));
-        list_capacity_ = (uint32_t)c;
+        KZ_TRY(alloc_lists(c));
     }
+    return hipSuccess;
+}
+
+hipError_t Engine::alloc_lists(uint64_t capacity) {
+    list_capacity_ = 0;
+    KZ_TRY(regrow(tile_lists_, capacity));
+    if (radix_) {
+        for (int i = 0; i < 2; ++i) {
+            KZ_TRY(regrow(pair_keys_[i], capacity));
+            KZ_TRY(regrow(pair_vals_[i], capacity));
+        }
+        KZ_TRY(regrow(radix_table_, (uint64_t)kz_radix_digits_max() * kz_radix_blocks(capacity)));
+    } else {
+        KZ_TRY(regrow(sort_tmp_, capacity));
+    }
+    list_capacity_ = (uint32_t)capacity;
     return hipSuccess;
 }
 
@@ -340,27 +373,55 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     const uint32_t tri_blocks = (f.total_tris + 255) / 256;
     uint32_t launches = 0;
 
-    // binning: pixel boxes, pairs per tile, offsets, lists, primitive order
+    // binning: pixel boxes, then the per-tile lists in primitive order
     KZ_TRY(mark(KS_CLEAR_COUNTS, false));
     KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
     if (acc) KZ_TRY(kz_launch_fill_u32(stream_, tile_acc_, (uint64_t)KZ_ACC * ntiles, 0u));
     KZ_TRY(mark(KS_CLEAR_COUNTS, true));
-    KZ_TRY(mark(KS_BIN_COUNT, false));
-    KZ_TRY(hipModuleLaunchKernel(kern->bin_count, tri_blocks, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
-    KZ_TRY(mark(KS_BIN_COUNT, true));
-    KZ_TRY(mark(KS_BIN_HIST, false));
-    KZ_TRY(acc ? kz_launch_bin_hist_acc(stream_, f) : kz_launch_bin_hist(stream_, f));
-    KZ_TRY(mark(KS_BIN_HIST, true));
-    KZ_TRY(mark(KS_BIN_SCAN, false));
-    KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_));
-    KZ_TRY(mark(KS_BIN_SCAN, true));
-    KZ_TRY(mark(KS_BIN_SCATTER, false));
-    KZ_TRY(kz_launch_bin_scatter(stream_, f));
-    KZ_TRY(mark(KS_BIN_SCATTER, true));
-    KZ_TRY(mark(KS_BIN_SORT, false));
-    KZ_TRY(kz_launch_sort_lists(stream_, f, ntiles));
-    KZ_TRY(mark(KS_BIN_SORT, true));
-    launches += acc ? 8 : 7;
+    launches += acc ? 2 : 1;
+    if (radix_) {
+        // pairs in primitive order, a stable radix sort by tile (whose last
+        // pass writes the lists and counts), offsets
+        f.block_pairs = block_pairs_;
+        KzRadixScratch rs{chunk_offsets_, bin_ctrl_, tile_flag_, {pair_keys_[0], pair_keys_[1]},
+                          {pair_vals_[0], pair_vals_[1]}, radix_table_, radix_totals_,
+                          kz_radix_blocks(list_capacity_)};
+        KZ_TRY(mark(KS_BIN_COUNT, false));
+        KZ_TRY(hipModuleLaunchKernel(kern->bin_count, tri_blocks, 1, 1, KZ_BIN_BLOCK, 1, 1, 0, stream_, args,
+                                     nullptr));
+        KZ_TRY(mark(KS_BIN_COUNT, true));
+        KZ_TRY(mark(KS_BIN_HIST, false));
+        KZ_TRY(kz_launch_bin_pairs(stream_, f, rs));
+        KZ_TRY(mark(KS_BIN_HIST, true));
+        KZ_TRY(mark(KS_BIN_SCATTER, false));
+        KZ_TRY(kz_launch_bin_radix(stream_, f, rs, ntiles));
+        KZ_TRY(mark(KS_BIN_SCATTER, true));
+        KZ_TRY(mark(KS_BIN_SCAN, false));
+        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, nullptr, ntiles, list_capacity_, nullptr,
+                              tile_flag_));
+        KZ_TRY(mark(KS_BIN_SCAN, true));
+        launches += 1 + 2 + 3 * kz_radix_passes(ntiles) + 1;
+    } else {
+        // per-tile counts, offsets, an unordered scatter, a sort of each list
+        KZ_TRY(mark(KS_BIN_COUNT, false));
+        KZ_TRY(hipModuleLaunchKernel(kern->bin_count, tri_blocks, 1, 1, KZ_BIN_BLOCK, 1, 1, 0, stream_, args,
+                                     nullptr));
+        KZ_TRY(mark(KS_BIN_COUNT, true));
+        KZ_TRY(mark(KS_BIN_HIST, false));
+        KZ_TRY(acc ? kz_launch_bin_hist_acc(stream_, f) : kz_launch_bin_hist(stream_, f));
+        KZ_TRY(mark(KS_BIN_HIST, true));
+        KZ_TRY(mark(KS_BIN_SCAN, false));
+        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_,
+                              nullptr));
+        KZ_TRY(mark(KS_BIN_SCAN, true));
+        KZ_TRY(mark(KS_BIN_SCATTER, false));
+        KZ_TRY(kz_launch_bin_scatter(stream_, f));
+        KZ_TRY(mark(KS_BIN_SCATTER, true));
+        KZ_TRY(mark(KS_BIN_SORT, false));
+        KZ_TRY(kz_launch_sort_lists(stream_, f, ntiles));
+        KZ_TRY(mark(KS_BIN_SORT, true));
+        launches += 6;
+    }
 
     if (!balanced_) {
         if (acc) {
@@ -648,10 +709,7 @@ void Engine::collect_stats() {
         if (list_limit_) need = std::min<uint64_t>(need, list_limit_);
         if (need > list_capacity_ && need < 0xFFFFFFF0ull) {
             hipStreamSynchronize(stream_);
-            if (regrow(tile_lists_, need) == hipSuccess && regrow(sort_tmp_, need) == hipSuccess)
-                list_capacity_ = (uint32_t)need;
-            else
-                list_capacity_ = 0, cap_tris_ = 0;  // next ensure_scratch reallocates
+            if (alloc_lists(need) != hipSuccess) list_capacity_ = 0, cap_tris_ = 0;  // next ensure_scratch reallocates
         }
     }
     // Likewise for sub-tile lists (their overflow fallback filters the

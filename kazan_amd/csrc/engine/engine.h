@@ -276,6 +276,15 @@ class Engine {
     uint32_t *tile_counts_ = nullptr, *tile_offsets_ = nullptr, *tile_cursor_ = nullptr;
     uint32_t *tile_lists_ = nullptr, *sort_tmp_ = nullptr;
     uint64_t *tri_rect_ = nullptr;
+    // Binning: radix (pairs in primitive order, stable radix sort by tile)
+    // or, with KAZAN_BINNING=sort, per-tile counts, an unordered scatter and
+    // a sort of every list.
+    bool radix_ = true;
+    uint32_t *block_pairs_ = nullptr, *chunk_offsets_ = nullptr, *bin_ctrl_ = nullptr, *tile_flag_ = nullptr;
+    uint32_t *pair_keys_[2] = {nullptr, nullptr}, *pair_vals_[2] = {nullptr, nullptr};
+    uint32_t *radix_table_ = nullptr, *radix_totals_ = nullptr;
+    // the buffers sized by the list capacity
+    hipError_t alloc_lists(uint64_t capacity);
     // setup records written by binning for the raster (KAZAN_SETUP_RECORDS=1);
     // off by default: a record (96-112 B) costs as many HBM bytes as the
     // vertices it replaces, and writing it slows binning more than the

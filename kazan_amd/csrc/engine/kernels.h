@@ -8,10 +8,33 @@
 hipError_t kz_launch_fill_u32(hipStream_t st, uint32_t *dst, uint64_t n, uint32_t value);
 hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width, int32_t x0, int32_t y0, uint32_t w,
                                    uint32_t h, uint32_t spp, uint32_t value);
-// Exclusive scan of counts[0, n) into offsets and cursor; adds the total to
-// the list-entry statistic and flags an overflow beyond `capacity`.
-hipError_t kz_launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats);
+// Exclusive scan of counts[0, n) into offsets and cursor (or null); with
+// stats, adds the total to the list-entry statistic and flags an overflow
+// beyond `capacity`. Tiles flagged in tile_flag (or null) get offset
+// `capacity` and a count of at least 1, which sends them to the raster's
+// in-order scan of all pixel boxes; their flags are cleared.
+hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
+                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag);
+// Radix binning, after kz_bin_count wrote f.tri_rect and f.block_pairs.
+struct KzRadixScratch {
+    uint32_t *chunk_offsets;  // [ceil(tris / KZ_PAIR_CHUNK)]
+    uint32_t *ctrl;           // KZ_BIN_WORDS
+    uint32_t *tile_flag;      // [tiles], zero between batches
+    uint32_t *keys[2], *vals[2];  // [list capacity] each
+    uint32_t *table;          // [kz_radix_digits_max() * tstride]
+    uint32_t *totals;         // [kz_radix_digits_max()]
+    uint32_t tstride;         // kz_radix_blocks(list capacity)
+};
+uint32_t kz_radix_blocks(uint64_t capacity);
+uint32_t kz_radix_digits_max();
+// counting-sort passes for `ntiles` keys
+uint32_t kz_radix_passes(uint32_t ntiles);
+// Pair offsets per chunk and the pairs in primitive order (2 launches).
+hipError_t kz_launch_bin_pairs(hipStream_t st, const KzFrame &f, const KzRadixScratch &s);
+// The passes, the last into f.tile_lists with f.tile_counts (zeroed by the
+// caller); with f.tile_acc (zeroed by the caller), the pair statistics
+// (3 launches per pass).
+hipError_t kz_launch_bin_radix(hipStream_t st, const KzFrame &f, const KzRadixScratch &s, uint32_t ntiles);
 // Binning from the per-triangle pixel boxes: pairs per tile, then the lists.
 hipError_t kz_launch_bin_hist(hipStream_t st, const KzFrame &f);
 hipError_t kz_launch_bin_scatter(hipStream_t st, const KzFrame &f);

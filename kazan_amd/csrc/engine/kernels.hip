@@ -3,13 +3,17 @@
 //   kz_fill_u32 / kz_fill_rect_u32  attachment clears and counter resets (a
 //                    fill kernel rather than a memset, so command buffers can
 //                    be captured into hipGraphs)
-//   kz_bin_hist      (triangle, tile) pairs per tile
+//   kz_pair_scan, kz_pair_write, kz_radix_hist / scan / scatter  radix
+//                    binning (the default): (triangle, tile) pairs in
+//                    primitive order, sorted by tile with a stable LSD radix
+//                    sort whose last pass writes the lists, the tile counts
+//                    and the cost model's pair statistics
 //   kz_scan          exclusive prefix sum of per-list counts
-//   kz_bin_scatter   writes triangle ids into the per-tile lists
-//   kz_sort_short / kz_sort_lists  sort each list into primitive order:
-//                    up to 256 ids in one wave's registers; longer ones by a
-//                    workgroup, bitonic in LDS (16K ids), beyond that in
-//                    LDS-sized runs merged with merge-path in HBM
+//   kz_bin_hist, kz_bin_scatter, kz_sort_short / kz_sort_lists  the other
+//                    binning: pairs per tile, ids scattered into the lists
+//                    in no order, each list sorted into primitive order (up
+//                    to 256 ids in one wave's registers; longer ones bitonic
+//                    in LDS, beyond 16K ids in runs merged with merge path)
 //   kz_bin_hist_acc, kz_tile_features, kz_cost_mlp, kz_cost_train  the
 //                    learned cost model: statistics, features, bf16 MFMA
 //                    inference and training
@@ -129,17 +133,28 @@ __global__ void kz_fill_rect_u32(uint32_t *dst, uint32_t width, int32_t x0, int3
     }
 }
 
-// Exclusive prefix sum of counts[0, n) into offsets and cursor. Adds the
-// total to the list-entry statistic and, when `capacity` is exceeded,
-// counts an overflow.
-__global__ __launch_bounds__(kScanThreads) void kz_scan(const uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
-                                                        uint32_t n, uint32_t capacity, unsigned long long *stats) {
+// Exclusive prefix sum of counts[0, n) into offsets and cursor. With stats,
+// adds the total to the list-entry statistic and, when `capacity` is
+// exceeded, counts an overflow. Flagged tiles (tile_flag) are sent to the
+// in-order scan: offset `capacity`, count at least 1; flags are cleared.
+__global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
+                                                        uint32_t n, uint32_t capacity, unsigned long long *stats,
+                                                        uint32_t *tile_flag) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_carry;
     const uint32_t total = block_exclusive_scan(counts, offsets, cursor, n, s_wave, &s_carry);
     if (threadIdx.x == 0 && stats) {
         atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], (unsigned long long)total);
         if (total > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
+    }
+    if (tile_flag) {
+        // thread i % kScanThreads wrote offsets[i] above
+        for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
+            if (!tile_flag[i]) continue;
+            offsets[i] = capacity;
+            counts[i] = max(counts[i], 1u);
+            tile_flag[i] = 0;
+        }
     }
 }
 
@@ -539,6 +554,420 @@ __global__ __launch_bounds__(kSortThreads) void kz_sort_lists(const uint32_t *co
     }
 }
 
+// --- radix binning ------------------------------------------------------
+//
+// The batch's (triangle, tile) pairs are generated in primitive order
+// (triangle by triangle, each one's tiles row-major in its pixel box) and
+// sorted by tile with a stable least-significant-digit radix sort; each
+// tile's run of the result is its list, already in primitive order. A pass
+// costs the same whatever the skew: a tile holding a million pairs is as
+// cheap as a million tiles holding one.
+
+constexpr uint32_t kPairThreads = 1024;
+constexpr uint32_t kPairPer = KZ_PAIR_CHUNK / kPairThreads;  // triangles per thread
+constexpr uint32_t kBlocksPerChunk = KZ_PAIR_CHUNK / KZ_BIN_BLOCK;
+constexpr uint32_t kRadixThreads = 512;
+constexpr uint32_t kRadixWaves = kRadixThreads / 64;
+constexpr uint32_t kRadixRounds = 8;                            // pairs per lane
+constexpr uint32_t kRadixBlock = kRadixThreads * kRadixRounds;  // pairs per workgroup
+constexpr uint32_t kRadixMaxBits = 9;
+constexpr uint32_t kRadixMaxDigits = 1u << kRadixMaxBits;
+static_assert(kRadixMaxDigits <= kRadixThreads, "digit scans take one digit per thread");
+
+__device__ __forceinline__ uint32_t box_tiles(uint64_t r, uint32_t sh) {
+    if (r == KZ_RECT_EMPTY) return 0;
+    uint32_t x0, y0, x1, y1;
+    unpack_box(r, x0, y0, x1, y1);
+    return ((x1 >> sh) - (x0 >> sh) + 1) * ((y1 >> sh) - (y0 >> sh) + 1);
+}
+
+// Pair offsets of the KZ_PAIR_CHUNK-triangle chunks (one workgroup; chunk c
+// sums blocks [c * kBlocksPerChunk, ...) of kz_bin_count). When the pairs
+// exceed `capacity`, the chunks from the first one that does not fit on are
+// cut: ctrl[KZ_BIN_CUT] is that chunk and ctrl[KZ_BIN_PAIRS] the pairs
+// before it (kz_pair_write then flags the tiles of the cut triangles).
+__global__ __launch_bounds__(kScanThreads) void kz_pair_scan(const uint32_t *block_pairs, uint32_t nblocks,
+                                                             uint32_t *chunk_offsets, uint32_t nchunks,
+                                                             uint32_t capacity, uint32_t *ctrl,
+                                                             unsigned long long *stats) {
+    __shared__ unsigned long long s_wave[kScanThreads / 64];
+    __shared__ unsigned long long s_carry, s_cut;  // s_cut: first cut chunk << 32 | its offset
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        s_carry = 0;
+        s_cut = ~0ull;
+    }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < nchunks; c0 += kScanThreads) {
+        const uint32_t c = c0 + tid;
+        unsigned long long v = 0;
+        if (c < nchunks) {
+            const uint32_t b1 = min(c * kBlocksPerChunk + kBlocksPerChunk, nblocks);
+            for (uint32_t b = c * kBlocksPerChunk; b < b1; ++b) v += block_pairs[b];
+        }
+        unsigned long long x = v;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const unsigned long long o = __shfl_up(x, d, 64);
+            if (lane >= d) x += o;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        unsigned long long wbase = 0, total = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            const unsigned long long t = s_wave[w];
+            if (w < wave) wbase += t;
+            total += t;
+        }
+        const unsigned long long carry = s_carry;
+        if (c < nchunks) {
+            const unsigned long long excl = carry + wbase + x - v;
+            chunk_offsets[c] = (uint32_t)min(excl, (unsigned long long)capacity);
+            if (excl + v > capacity) atomicMin(&s_cut, ((unsigned long long)c << 32) | (uint32_t)min(excl, (unsigned long long)capacity));
+        }
+        __syncthreads();
+        if (tid == 0) s_carry = carry + total;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const unsigned long long pairs = s_carry, cut = s_cut;
+        const bool over = cut != ~0ull;
+        ctrl[KZ_BIN_CUT] = over ? (uint32_t)(cut >> 32) : nchunks;
+        ctrl[KZ_BIN_PAIRS] = over ? (uint32_t)cut : (uint32_t)pairs;
+        if (stats) {
+            atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], pairs);
+            if (over) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
+        }
+    }
+}
+
+// Writes the pairs of chunk c = blockIdx.x (triangles [c * KZ_PAIR_CHUNK,
+// ...)) from its offset on: the chunk's q-th pair is the k-th tile of the
+// triangle whose prefix range holds q. A cut chunk instead flags every tile
+// its triangles' boxes touch; kz_scan sends those tiles to the in-order
+// scan of all pixel boxes.
+__global__ __launch_bounds__(kPairThreads) void kz_pair_write(const uint64_t *rect, uint32_t ntris, uint32_t sh,
+                                                              uint32_t tiles_x, const uint32_t *chunk_offsets,
+                                                              const uint32_t *ctrl, uint32_t *keys, uint32_t *vals,
+                                                              uint32_t *tile_flag) {
+    __shared__ uint64_t s_rect[KZ_PAIR_CHUNK];
+    __shared__ uint32_t s_pfx[KZ_PAIR_CHUNK];
+    __shared__ uint32_t s_wave[kPairThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t t0 = blockIdx.x * KZ_PAIR_CHUNK;
+    if (t0 >= ntris) return;
+    const uint32_t n = min(KZ_PAIR_CHUNK, ntris - t0);
+    if (blockIdx.x >= ctrl[KZ_BIN_CUT]) {
+        for (uint32_t i = tid; i < n; i += kPairThreads) {
+            const uint64_t r = rect[t0 + i];
+            if (r == KZ_RECT_EMPTY) continue;
+            uint32_t x0, y0, x1, y1;
+            unpack_box(r, x0, y0, x1, y1);
+            for (uint32_t ty = y0 >> sh; ty <= (y1 >> sh); ++ty)
+                for (uint32_t tx = x0 >> sh; tx <= (x1 >> sh); ++tx) tile_flag[ty * tiles_x + tx] = 1u;
+        }
+        return;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPairPer; ++j) {
+        const uint32_t i = j * kPairThreads + tid;
+        s_rect[i] = i < n ? rect[t0 + i] : KZ_RECT_EMPTY;
+    }
+    __syncthreads();
+    // prefix of the pair counts, kPairPer consecutive triangles per thread
+    uint32_t cnt[kPairPer], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPairPer; ++j) {
+        cnt[j] = box_tiles(s_rect[tid * kPairPer + j], sh);
+        sum += cnt[j];
+    }
+    uint32_t x = sum;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(x, d, 64);
+        if (lane >= d) x += o;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t run = x - sum, total = 0;
+    for (uint32_t w = 0; w < kPairThreads / 64; ++w) {
+        const uint32_t t = s_wave[w];
+        if (w < wave) run += t;
+        total += t;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPairPer; ++j) {
+        s_pfx[tid * kPairPer + j] = run;
+        run += cnt[j];
+    }
+    __syncthreads();
+    const uint32_t out = chunk_offsets[blockIdx.x];
+    for (uint32_t q = tid; q < total; q += kPairThreads) {
+        // the last triangle whose prefix is <= q (s_pfx[0] = 0 <= q)
+        uint32_t lo = 0, hi = KZ_PAIR_CHUNK;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pfx[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t k = q - s_pfx[lo];
+        uint32_t x0, y0, x1, y1;
+        unpack_box(s_rect[lo], x0, y0, x1, y1);
+        const uint32_t tx0 = x0 >> sh, tw = (x1 >> sh) - tx0 + 1;
+        const uint32_t ky = k / tw, kx = k - ky * tw;
+        keys[out + q] = ((y0 >> sh) + ky) * tiles_x + tx0 + kx;
+        vals[out + q] = t0 + lo;
+    }
+}
+
+// The lanes of `valid` whose digit (of `bits` bits) equals this lane's.
+__device__ __forceinline__ uint64_t digit_match(uint32_t d, uint32_t bits, uint64_t valid) {
+    uint64_t m = valid;
+    for (uint32_t i = 0; i < bits; ++i) {
+        const bool bit = (d >> i) & 1u;
+        const uint64_t b = __ballot(bit);
+        m &= bit ? b : ~b;
+    }
+    return m;
+}
+
+// Histogram of the digit (key >> shift) & (2^bits - 1) per block of
+// kRadixBlock pairs: table[digit * tstride + block]. A wave counts one
+// digit group per ballot match, so a skewed digit costs no LDS contention.
+__global__ __launch_bounds__(kRadixThreads) void kz_radix_hist(const uint32_t *keys, const uint32_t *ctrl,
+                                                               uint32_t shift, uint32_t bits, uint32_t *table,
+                                                               uint32_t tstride) {
+    __shared__ uint32_t s_cnt[kRadixMaxDigits];
+    const uint32_t npairs = ctrl[KZ_BIN_PAIRS];
+    const uint32_t b0 = blockIdx.x * kRadixBlock;
+    if (b0 >= npairs) return;
+    const uint32_t nd = 1u << bits, tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    for (uint32_t i = tid; i < nd; i += kRadixThreads) s_cnt[i] = 0;
+    __syncthreads();
+    const uint32_t base = b0 + wave * (kRadixBlock / kRadixWaves);
+    uint32_t dg[kRadixRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const uint32_t i = base + r * 64 + lane;
+        dg[r] = i < npairs ? (keys[i] >> shift) & (nd - 1) : kNone;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const bool ok = dg[r] != kNone;
+        const uint64_t m = digit_match(dg[r], bits, __ballot(ok));
+        if (ok && rank_in(m) == 0) atomicAdd(&s_cnt[dg[r]], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nd; i += kRadixThreads) table[(size_t)i * tstride + blockIdx.x] = s_cnt[i];
+}
+
+// Exclusive scan of each digit's row over the blocks in use (one workgroup
+// per digit); totals[digit] = the digit's pairs.
+__global__ __launch_bounds__(kScanThreads) void kz_radix_scan(uint32_t *table, uint32_t tstride,
+                                                              const uint32_t *ctrl, uint32_t *totals) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_carry;
+    const uint32_t nb = (ctrl[KZ_BIN_PAIRS] + kRadixBlock - 1) / kRadixBlock;
+    uint32_t *row = table + (size_t)blockIdx.x * tstride;
+    const uint32_t total = block_exclusive_scan(row, row, nullptr, nb, s_wave, &s_carry);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// Exclusive scans of a[0, n) and b[0, n) in place, n <= kRadixThreads.
+__device__ void radix_scan2(uint32_t *a, uint32_t *b, uint32_t n, uint32_t *s_wa, uint32_t *s_wb) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t va = tid < n ? a[tid] : 0u, vb = tid < n ? b[tid] : 0u;
+    uint32_t xa = va, xb = vb;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t oa = __shfl_up(xa, d, 64), ob = __shfl_up(xb, d, 64);
+        if (lane >= d) {
+            xa += oa;
+            xb += ob;
+        }
+    }
+    if (lane == 63) {
+        s_wa[wave] = xa;
+        s_wb[wave] = xb;
+    }
+    __syncthreads();
+    uint32_t ba = 0, bb = 0;
+    for (uint32_t w = 0; w < wave; ++w) {
+        ba += s_wa[w];
+        bb += s_wb[w];
+    }
+    if (tid < n) {
+        a[tid] = ba + xa - va;
+        b[tid] = bb + xb - vb;
+    }
+    __syncthreads();
+}
+
+// Segmented sum / max over runs of adjacent lanes (same[s]: lane + 2^s is
+// in this lane's run): the run's first lane ends with the run's total.
+__device__ __forceinline__ uint32_t seg_sum(uint32_t v, const bool *same) {
+    for (uint32_t s = 0, d = 1; d < 64; ++s, d <<= 1) {
+        const uint32_t o = __shfl_down(v, d, 64);
+        if (same[s]) v += o;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t seg_max(uint32_t v, const bool *same) {
+    for (uint32_t s = 0, d = 1; d < 64; ++s, d <<= 1) {
+        const uint32_t o = __shfl_down(v, d, 64);
+        if (same[s]) v = max(v, o);
+    }
+    return v;
+}
+
+// The cost model's per-tile pair statistics, gathered in the last pass.
+struct RadixAcc {
+    const uint64_t *rect;
+    const uint32_t *tri_z;
+    uint32_t *acc;  // KZ_ACC words per tile, or null
+    uint32_t sh, tiles_x;
+};
+
+// One stable counting-sort pass by the digit (key >> shift) & (2^bits - 1)
+// over blocks of kRadixBlock pairs. A wave ranks its 8 rounds of 64 pairs,
+// in pair order, by digit-match ballots against a count per digit of its
+// own, so equal digits keep their order within the wave; waves are ordered
+// by a scan of those counts, blocks by the scanned table, digits by the
+// scanned totals. The block is regrouped by digit in LDS so that each
+// digit's run is written contiguously. The last pass (lists != null)
+// writes only the triangle ids, into the tile lists, and adds each tile's
+// run lengths to tile_counts (and, with acc, the pair statistics).
+__global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t *keys_in, const uint32_t *vals_in,
+                                                                  uint32_t *keys_out, uint32_t *vals_out,
+                                                                  uint32_t *lists, uint32_t *tile_counts,
+                                                                  const uint32_t *ctrl, const uint32_t *table,
+                                                                  uint32_t tstride, const uint32_t *totals,
+                                                                  uint32_t shift, uint32_t bits, RadixAcc acc) {
+    __shared__ uint32_t s_wcnt[kRadixWaves * kRadixMaxDigits];
+    __shared__ uint32_t s_lstart[kRadixMaxDigits], s_gbase[kRadixMaxDigits];
+    __shared__ uint32_t s_key[kRadixBlock], s_val[kRadixBlock];
+    __shared__ uint32_t s_wa[kRadixWaves], s_wb[kRadixWaves];
+    const uint32_t npairs = ctrl[KZ_BIN_PAIRS];
+    const uint32_t b0 = blockIdx.x * kRadixBlock;
+    if (b0 >= npairs) return;
+    const uint32_t nitems = min(kRadixBlock, npairs - b0);
+    const uint32_t nd = 1u << bits, dmask = nd - 1, tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    for (uint32_t i = tid; i < kRadixWaves * nd; i += kRadixThreads) s_wcnt[i] = 0;
+    __syncthreads();
+
+    // ranks within the wave
+    const uint32_t base = b0 + wave * (kRadixBlock / kRadixWaves);
+    uint32_t key[kRadixRounds], val[kRadixRounds], rk[kRadixRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const uint32_t i = base + r * 64 + lane;
+        const bool ok = i < npairs;
+        key[r] = ok ? keys_in[i] : kNone;
+        val[r] = ok ? vals_in[i] : 0u;
+    }
+    uint32_t *wcnt = s_wcnt + wave * nd;
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const bool ok = key[r] != kNone;
+        const uint32_t d = (key[r] >> shift) & dmask;
+        const uint64_t m = digit_match(d, bits, __ballot(ok));
+        const uint32_t lead = ok ? (uint32_t)__builtin_ctzll(m) : lane;
+        uint32_t old = 0;
+        if (ok && lead == lane) {
+            old = wcnt[d];
+            wcnt[d] = old + (uint32_t)__popcll(m);
+        }
+        old = (uint32_t)__shfl((int)old, (int)lead, 64);
+        rk[r] = old + rank_in(m);
+    }
+    __syncthreads();
+
+    // waves' starts per digit, the block's digit starts, the global starts
+    for (uint32_t d = tid; d < nd; d += kRadixThreads) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kRadixWaves; ++w) {
+            const uint32_t c = s_wcnt[w * nd + d];
+            s_wcnt[w * nd + d] = s;
+            s += c;
+        }
+        s_lstart[d] = s;
+        s_gbase[d] = totals[d];
+    }
+    __syncthreads();
+    radix_scan2(s_lstart, s_gbase, nd, s_wa, s_wb);
+    for (uint32_t d = tid; d < nd; d += kRadixThreads) s_gbase[d] += table[(size_t)d * tstride + blockIdx.x];
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        if (key[r] == kNone) continue;
+        const uint32_t d = (key[r] >> shift) & dmask;
+        const uint32_t p = s_lstart[d] + wcnt[d] + rk[r];  // < nitems by construction
+        if (p < nitems) {
+            s_key[p] = key[r];
+            s_val[p] = val[r];
+        }
+    }
+    __syncthreads();
+
+    // out, digit run by digit run
+    for (uint32_t j0 = 0; j0 < nitems; j0 += kRadixThreads) {
+        const uint32_t j = j0 + tid;
+        const bool ok = j < nitems;
+        const uint32_t k = ok ? s_key[j] : kNone, v = ok ? s_val[j] : 0u;
+        if (ok) {
+            const uint32_t d = (k >> shift) & dmask;
+            const uint32_t g = s_gbase[d] + (j - s_lstart[d]);  // < npairs by construction
+            if (g < npairs) {
+                if (lists) {
+                    lists[g] = v;
+                } else {
+                    keys_out[g] = k;
+                    vals_out[g] = v;
+                }
+            }
+        }
+        if (!lists) continue;
+        // the last pass's block is sorted by tile: one update per run of a tile
+        const Run run = run_of(k, lane);
+        if (run.leader) atomicAdd(&tile_counts[k], run.len);
+        if (!acc.acc) continue;
+        uint32_t area = 0, lo = 0, hi = 0, zinv = 0, zmax = 0;
+        if (ok) {
+            uint32_t x0, y0, x1, y1;
+            const uint64_t r = acc.rect[v];
+            unpack_box(r, x0, y0, x1, y1);
+            const uint32_t X0 = (k % acc.tiles_x) << acc.sh, Y0 = (k / acc.tiles_x) << acc.sh;
+            const uint32_t X1 = X0 + (1u << acc.sh) - 1, Y1 = Y0 + (1u << acc.sh) - 1;
+            area = (min(x1, X1) - max(x0, X0) + 1) * (min(y1, Y1) - max(y0, Y0) + 1);
+            const uint32_t bin = area <= 2 ? 0u : area <= 8 ? 1u : area <= 32 ? 2u : area <= 128 ? 3u : 4u;
+            const uint32_t shared = box_tiles(r, acc.sh) > 1 ? 1u : 0u;
+            // six counters of at most 64 per wave, 10 bits each
+            lo = bin < 3 ? 1u << (10 * bin) : 0u;
+            hi = (bin >= 3 ? 1u << (10 * (bin - 3)) : 0u) + (shared << 20);
+            const uint32_t z = acc.tri_z[v];
+            zinv = 65535u - (z & 0xFFFFu);
+            zmax = z >> 16;
+        }
+        bool same[6];
+        for (uint32_t s = 0, d = 1; d < 64; ++s, d <<= 1) {
+            // every lane takes part in the shuffle (a lane left out reads as 0)
+            const uint32_t h = (uint32_t)__shfl_down((int)run.head, d, 64);
+            same[s] = lane + d < 64 && h == run.head;
+        }
+        area = seg_sum(area, same);
+        lo = seg_sum(lo, same);
+        hi = seg_sum(hi, same);
+        zinv = seg_max(zinv, same);
+        zmax = seg_max(zmax, same);
+        if (run.leader) {
+            uint32_t *c = acc.acc + (size_t)k * KZ_ACC;
+            atomicAdd(&c[KZ_ACC_AREA], area);
+            const uint32_t h[6] = {lo & 1023u, (lo >> 10) & 1023u, lo >> 20, hi & 1023u, (hi >> 10) & 1023u, hi >> 20};
+            for (uint32_t b = 0; b < 5; ++b)
+                if (h[b]) atomicAdd(&c[KZ_ACC_HIST + b], h[b]);
+            if (h[5]) atomicAdd(&c[KZ_ACC_SHARED], h[5]);
+            atomicMax(&c[KZ_ACC_ZMIN], zinv);
+            atomicMax(&c[KZ_ACC_ZMAX], zmax);
+        }
+    }
+}
+
 // --- load-balanced mode -------------------------------------------------
 
 // A tile's cost in list entries: the binned triangle count, plus the
@@ -604,6 +1033,7 @@ __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntil
             f.split_slot[t] = slot;
             f.split_info[2 * slot] = atomicAdd(&s_chunks, (n + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
             f.split_info[2 * slot + 1] = t;
+            for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
         }
     }
     __syncthreads();
@@ -639,13 +1069,17 @@ __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntil
 struct Chunk {
     const uint32_t *ids;
     uint32_t n, nsub, level, ssh;
+    uint32_t slot, c0, j;  // split slot, its first chunk, this chunk's index in the tile
     int32_t X0, Y0;
 };
 
 __device__ __forceinline__ Chunk chunk_at(const KzFrame &f, uint32_t c) {
     Chunk k;
     const uint32_t t = f.chunk_tile[c];
-    const uint32_t j = c - f.split_info[2 * f.split_slot[t]];
+    k.slot = f.split_slot[t];
+    k.c0 = f.split_info[2 * k.slot];
+    const uint32_t j = c - k.c0;
+    k.j = j;
     const uint32_t n = f.tile_counts[t];
     k.ids = f.tile_lists + f.tile_offsets[t] + j * KZ_SPLIT_CHUNK;
     k.n = min(KZ_SPLIT_CHUNK, n - j * KZ_SPLIT_CHUNK);
@@ -670,7 +1104,8 @@ __device__ __forceinline__ uint32_t sub_mask(uint64_t r, const Chunk &k) {
     return m;
 }
 
-// Entries per sub-tile of every chunk (one wave per chunk).
+// Entries per sub-tile of every chunk (one wave per chunk), kept per chunk
+// and added to the sub-tile totals of its split slot (zeroed by kz_plan).
 __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     const uint32_t lane = lane_id();
     const uint32_t nch = f.ctrl[KZ_CTRL_CHUNKS];
@@ -686,43 +1121,57 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
                 if (lane == s) cnt += c_s;
             }
         }
-        if (lane < 16) f.chunk_counts[c * 16 + lane] = cnt;
+        if (lane < 16) {
+            f.chunk_counts[c * 16 + lane] = cnt;
+            if (cnt) atomicAdd(&f.split_counts[k.slot * 16 + lane], cnt);
+        }
     }
 }
 
-// One workgroup: sub-tile list sizes and offsets, then each chunk's write
-// base per sub-tile (in place of its counts).
+// One workgroup: the offsets of the sub-tile lists, slot after slot (each
+// slot's 16 lists consecutive), from the totals of kz_split_count.
 __global__ __launch_bounds__(kScanThreads) void kz_split_offsets(KzFrame f) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_carry;
-    const uint32_t n_items = f.ctrl[KZ_CTRL_SPLITS] * 16;
-    for (uint32_t i = threadIdx.x; i < n_items; i += kScanThreads) {
-        const uint32_t slot = i >> 4, s = i & 15;
-        const uint32_t c0 = f.split_info[2 * slot], t = f.split_info[2 * slot + 1];
-        const uint32_t nc = (f.tile_counts[t] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
-        uint32_t total = 0;
-        if (s < (1u << (2 * f.tile_level[t])))
-            for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) total += f.chunk_counts[(c0 + j) * 16 + s];
-        f.split_counts[i] = total;
-    }
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t n_slots = f.ctrl[KZ_CTRL_SPLITS];
+    if (tid == 0) s_carry = 0;
     __syncthreads();
-    const uint32_t total = block_exclusive_scan(f.split_counts, f.split_offsets, nullptr, n_items, s_wave, &s_carry);
-    if (threadIdx.x == 0) {
-        atomicAdd(&f.stats[KZ_STAT_SPLIT_ENTRIES], (unsigned long long)total);
-        atomicMax(&f.ctrl[KZ_CTRL_SPLIT_MAX], total);
-    }
-    for (uint32_t i = threadIdx.x; i < n_items; i += kScanThreads) {
-        const uint32_t slot = i >> 4, s = i & 15;
-        const uint32_t c0 = f.split_info[2 * slot], t = f.split_info[2 * slot + 1];
-        const uint32_t nc = (f.tile_counts[t] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
-        if (s >= (1u << (2 * f.tile_level[t]))) continue;
-        uint32_t run = f.split_offsets[i];
-        for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) {
-            uint32_t *p = &f.chunk_counts[(c0 + j) * 16 + s];
-            const uint32_t v = *p;
-            *p = run;
-            run += v;
+    for (uint32_t i0 = 0; i0 < n_slots; i0 += kScanThreads) {
+        const uint32_t i = i0 + tid;
+        uint32_t c[16], v = 0;
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s) {
+            c[s] = i < n_slots ? f.split_counts[i * 16 + s] : 0u;
+            v += c[s];
         }
+        uint32_t x = v;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= d) x += o;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        uint32_t run = s_carry + x - v, total = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            const uint32_t t = s_wave[w];
+            if (w < wave) run += t;
+            total += t;
+        }
+        if (i < n_slots) {
+#pragma unroll
+            for (uint32_t s = 0; s < 16; ++s) {
+                f.split_offsets[i * 16 + s] = run;
+                run += c[s];
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        atomicAdd(&f.stats[KZ_STAT_SPLIT_ENTRIES], (unsigned long long)s_carry);
+        atomicMax(&f.ctrl[KZ_CTRL_SPLIT_MAX], s_carry);
     }
 }
 
@@ -734,7 +1183,16 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
     const uint32_t waves = gridDim.x * (kSplitThreads / 64);
     for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
         const Chunk k = chunk_at(f, c);
-        uint32_t base = lane < 16 ? f.chunk_counts[c * 16 + lane] : 0u;  // lane s: next slot of sub-tile s
+        // lane s < 16: next slot of sub-tile s, after the entries of the
+        // tile's earlier chunks (lanes 16 s'..16 s' + 15 sum every 4th chunk)
+        uint32_t before = 0;
+        for (uint32_t j0 = 0; j0 < k.j; j0 += 4) {
+            const uint32_t j = j0 + (lane >> 4);
+            if (j < k.j) before += f.chunk_counts[(k.c0 + j) * 16 + (lane & 15)];
+        }
+        before += __shfl_xor(before, 16, 64);
+        before += __shfl_xor(before, 32, 64);
+        uint32_t base = lane < 16 ? f.split_offsets[k.slot * 16 + lane] + before : 0u;
         for (uint32_t e0 = 0; e0 < k.n; e0 += 64) {
             const uint32_t e = e0 + lane;
             uint32_t id = 0, m = 0;
@@ -1229,9 +1687,53 @@ hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width
     return hipGetLastError();
 }
 
-hipError_t kz_launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats) {
-    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats);
+hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
+                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag) {
+    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats, tile_flag);
+    return hipGetLastError();
+}
+
+uint32_t kz_radix_blocks(uint64_t capacity) {
+    return (uint32_t)std::max<uint64_t>((capacity + kRadixBlock - 1) / kRadixBlock, 1);
+}
+
+uint32_t kz_radix_digits_max() { return kRadixMaxDigits; }
+
+uint32_t kz_radix_passes(uint32_t ntiles) {
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < ntiles) ++bits;
+    return (bits + kRadixMaxBits - 1) / kRadixMaxBits;
+}
+
+hipError_t kz_launch_bin_pairs(hipStream_t st, const KzFrame &f, const KzRadixScratch &s) {
+    if (f.total_tris == 0) return hipSuccess;
+    const uint32_t nblocks = (f.total_tris + KZ_BIN_BLOCK - 1) / KZ_BIN_BLOCK;
+    const uint32_t nchunks = (f.total_tris + KZ_PAIR_CHUNK - 1) / KZ_PAIR_CHUNK;
+    kz_pair_scan<<<1, kScanThreads, 0, st>>>(f.block_pairs, nblocks, s.chunk_offsets, nchunks, f.list_capacity,
+                                             s.ctrl, f.stats);
+    kz_pair_write<<<nchunks, kPairThreads, 0, st>>>(f.tri_rect, f.total_tris, f.tile_shift, f.tiles_x,
+                                                    s.chunk_offsets, s.ctrl, s.keys[0], s.vals[0], s.tile_flag);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_bin_radix(hipStream_t st, const KzFrame &f, const KzRadixScratch &s, uint32_t ntiles) {
+    if (f.total_tris == 0) return hipSuccess;
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < ntiles) ++bits;
+    const uint32_t passes = kz_radix_passes(ntiles);
+    const uint32_t d = (bits + passes - 1) / passes;
+    RadixAcc acc{f.tri_rect, f.tri_z, f.tile_acc, f.tile_shift, f.tiles_x};
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t in = p & 1u, out = in ^ 1u;
+        const uint32_t shift = p * d, pb = std::min(d, bits - shift);
+        const bool last = p + 1 == passes;
+        kz_radix_hist<<<s.tstride, kRadixThreads, 0, st>>>(s.keys[in], s.ctrl, shift, pb, s.table, s.tstride);
+        kz_radix_scan<<<1u << pb, kScanThreads, 0, st>>>(s.table, s.tstride, s.ctrl, s.totals);
+        kz_radix_scatter<<<s.tstride, kRadixThreads, 0, st>>>(
+            s.keys[in], s.vals[in], last ? nullptr : s.keys[out], last ? nullptr : s.vals[out],
+            last ? f.tile_lists : nullptr, last ? f.tile_counts : nullptr, s.ctrl, s.table, s.tstride, s.totals,
+            shift, pb, last ? acc : RadixAcc{nullptr, nullptr, nullptr, 0, 0});
+    }
     return hipGetLastError();
 }
 

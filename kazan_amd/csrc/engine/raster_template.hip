@@ -36,6 +36,9 @@
 #define KZ_TP (KZ_TS * KZ_TS)
 #define KZ_FRAG_CAP (KZ_TP * 4)
 #define KZ_SMALL_CAND 16
+// binning tests coverage of triangles whose pixel box has at most this many
+// pixels (and bins them by the box of the pixels they cover)
+#define KZ_TIGHT_CAND 16
 #define KZ_NO_CLAIM 0xFFFFFFFFu
 
 #if KZ_NV > 0
@@ -384,10 +387,10 @@ __device__ __forceinline__ void kz_load_rec(const KzFrame &f, uint32_t g, KzTri 
     }
 }
 
-extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
+extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = kz_lane();
-    uint32_t n_setup = 0, n_unsup = 0;
+    uint32_t n_setup = 0, n_unsup = 0, n_pairs = 0;
     if (g < f.total_tris) {
         float p[3][4], vary[3][KZ_NVA];
         kz_fetch_triangle(f, g, p, vary);
@@ -396,6 +399,25 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
         uint64_t rect = KZ_RECT_EMPTY;
         if (st == 1) {
             n_setup = 1;
+            // A small triangle's box shrinks to the pixels it covers, and a
+            // triangle that covers no sample is not binned at all: pixels
+            // cannot depend on it (no fragment, no depth or colour write).
+            if (t.px0 <= t.px1 && t.py0 <= t.py1 && (t.px1 - t.px0 + 1) * (t.py1 - t.py0 + 1) <= KZ_TIGHT_CAND) {
+                const KzCov c = kz_cov_setup(t);
+                int32_t cx0 = t.px1 + 1, cy0 = t.py1 + 1, cx1 = t.px0 - 1, cy1 = t.py0 - 1;
+                for (int32_t py = t.py0; py <= t.py1; ++py)
+                    for (int32_t px = t.px0; px <= t.px1; ++px)
+                        if (kz_cover_mask(t, c, px, py)) {
+                            cx0 = min(cx0, px);
+                            cx1 = max(cx1, px);
+                            cy0 = min(cy0, py);
+                            cy1 = max(cy1, py);
+                        }
+                t.px0 = cx0;
+                t.px1 = cx1;
+                t.py0 = cy0;
+                t.py1 = cy1;
+            }
             if (t.px0 <= t.px1 && t.py0 <= t.py1) {
                 rect = (uint64_t)(uint32_t)t.px0 | ((uint64_t)(uint32_t)t.py0 << 16) |
                        ((uint64_t)(uint32_t)t.px1 << 32) | ((uint64_t)(uint32_t)t.py1 << 48);
@@ -411,6 +433,10 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
             n_unsup = 1;
         }
         f.tri_rect[g] = rect;
+        // (triangle, tile) pairs: the tiles of the pixel box
+        if (rect != KZ_RECT_EMPTY)
+            n_pairs = (uint32_t)(((t.px1 >> KZ_TILE_SHIFT) - (t.px0 >> KZ_TILE_SHIFT) + 1) *
+                                 ((t.py1 >> KZ_TILE_SHIFT) - (t.py0 >> KZ_TILE_SHIFT) + 1));
     }
     const uint64_t ms = kz_ballot(n_setup != 0), mu = kz_ballot(n_unsup != 0), mi = kz_ballot(g < f.total_tris);
     if (lane == 0) {
@@ -418,6 +444,17 @@ extern "C" __global__ __launch_bounds__(256) void kz_bin_count(KzFrame f) {
         if (mi) atomicAdd(&st[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
         if (ms) atomicAdd(&st[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
         if (mu) atomicAdd(&st[KZ_STAT_TRIANGLES_UNSUPPORTED], (unsigned long long)__popcll(mu));
+    }
+    if (f.block_pairs) {  // the block's pair total, for the pair offsets of radix binning
+        __shared__ uint32_t s_pairs[KZ_BIN_BLOCK / 64];
+        for (uint32_t d = 32; d > 0; d >>= 1) n_pairs += (uint32_t)__shfl_xor((int)n_pairs, (int)d, 64);
+        if (lane == 0) s_pairs[threadIdx.x >> 6] = n_pairs;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+            for (uint32_t w = 0; w < KZ_BIN_BLOCK / 64; ++w) s += s_pairs[w];
+            f.block_pairs[blockIdx.x] = s;
+        }
     }
 }
 

@@ -189,6 +189,22 @@ def test_tile_profile_counts(driver, monkeypatch, oracle):
     assert (feat[:, 8] >= 0).all() and (feat[:, 8] <= 1).all()
 
 
+def test_features_do_not_depend_on_the_binning(driver, monkeypatch):
+    # the pair statistics gathered by the radix sort's last pass equal those
+    # of the per-tile count kernel of the other binning, tile for tile
+    from kazan_amd.scenes.renderer import SceneRenderer
+    s = g.gaussian_cluster(100_000, 320, 240)
+    feats = []
+    for binning in ("radix", "sort"):
+        monkeypatch.setenv("KAZAN_BINNING", binning)
+        ctx = _ctx(driver, monkeypatch, "plain")
+        ctx.d.vkSetTileProfileKAZAN(ctx.device, 1)
+        SceneRenderer(ctx, s).render(1)
+        feats.append(_profile(ctx)[:2])
+    np.testing.assert_array_equal(feats[0][0], feats[1][0])
+    np.testing.assert_array_equal(feats[0][1], feats[1][1])
+
+
 @pytest.mark.parametrize("b4", [20.0, 200.0, -20.0])
 def test_balanced_pixels_do_not_depend_on_the_model(driver, monkeypatch, oracle, b4):
     # b4 = 20: every tile predicted at a million fragments, so every tile is

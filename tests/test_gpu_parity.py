@@ -248,6 +248,19 @@ def test_phong_msaa4_stress_small(mctx, oracle):
 
 
 @pytest.mark.parametrize("form", ["balanced", "plain"])
+def test_sort_binning(driver, monkeypatch, oracle, form):
+    # the other binning (KAZAN_BINNING=sort): per-tile counts, an unordered
+    # scatter, then every list sorted into primitive order
+    from kazan_amd.scenes.renderer import Context
+    monkeypatch.setenv("KAZAN_PIPELINE", form)
+    monkeypatch.setenv("KAZAN_BINNING", "sort")
+    ctx = Context(driver)
+    for s in (g.jittered_grid(32, 64, 160, 120, 64), g.gaussian_cluster(300_000, 256, 256, sigma_frac=0.02),
+              g.zipf_draws(64, 20_000, 192, 128)):
+        check_parity(ctx, oracle, s)
+
+
+@pytest.mark.parametrize("form", ["balanced", "plain"])
 def test_setup_records(driver, monkeypatch, oracle, form):
     # the opt-in path where binning stores each triangle's setup and the
     # raster loads it instead of running the vertex shader and setup again
