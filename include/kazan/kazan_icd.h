@@ -148,17 +148,21 @@ typedef VkResult(VKAPI_PTR *PFN_vkFillClusterSceneKAZAN)(VkDevice device, VkBuff
                                                          const KazanClusterScene *pScene);
 
 /* Batched launch. In the balanced form a command buffer's batches are
- * autotuned over their first submissions (tile size 8..64 and persistent
- * raster grid, one timed candidate per submission; KAZAN_AUTOTUNE=0 turns
- * this off), after which the whole command buffer is captured into a
- * hipGraph on its next submission and replayed as one graph launch
- * (KAZAN_GRAPHS=0 turns this off; re-recording, or any reallocation of the
- * device's scratch buffers, makes it capture again).
+ * autotuned over their first submissions (one timed candidate per
+ * submission: tile size 8..64, then the persistent raster's grid, then the
+ * plan without the cost model and the plain one-workgroup-per-tile raster;
+ * KAZAN_AUTOTUNE=0 turns this off), after which the whole command buffer is
+ * captured into a hipGraph on its next submission and replayed as one graph
+ * launch (KAZAN_GRAPHS=0 turns this off; re-recording, or any reallocation
+ * of the device's scratch buffers, makes it capture again).
  * vkGetGraphStatsKAZAN: graph captures and graph launches so far.
- * vkGetTuningKAZAN: the tile size (8..64, 0 before any tuning finished) and
- * persistent-grid divisor the most recently finished tuning chose. */
+ * vkGetTuningKAZAN: what the most recently finished tuning chose: tile size
+ * (8..64, 0 before any tuning finished), persistent-grid divisor, whether
+ * the plan uses the cost model, whether the raster is the persistent one
+ * (any pointer may be null). */
 typedef void(VKAPI_PTR *PFN_vkGetGraphStatsKAZAN)(VkDevice device, uint32_t *pCaptures, uint32_t *pLaunches);
-typedef void(VKAPI_PTR *PFN_vkGetTuningKAZAN)(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor);
+typedef void(VKAPI_PTR *PFN_vkGetTuningKAZAN)(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor,
+                                              uint32_t *pCostModel, uint32_t *pPersistentRaster);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,7 @@ struct KzFrame {
     uint64_t *tri_rect;      // [total_tris] packed pixel bounding box or ~0
     uint32_t *tri_rec;       // [total_tris * KZ_REC_WORDS_FOR(varyings)] setup records, or null
     uint32_t *block_pairs;   // [ceil(total_tris / 256)] (triangle, tile) pairs per binning block, or null
+    uint32_t *tri_draw;      // [total_tris] draw of each triangle (batches of several draws), or null
     unsigned long long *stats;  // KZ_STAT_* counters
 
     // Load-balanced mode. A work item is a square sub-tile of a screen

@@ -47,7 +47,7 @@ Engine::~Engine() {
     hipFree(sort_tmp_);
     hipFree(tri_rect_);
     hipFree(tri_rec_);
-    for (uint32_t *p : {block_pairs_, chunk_offsets_, bin_ctrl_, tile_flag_, pair_keys_[0], pair_keys_[1],
+    for (uint32_t *p : {tri_draw_, block_pairs_, chunk_offsets_, bin_ctrl_, tile_flag_, pair_keys_[0], pair_keys_[1],
                         pair_vals_[0], pair_vals_[1], radix_table_, radix_totals_})
         hipFree(p);
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
@@ -241,6 +241,7 @@ hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
     if (tris > cap_tris_) {
         const uint32_t t = std::max(tris, cap_tris_ + cap_tris_ / 2);
         KZ_TRY(regrow(tri_rect_, t));
+        KZ_TRY(regrow(tri_draw_, t));
         if (radix_) {
             KZ_TRY(regrow(block_pairs_, (t + KZ_BIN_BLOCK - 1) / KZ_BIN_BLOCK));
             KZ_TRY(regrow(chunk_offsets_, (t + KZ_PAIR_CHUNK - 1) / KZ_PAIR_CHUNK));
@@ -307,9 +308,12 @@ hipError_t Engine::prepare_batch(const Op &op, uint32_t shift) {
     return hipSuccess;
 }
 
-hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
+hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
     KzFrame f = op.frame;
     if (f.total_tris == 0) return hipSuccess;
+    const uint32_t shift = cfg.shift, grid_div = cfg.grid_div;
+    // the balanced form's planned, persistent raster (else one workgroup per tile)
+    const bool persistent = balanced_ && cfg.persistent;
     std::string err;
     const PipelineKernels *kern = op.variants->get(shift, &err);
     if (!kern) {
@@ -330,10 +334,11 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     f.sort_tmp = sort_tmp_;
     f.list_capacity = list_capacity_;
     f.tri_rect = tri_rect_;
+    f.tri_draw = f.ndraws > 1 ? tri_draw_ : nullptr;
     f.tri_rec = setup_records_ ? tri_rec_ : nullptr;
     f.stats = stats_d_;
     f.tile_shift = shift;
-    if (balanced_) {
+    if (persistent) {
         f.balanced = 1;
         f.split_capacity = split_capacity_;
         f.queue = queue_;
@@ -353,14 +358,14 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
     // model will run or tiles are being profiled
     // (predictions also run in a profiled static batch, to measure the
     // model's error against the fragments the rasteriser counts)
-    const bool model = cost_model_d_ != nullptr && (balanced_ || tile_profile_);
+    const bool model = cost_model_d_ != nullptr && ((persistent && cfg.model) || tile_profile_);
     const bool acc = model || tile_profile_;
-    const bool labels = tile_profile_ && !balanced_;
+    const bool labels = tile_profile_ && !persistent;
     if (acc) {
         f.tri_z = tri_z_;
         f.tile_acc = tile_acc_;
         f.tile_feat = tile_feat_;
-        if (model && balanced_) {
+        if (model && persistent) {
             f.tile_pred = tile_pred_;
             f.frag_weight = frag_weight_;
         }
@@ -423,7 +428,7 @@ hipError_t Engine::run_batch(const Op &op, uint32_t shift, uint32_t grid_div) {
         launches += 6;
     }
 
-    if (!balanced_) {
+    if (!persistent) {
         if (acc) {
             KZ_TRY(kz_launch_tile_features(stream_, f, ntiles));
             launches += 1;
@@ -499,24 +504,46 @@ hipError_t Engine::regrow(T *&p, uint64_t count) {
 
 // Tuning candidates of a batch: every tile size whose kernels fit (full
 // grid) first; then, at the fastest tile size, a half and a quarter grid.
-hipError_t Engine::pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool &timed) {
-    shift = op.tile_shift;
-    grid_div = 1;
+BatchConfig Engine::default_config(const Op &op) const {
+    BatchConfig c;
+    c.shift = op.tile_shift;
+    c.grid_div = 1;
+    c.model = true;
+    c.persistent = balanced_;
+    return c;
+}
+
+namespace {
+int fastest(const BatchTuning &t) {
+    int best = -1;
+    for (size_t i = 0; i < t.cand.size(); ++i)
+        if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
+    return best;
+}
+}  // namespace
+
+// Candidates, one HIP-event-timed trial per submission: every tile size
+// whose kernels fit (full grid, the plan with the cost model when there is
+// one); at the fastest size, a half and a quarter grid; at the fastest of
+// those, the plan without the model and the plain raster (one workgroup per
+// tile, no plan). The fastest candidate is kept.
+hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
+    cfg = default_config(op);
     timed = false;
     if (!autotune()) return hipSuccess;
+    const bool model = cost_model_d_ != nullptr;
     if (!op.tuning) {
         op.tuning = std::make_shared<BatchTuning>();
         BatchTuning &t = *op.tuning;
         for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s)
-            if (op.variants->fits(s)) t.cand.push_back({s, 1u});
+            if (op.variants->fits(s)) t.cand.push_back({s, 1u, model, true});
         t.ms.assign(t.cand.size(), -1.0f);
         KZ_TRY(hipEventCreate(&t.ev[0]));
         KZ_TRY(hipEventCreate(&t.ev[1]));
     }
     BatchTuning &t = *op.tuning;
     if (t.done) {
-        shift = t.shift;
-        grid_div = t.grid_div;
+        cfg = t.choice;
         return hipSuccess;
     }
     if (t.pending >= 0) {  // the previous submission's trial
@@ -526,35 +553,34 @@ hipError_t Engine::pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool
         t.pending = -1;
     }
     // skip candidates whose kernels do not build
-    while (t.next < t.cand.size() && !op.variants->get(t.cand[t.next].first)) ++t.next;
-    if (t.next == t.cand.size() && !t.grid_phase) {
-        t.grid_phase = true;
-        int best = -1;
-        for (size_t i = 0; i < t.cand.size(); ++i)
-            if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
+    while (t.next < t.cand.size() && !op.variants->get(t.cand[t.next].shift)) ++t.next;
+    if (t.next == t.cand.size() && t.phase < 2) {
+        const int best = fastest(t);
         if (best >= 0) {
-            t.cand.push_back({t.cand[best].first, 2u});
-            t.cand.push_back({t.cand[best].first, 4u});
+            const BatchConfig b = t.cand[best];
+            if (t.phase == 0) {
+                t.cand.push_back({b.shift, 2u, b.model, true});
+                t.cand.push_back({b.shift, 4u, b.model, true});
+            } else {
+                if (b.model) t.cand.push_back({b.shift, b.grid_div, false, true});
+                t.cand.push_back({b.shift, 1u, false, false});
+            }
             t.ms.resize(t.cand.size(), -1.0f);
         }
+        ++t.phase;
     }
     if (t.next < t.cand.size()) {
-        shift = t.cand[t.next].first;
-        grid_div = t.cand[t.next].second;
+        cfg = t.cand[t.next];
         t.pending = (int)t.next++;
         timed = true;
         return hipSuccess;
     }
-    int best = -1;
-    for (size_t i = 0; i < t.cand.size(); ++i)
-        if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
+    const int best = fastest(t);
     t.done = true;
-    t.shift = best >= 0 ? t.cand[best].first : op.tile_shift;
-    t.grid_div = best >= 0 ? t.cand[best].second : 1u;
-    shift = t.shift;
-    grid_div = t.grid_div;
-    last_tuned_shift_ = shift;
-    last_tuned_grid_div_ = grid_div;
+    if (best >= 0) t.choice = t.cand[best];
+    else t.choice = default_config(op);
+    cfg = t.choice;
+    last_tuned_ = cfg;
     return hipSuccess;
 }
 
@@ -575,11 +601,11 @@ hipError_t Engine::execute(std::vector<Op> &ops, GraphCache *cache) {
     // every allocation before the capture
     for (Op &op : ops)
         if (op.kind == Op::Batch) {
-            uint32_t shift, grid_div;
+            BatchConfig cfg;
             bool timed;
-            KZ_TRY(pick_config(op, shift, grid_div, timed));
-            if (!op.variants->get(shift)) return execute_eager(ops);
-            KZ_TRY(prepare_batch(op, shift));
+            KZ_TRY(pick_config(op, cfg, timed));
+            if (!op.variants->get(cfg.shift)) return execute_eager(ops);
+            KZ_TRY(prepare_batch(op, cfg.shift));
         }
     if (!cache->exec || cache->generation != generation_) {
         cache->reset();
@@ -618,11 +644,11 @@ hipError_t Engine::execute(std::vector<Op> &ops, GraphCache *cache) {
 hipError_t Engine::execute_eager(std::vector<Op> &ops) {
     for (Op &op : ops)
         if (op.kind == Op::Batch) {
-            uint32_t shift, grid_div;
+            BatchConfig cfg;
             bool timed;
-            KZ_TRY(pick_config(op, shift, grid_div, timed));
+            KZ_TRY(pick_config(op, cfg, timed));
             if (timed) KZ_TRY(hipEventRecord(op.tuning->ev[0], stream_));
-            KZ_TRY(run_batch(op, shift, grid_div));
+            KZ_TRY(run_batch(op, cfg));
             if (timed) KZ_TRY(hipEventRecord(op.tuning->ev[1], stream_));
         } else {
             KZ_TRY(launch_ops_one(op));
@@ -634,7 +660,7 @@ hipError_t Engine::launch_ops(std::vector<Op> &ops) {
     for (Op &op : ops) {
         if (op.kind == Op::Batch) {
             const BatchTuning *t = op.tuning.get();
-            KZ_TRY(run_batch(op, t && t->done ? t->shift : op.tile_shift, t && t->done ? t->grid_div : 1u));
+            KZ_TRY(run_batch(op, t && t->done ? t->choice : default_config(op)));
         } else {
             KZ_TRY(launch_ops_one(op));
         }

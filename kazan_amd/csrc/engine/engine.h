@@ -86,19 +86,28 @@ class PipelineVariants {
 // candidate (tile size, persistent-raster grid) pairs with HIP events, one
 // candidate per submission; then the fastest is kept. Pixels never depend
 // on the choice.
+// How a batch runs: tile size, the persistent raster's grid divisor, and,
+// in the balanced form, whether the plan uses the cost model and whether the
+// raster is the persistent one over the planned queue (else one workgroup
+// per tile, no plan).
+struct BatchConfig {
+    uint32_t shift = 4, grid_div = 1;
+    bool model = true, persistent = true;
+};
+
 struct BatchTuning {
     BatchTuning() = default;
     BatchTuning(const BatchTuning &) = delete;
     BatchTuning &operator=(const BatchTuning &) = delete;
     ~BatchTuning();
     bool done = false;
-    bool grid_phase = false;                           // tile sizes timed; now grid sizes
-    std::vector<std::pair<uint32_t, uint32_t>> cand;   // (tile shift, grid divisor)
-    std::vector<float> ms;                             // per candidate, -1 = not timed
-    uint32_t next = 0;                                 // next candidate to try
-    int pending = -1;                                  // candidate whose events are in flight
+    int phase = 0;                   // 0: tile sizes, 1: grid sizes, 2: plan and raster forms
+    std::vector<BatchConfig> cand;
+    std::vector<float> ms;           // per candidate, -1 = not timed
+    uint32_t next = 0;               // next candidate to try
+    int pending = -1;                // candidate whose events are in flight
     hipEvent_t ev[2] = {nullptr, nullptr};
-    uint32_t shift = 4, grid_div = 1;                  // the choice once done
+    BatchConfig choice;              // once done
 };
 
 struct Op {
@@ -179,9 +188,9 @@ class Engine {
     hipError_t end_submit(hipEvent_t fence_event);
     uint32_t graph_launches() const { return graph_launches_; }
     uint32_t graph_captures() const { return graph_captures_; }
-    // the configuration the most recently finished tuning chose (0 = none yet)
-    uint32_t last_tuned_shift() const { return last_tuned_shift_; }
-    uint32_t last_tuned_grid_div() const { return last_tuned_grid_div_; }
+    // the configuration the most recently finished tuning chose (shift 0 =
+    // none yet)
+    const BatchConfig &last_tuned() const { return last_tuned_; }
     // After the submission's work is complete: read stats, adapt capacity.
     void collect_stats();
     const SubmitStats &last_stats() const { return last_; }
@@ -230,9 +239,10 @@ class Engine {
     hipError_t ensure_scratch(uint32_t tiles, uint32_t tris);
     // the configuration a batch runs with this submission (tuning trial or
     // the tuned choice), and its scratch allocated for it
-    hipError_t pick_config(Op &op, uint32_t &shift, uint32_t &grid_div, bool &timed);
+    hipError_t pick_config(Op &op, BatchConfig &cfg, bool &timed);
+    BatchConfig default_config(const Op &op) const;
     hipError_t prepare_batch(const Op &op, uint32_t shift);
-    hipError_t run_batch(const Op &op, uint32_t shift, uint32_t grid_div);
+    hipError_t run_batch(const Op &op, const BatchConfig &cfg);
     hipError_t execute_eager(std::vector<Op> &ops);
     hipError_t launch_ops(std::vector<Op> &ops);  // tuned configurations, no tuning events
     hipError_t launch_ops_one(const Op &op);      // a non-batch op
@@ -246,7 +256,7 @@ class Engine {
     bool autotune_ = true, graphs_ = true;
     uint64_t generation_ = 1;  // bumped whenever captured graphs go stale
     uint32_t graph_launches_ = 0, graph_captures_ = 0;
-    uint32_t last_tuned_shift_ = 0, last_tuned_grid_div_ = 0;
+    BatchConfig last_tuned_{0, 0, false, false};
     uint64_t list_limit_ = 0;
     // balanced-mode scratch: up to 16 work items per tile; chunk tables
     // sized from the list capacity; sub-tile lists grown on demand
@@ -276,6 +286,7 @@ class Engine {
     uint32_t *tile_counts_ = nullptr, *tile_offsets_ = nullptr, *tile_cursor_ = nullptr;
     uint32_t *tile_lists_ = nullptr, *sort_tmp_ = nullptr;
     uint64_t *tri_rect_ = nullptr;
+    uint32_t *tri_draw_ = nullptr;  // draw of each triangle, for batches of several draws
     // Binning: radix (pairs in primitive order, stable radix sort by tile)
     // or, with KAZAN_BINNING=sort, per-tile counts, an unordered scatter and
     // a sort of every list.

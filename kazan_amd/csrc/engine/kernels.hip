@@ -81,16 +81,24 @@ __device__ __forceinline__ void unpack_box(uint64_t r, uint32_t &x0, uint32_t &y
 }
 
 // Exclusive scan of in[0, n) into out (in == out allowed) by one workgroup of
-// kScanThreads; returns the total to every thread.
+// kScanThreads; returns the total to every thread. A step takes kScanPer
+// consecutive values per thread, loaded together (the scan is bound by the
+// latency of its loads, not by arithmetic).
+constexpr uint32_t kScanPer = 8;
+
 __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t *cursor, uint32_t n,
                                          uint32_t *s_wave, uint32_t *s_carry) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) *s_carry = 0;
     __syncthreads();
-    for (uint32_t b = 0; b < n; b += kScanThreads) {
-        const uint32_t i = b + tid;
-        const uint32_t v = i < n ? in[i] : 0u;
-        uint32_t x = v;
+    for (uint32_t b = 0; b < n; b += kScanThreads * kScanPer) {
+        const uint32_t i0 = b + tid * kScanPer;
+        uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPer; ++j) v[j] = i0 + j < n ? in[i0 + j] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPer; ++j) sum += v[j];
+        uint32_t x = sum;
         for (uint32_t d = 1; d < 64; d <<= 1) {
             const uint32_t o = __shfl_up(x, d, 64);
             if (lane >= d) x += o;
@@ -103,14 +111,17 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
             if (w < wave) wbase += t;
             total += t;
         }
-        const uint32_t carry = *s_carry;
-        if (i < n) {
-            const uint32_t excl = carry + wbase + x - v;
-            out[i] = excl;
-            if (cursor) cursor[i] = excl;
+        uint32_t run = *s_carry + wbase + x - sum;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPer; ++j) {
+            if (i0 + j < n) {
+                out[i0 + j] = run;
+                if (cursor) cursor[i0 + j] = run;
+            }
+            run += v[j];
         }
         __syncthreads();
-        if (tid == 0) *s_carry = carry + total;
+        if (tid == 0) *s_carry += total;
         __syncthreads();
     }
     return *s_carry;
@@ -148,7 +159,7 @@ __global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32
         if (total > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
     }
     if (tile_flag) {
-        // thread i % kScanThreads wrote offsets[i] above
+        // the scan's writes are behind its final barrier
         for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
             if (!tile_flag[i]) continue;
             offsets[i] = capacity;
@@ -601,9 +612,10 @@ __global__ __launch_bounds__(kScanThreads) void kz_pair_scan(const uint32_t *blo
     for (uint32_t c0 = 0; c0 < nchunks; c0 += kScanThreads) {
         const uint32_t c = c0 + tid;
         unsigned long long v = 0;
-        if (c < nchunks) {
-            const uint32_t b1 = min(c * kBlocksPerChunk + kBlocksPerChunk, nblocks);
-            for (uint32_t b = c * kBlocksPerChunk; b < b1; ++b) v += block_pairs[b];
+#pragma unroll
+        for (uint32_t j = 0; j < kBlocksPerChunk; ++j) {
+            const uint32_t b = c * kBlocksPerChunk + j;
+            v += c < nchunks && b < nblocks ? block_pairs[b] : 0u;
         }
         unsigned long long x = v;
         for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -975,10 +987,11 @@ __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t
 // The prediction is capped at what n triangles can cover (every sample of
 // the tile, at most 4 per pixel, once each), so an input far outside the
 // training range cannot make a tile's cost, or the frame's total, infinite.
-__device__ __forceinline__ float tile_cost(const KzFrame &f, uint32_t t, uint32_t n) {
+// (n binned triangles, pred = the model's prediction for the tile)
+__device__ __forceinline__ float tile_cost(const KzFrame &f, uint32_t n, float pred) {
     if (!f.tile_pred) return (float)n;
     const float cap = (float)n * (float)(4u << (2 * f.tile_shift));
-    return (float)n + f.frag_weight * fminf(f.tile_pred[t], cap);
+    return (float)n + f.frag_weight * fminf(pred, cap);
 }
 
 // Split level of a non-empty tile: a tile above the fair share `share`
@@ -986,11 +999,12 @@ __device__ __forceinline__ float tile_cost(const KzFrame &f, uint32_t t, uint32_
 // the frame's end, so it becomes 4 items, or 16 above 4 shares. A tile
 // whose list overflowed stays whole: its triangles come from the in-order
 // scan.
-__device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uint32_t n, float share, float &cost) {
-    cost = tile_cost(f, t, n);
+__device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uint32_t n, float pred, float share,
+                                               float &cost) {
+    cost = tile_cost(f, n, pred);
     uint32_t level = cost > 4.0f * share ? 2u : cost > share ? 1u : 0u;
     while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px wide
-    if ((uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
+    if (level > 0 && (uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
     return level;
 }
 
@@ -1003,41 +1017,85 @@ __device__ __forceinline__ uint32_t cost_bucket(float cost) {
 // One workgroup: split levels, split slots and their chunk ranges, and the
 // queue of all items ordered by bucket, most expensive first (order within
 // a bucket is arbitrary; pixels never depend on it).
+// Reserves `items` slots of LDS counter s[bucket] for every active lane and
+// returns the lane's first slot. Lanes with one item and the same bucket
+// share one atomic (a frame puts most tiles into a few buckets).
+__device__ __forceinline__ uint32_t bucket_reserve(uint32_t *s, uint32_t bucket, uint32_t items, bool active) {
+    const uint32_t lane = lane_id();
+    const bool single = active && items == 1;
+    const uint64_t m = digit_match(bucket, 5, __ballot(single));
+    uint32_t base = 0;
+    if (single && rank_in(m) == 0) base = atomicAdd(&s[bucket], (uint32_t)__popcll(m));
+    const uint32_t lead = single ? (uint32_t)__builtin_ctzll(m) : lane;
+    base = (uint32_t)__shfl((int)base, (int)lead, 64);
+    if (single) return base + rank_in(m);
+    return active ? atomicAdd(&s[bucket], items) : 0u;
+}
+
+// Tiles per thread whose loads one step of kz_plan issues together.
+constexpr uint32_t kPlanBatch = 8;
+
+__device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t t0, uint32_t ntiles, uint32_t *n, float *pred) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPlanBatch; ++j) {
+        const uint32_t t = t0 + j * kPlanThreads;
+        n[j] = t < ntiles ? f.tile_counts[t] : 0u;
+        pred[j] = t < ntiles && f.tile_pred ? f.tile_pred[t] : 0.0f;
+    }
+}
+
+// One workgroup: split levels, split slots and their chunk ranges, and the
+// queue of all items ordered by bucket, most expensive first (order within
+// a bucket is arbitrary; pixels never depend on it). Each pass over the
+// tiles takes kPlanBatch tiles per thread per step, loads first.
 __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntiles) {
     __shared__ uint32_t s_bucket[32], s_cursor[32], s_split, s_chunks;
     __shared__ float s_total[kPlanThreads / 64];
-    if (threadIdx.x < 32) s_bucket[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_split = s_chunks = 0;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 32) s_bucket[tid] = 0;
+    if (tid == 0) s_split = s_chunks = 0;
+    const uint32_t step = kPlanThreads * kPlanBatch;
     // the frame's total cost -> the fair share of one resident workgroup
     float part = 0.0f;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
-        const uint32_t n = f.tile_counts[t];
-        if (n) part += tile_cost(f, t, n);
+    for (uint32_t t0 = tid; t0 < ntiles; t0 += step) {
+        uint32_t n[kPlanBatch];
+        float pred[kPlanBatch];
+        plan_loads(f, t0, ntiles, n, pred);
+#pragma unroll
+        for (uint32_t j = 0; j < kPlanBatch; ++j)
+            if (n[j]) part += tile_cost(f, n[j], pred[j]);
     }
     for (uint32_t d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
-    if ((threadIdx.x & 63) == 0) s_total[threadIdx.x >> 6] = part;
+    if ((tid & 63) == 0) s_total[tid >> 6] = part;
     __syncthreads();
     float total = 0.0f;
     for (uint32_t w = 0; w < kPlanThreads / 64; ++w) total += s_total[w];
     const float share = fmaxf(total / (float)max(f.raster_slots, 1u), KZ_SPLIT_MIN);
-    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
-        const uint32_t n = f.tile_counts[t];
-        if (n == 0) continue;
-        float cost;
-        const uint32_t level = plan_level(f, t, n, share, cost);
-        f.tile_level[t] = level;
-        const uint32_t items = 1u << (2 * level);
-        atomicAdd(&s_bucket[cost_bucket(cost / (float)items)], items);
-        if (level) {
-            const uint32_t slot = atomicAdd(&s_split, 1u);
-            f.split_slot[t] = slot;
-            f.split_info[2 * slot] = atomicAdd(&s_chunks, (n + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
-            f.split_info[2 * slot + 1] = t;
-            for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
+    // wave-uniform loops: bucket_reserve needs every lane
+    for (uint32_t b0 = 0; b0 < ntiles; b0 += step) {
+        const uint32_t t0 = b0 + tid;
+        uint32_t n[kPlanBatch];
+        float pred[kPlanBatch];
+        plan_loads(f, t0, ntiles, n, pred);
+#pragma unroll
+        for (uint32_t j = 0; j < kPlanBatch; ++j) {
+            const uint32_t t = t0 + j * kPlanThreads;
+            float cost = 0.0f;
+            const uint32_t level = n[j] ? plan_level(f, t, n[j], pred[j], share, cost) : 0u;
+            const uint32_t items = 1u << (2 * level);
+            bucket_reserve(s_bucket, cost_bucket(cost / (float)items), items, n[j] != 0);
+            if (n[j]) f.tile_level[t] = level;
+            if (n[j] && level) {
+                const uint32_t slot = atomicAdd(&s_split, 1u);
+                f.split_slot[t] = slot;
+                f.split_info[2 * slot] = atomicAdd(&s_chunks, (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
+                f.split_info[2 * slot + 1] = t;
+                for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
+            }
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         uint32_t s = 0;
         for (int b = 31; b >= 0; --b) {
             s_cursor[b] = s;
@@ -1049,18 +1107,25 @@ __global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntil
         f.ctrl[KZ_CTRL_CHUNKS] = min(s_chunks, f.max_chunks);
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
-        const uint32_t n = f.tile_counts[t];
-        if (n == 0) continue;
-        float cost;
-        const uint32_t level = plan_level(f, t, n, share, cost);
-        const uint32_t items = 1u << (2 * level);
-        const uint32_t pos = atomicAdd(&s_cursor[cost_bucket(cost / (float)items)], items);
-        for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t, level, s);
-        if (level) {
-            const uint32_t c0 = f.split_info[2 * f.split_slot[t]];
-            const uint32_t nc = (n + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
-            for (uint32_t j = 0; j < nc && c0 + j < f.max_chunks; ++j) f.chunk_tile[c0 + j] = t;
+    for (uint32_t b0 = 0; b0 < ntiles; b0 += step) {
+        const uint32_t t0 = b0 + tid;
+        uint32_t n[kPlanBatch];
+        float pred[kPlanBatch];
+        plan_loads(f, t0, ntiles, n, pred);
+#pragma unroll
+        for (uint32_t j = 0; j < kPlanBatch; ++j) {
+            const uint32_t t = t0 + j * kPlanThreads;
+            float cost = 0.0f;
+            const uint32_t level = n[j] ? plan_level(f, t, n[j], pred[j], share, cost) : 0u;
+            const uint32_t items = 1u << (2 * level);
+            const uint32_t pos = bucket_reserve(s_cursor, cost_bucket(cost / (float)items), items, n[j] != 0);
+            if (!n[j]) continue;
+            for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t, level, s);
+            if (level) {
+                const uint32_t c0 = f.split_info[2 * f.split_slot[t]];
+                const uint32_t nc = (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+                for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c) f.chunk_tile[c0 + c] = t;
+            }
         }
     }
 }
@@ -1106,18 +1171,39 @@ __device__ __forceinline__ uint32_t sub_mask(uint64_t r, const Chunk &k) {
 
 // Entries per sub-tile of every chunk (one wave per chunk), kept per chunk
 // and added to the sub-tile totals of its split slot (zeroed by kz_plan).
+constexpr uint32_t kSplitRounds = KZ_SPLIT_CHUNK / 64;
+
+// The sub-tile masks of a chunk's entries (lane l: entries r * 64 + l): all
+// ids, then all pixel boxes, are requested before any is used, so a wave
+// waits for two loads per chunk rather than two per 64 entries.
+__device__ __forceinline__ void chunk_masks(const KzFrame &f, const Chunk &k, uint32_t lane, uint32_t *id,
+                                            uint32_t *m) {
+    uint64_t box[kSplitRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) {
+        const uint32_t e = r * 64 + lane;
+        id[r] = e < k.n ? k.ids[e] : kNone;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) box[r] = id[r] != kNone ? f.tri_rect[id[r]] : KZ_RECT_EMPTY;
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) m[r] = box[r] != KZ_RECT_EMPTY ? sub_mask(box[r], k) : 0u;
+}
+
 __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     const uint32_t lane = lane_id();
     const uint32_t nch = f.ctrl[KZ_CTRL_CHUNKS];
     const uint32_t waves = gridDim.x * (kSplitThreads / 64);
     for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
         const Chunk k = chunk_at(f, c);
+        uint32_t id[kSplitRounds], m[kSplitRounds];
+        chunk_masks(f, k, lane, id, m);
         uint32_t cnt = 0;  // lane s < 16: entries of sub-tile s
-        for (uint32_t e0 = 0; e0 < k.n; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const uint32_t m = e < k.n ? sub_mask(f.tri_rect[k.ids[e]], k) : 0u;
+#pragma unroll
+        for (uint32_t r = 0; r < kSplitRounds; ++r) {
+            if (r * 64 >= k.n) break;
             for (uint32_t s = 0; s < k.nsub; ++s) {
-                const uint32_t c_s = (uint32_t)__popcll(__ballot((m >> s) & 1u));
+                const uint32_t c_s = (uint32_t)__popcll(__ballot((m[r] >> s) & 1u));
                 if (lane == s) cnt += c_s;
             }
         }
@@ -1193,19 +1279,17 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
         before += __shfl_xor(before, 16, 64);
         before += __shfl_xor(before, 32, 64);
         uint32_t base = lane < 16 ? f.split_offsets[k.slot * 16 + lane] + before : 0u;
-        for (uint32_t e0 = 0; e0 < k.n; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            uint32_t id = 0, m = 0;
-            if (e < k.n) {
-                id = k.ids[e];
-                m = sub_mask(f.tri_rect[id], k);
-            }
+        uint32_t id[kSplitRounds], m[kSplitRounds];
+        chunk_masks(f, k, lane, id, m);
+#pragma unroll
+        for (uint32_t r = 0; r < kSplitRounds; ++r) {
+            if (r * 64 >= k.n) break;
             for (uint32_t s = 0; s < k.nsub; ++s) {
-                const uint64_t bal = __ballot((m >> s) & 1u);
+                const uint64_t bal = __ballot((m[r] >> s) & 1u);
                 const uint32_t b = readlane(base, s);
-                if ((m >> s) & 1u) {
+                if ((m[r] >> s) & 1u) {
                     const uint32_t pos = b + rank_in(bal);
-                    if (pos < f.split_capacity) f.split_lists[pos] = id;
+                    if (pos < f.split_capacity) f.split_lists[pos] = id[r];
                 }
                 if (lane == s) base += (uint32_t)__popcll(bal);
             }

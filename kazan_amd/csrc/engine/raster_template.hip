@@ -282,9 +282,33 @@ __device__ __forceinline__ uint32_t kz_cover_mask(const KzTri &t, const KzCov &c
     return m;
 }
 
-__device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, float p[3][4],
+// The draw of batch triangle g: the per-triangle table binning wrote, the
+// only draw, or a search.
+__device__ __forceinline__ uint32_t kz_draw_of(const KzFrame &f, uint32_t g) {
+    if (f.ndraws == 1) return 0u;
+    return f.tri_draw ? f.tri_draw[g] : kz_find_draw(f, g);
+}
+
+// Vertex indices of batch triangle g of draw `draw`.
+__device__ __forceinline__ void kz_triangle_indices(const KzFrame &f, uint32_t g, uint32_t draw, uint32_t idx[3]) {
+    const KzDraw &d = f.draws[draw];
+    const uint32_t t = g - d.tri_begin;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) idx[j] = kz_vertex_index(d, t, j);
+}
+
+// Vertex shader outputs of the vertices idx of draw `draw`.
+__device__ __forceinline__ void kz_fetch_vertices(const KzFrame &f, uint32_t draw, const uint32_t idx[3],
+                                                  float p[3][4], float vary[3][KZ_NVA]) {
+    const KzDraw &d = f.draws[draw];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) kz_vertex(f, d, idx[j], p[j], vary[j]);
+}
+
+// Vertex shader outputs of batch triangle g of draw `draw`.
+__device__ __forceinline__ void kz_fetch_triangle(const KzFrame &f, uint32_t g, uint32_t draw, float p[3][4],
                                                   float vary[3][KZ_NVA]) {
-    const KzDraw &d = f.draws[kz_find_draw(f, g)];
+    const KzDraw &d = f.draws[draw];
     const uint32_t t = g - d.tri_begin;
 #pragma unroll
     for (int j = 0; j < 3; ++j) kz_vertex(f, d, kz_vertex_index(d, t, j), p[j], vary[j]);
@@ -393,7 +417,9 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
     uint32_t n_setup = 0, n_unsup = 0, n_pairs = 0;
     if (g < f.total_tris) {
         float p[3][4], vary[3][KZ_NVA];
-        kz_fetch_triangle(f, g, p, vary);
+        const uint32_t draw = kz_find_draw(f, g);
+        if (f.tri_draw) f.tri_draw[g] = draw;  // the raster's lookup, one load instead of a search
+        kz_fetch_triangle(f, g, draw, p, vary);
         KzTri t;
         const int st = kz_setup(f, p, t);
         uint64_t rect = KZ_RECT_EMPTY;
@@ -604,6 +630,33 @@ __device__ __forceinline__ uint32_t kz_pack_frag(uint32_t pix, uint32_t seq, uin
     return (pix << 10) | (seq << 4) | mask;
 }
 
+// A batch's triangles, acquired one batch ahead: the ids (list entries k ..
+// k + 63) are requested while the batch before is set up, their draws and
+// vertex indices while it resolves its fragments, so that a batch starts
+// with only its vertex loads outstanding.
+struct KzAhead {
+    uint32_t k;       // first list entry
+    uint32_t g;       // the lane's triangle
+    uint32_t draw;
+    uint32_t idx[3];  // its vertex indices (not with setup records)
+    bool valid;
+};
+
+__device__ __forceinline__ void kz_ahead_ids(KzAhead &a, const uint32_t *ids, uint32_t k, uint32_t n,
+                                             uint32_t lane) {
+    a.k = k;
+    a.valid = k + lane < n;
+    a.g = a.valid ? (ids ? ids[k + lane] : k + lane) : 0u;
+}
+
+__device__ __forceinline__ void kz_ahead_indices(KzAhead &a, const KzFrame &f) {
+    a.draw = 0;
+    a.idx[0] = a.idx[1] = a.idx[2] = 0;
+    if (!a.valid || f.tri_rec) return;
+    a.draw = kz_draw_of(f, a.g);
+    kz_triangle_indices(f, a.g, a.draw, a.idx);
+}
+
 // Rasterises n triangles into the square of side 1 << RS at (X0, Y0) (a
 // whole tile, or a sub-tile of the balanced mode), in primitive order. The
 // triangles are ids[0, n) (ids = null: triangles 0 .. n-1); with `filter`,
@@ -617,6 +670,10 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
     if (n == 0) return 0;
     const int32_t RW = 1 << RS;
     const uint32_t *gcolor = (const uint32_t *)f.color;
+
+    // the first batch's ids, requested before the tile's colour and depth
+    KzAhead nx;
+    if (!filter) kz_ahead_ids(nx, ids, 0, n, lane);
 
     for (uint32_t j = lane; j < (uint32_t)(RW * RW); j += 64) {
         const int32_t x = X0 + (int32_t)(j & (RW - 1)), y = Y0 + (int32_t)(j >> RS);
@@ -635,17 +692,32 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 
     unsigned long long nfrag = 0, npass = 0;
     uint32_t k = 0;
+    if (!filter) kz_ahead_indices(nx, f);
 
     while (k < n) {
-        uint32_t g = 0;
+        uint32_t g = 0, draw = 0, idx[3] = {0u, 0u, 0u};
         bool valid = k + lane < n;
-        if (valid) g = ids ? ids[k + lane] : k + lane;
         if (filter) {
+            if (valid) g = ids ? ids[k + lane] : k + lane;
             valid = valid && kz_box_overlaps(f.tri_rect[g], X0, Y0, RW);
             if (kz_ballot(valid) == 0) {
                 k += 64;
                 continue;
             }
+            if (valid && !f.tri_rec) {
+                draw = kz_draw_of(f, g);
+                kz_triangle_indices(f, g, draw, idx);
+            }
+        } else {
+            g = nx.g;
+            valid = nx.valid;
+            draw = nx.draw;
+            idx[0] = nx.idx[0];
+            idx[1] = nx.idx[1];
+            idx[2] = nx.idx[2];
+            // the next batch's ids, assuming it starts 64 entries on (the
+            // batch cut below corrects a shorter batch)
+            kz_ahead_ids(nx, ids, k + 64, n, lane);
         }
 
         // --- per-lane triangle: vertex shader + setup -------------------
@@ -670,7 +742,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
                 kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
             } else {
                 float p[3][4], vary[3][KZ_NVA];
-                kz_fetch_triangle(f, g, p, vary);
+                kz_fetch_vertices(f, draw, idx, p, vary);
                 ok = kz_setup(f, p, L.t) == 1;
 #pragma unroll
                 for (int i = 0; i < KZ_NV; ++i) {
@@ -751,6 +823,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         const bool taken = lane < take;
         const uint32_t total = (uint32_t)kz_readlane_i((int32_t)incl, take - 1);
         const uint32_t excl = incl - cnt;
+        if (!filter && nx.k != k + take) kz_ahead_ids(nx, ids, k + take, n, lane);
 
         // --- phase 2: write the fragment queue in primitive order ---------
         if (taken && small && cmask) {
@@ -802,6 +875,8 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             }
         }
         __syncthreads();
+        // the next batch's draws and vertex indices, during this resolve
+        if (!filter) kz_ahead_indices(nx, f);
 
         // --- resolve the queue 64 fragments at a time --------------------
         for (uint32_t c0 = 0; c0 < total; c0 += 64) {

@@ -422,10 +422,13 @@ VKAPI_ATTR void VKAPI_CALL vkGetGraphStatsKAZAN(VkDevice device, uint32_t *pCapt
     if (pLaunches) *pLaunches = e.graph_launches();
 }
 
-VKAPI_ATTR void VKAPI_CALL vkGetTuningKAZAN(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor) {
-    const Engine &e = *from<Device>(device)->engine;
-    if (pTileSize) *pTileSize = e.last_tuned_shift() ? 1u << e.last_tuned_shift() : 0u;
-    if (pGridDivisor) *pGridDivisor = e.last_tuned_grid_div();
+VKAPI_ATTR void VKAPI_CALL vkGetTuningKAZAN(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor,
+                                            uint32_t *pCostModel, uint32_t *pPersistentRaster) {
+    const kz::BatchConfig &c = from<Device>(device)->engine->last_tuned();
+    if (pTileSize) *pTileSize = c.shift ? 1u << c.shift : 0u;
+    if (pGridDivisor) *pGridDivisor = c.grid_div;
+    if (pCostModel) *pCostModel = c.model ? 1u : 0u;
+    if (pPersistentRaster) *pPersistentRaster = c.persistent ? 1u : 0u;
 }
 
 VKAPI_ATTR VkResult VKAPI_CALL vkFillClusterSceneKAZAN(VkDevice device, VkBuffer buffer, VkDeviceSize offset,

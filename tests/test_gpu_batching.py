@@ -44,17 +44,22 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     s = g.zipf_draws(96, 30000, 320, 180)
     oc, od, _, ost = oracle.render(s)
     r = SceneRenderer(ctx, s)
-    for frame in range(10):
+    assert ctx.d.vkHasCostModelKAZAN(ctx.device) == 1  # the shipped model
+    for frame in range(14):
         print("frame %d" % frame, flush=True)
         st = r.render(1)
         c, d, _ = r.read_back()  # eager work between graph replays
         assert st["fragments"] == ost["fragments"], frame
         np.testing.assert_array_equal(d.view(np.uint32), od.view(np.uint32))
         assert np.abs(c.astype(np.int16) - oc.astype(np.int16)).max() <= COLOR_TOL_LSB, frame
-    # 6 timed trials (tile sizes 8..64, then two grid sizes), the choice,
-    # then a capture and graph launches for the rest
+    # 8 timed trials (tile sizes 8..64, two grid sizes, the plan without the
+    # model, the plain raster), the choice, then a capture and graph launches
+    # for the rest
     captures, launches = _graph_stats(ctx)
-    assert captures == 1 and launches == 3, (captures, launches)
+    assert captures == 1 and launches == 5, (captures, launches)
+    ts, gd, cm, pr = (ctypes.c_uint32() for _ in range(4))
+    ctx.d.vkGetTuningKAZAN(ctx.device, *(ctypes.byref(x) for x in (ts, gd, cm, pr)))
+    assert ts.value in (8, 16, 32, 64) and gd.value in (1, 2, 4) and cm.value in (0, 1) and pr.value in (0, 1)
 
 
 def test_rerecording_recaptures(driver, monkeypatch, oracle):
