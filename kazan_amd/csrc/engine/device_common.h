@@ -50,6 +50,7 @@ struct KzTexture {
 struct KzFrame {
     const KzDraw *draws;
     const uint32_t *draw_begin;  // [ndraws] draws[i].tri_begin, densely
+    const uint32_t *block_draw;  // [ceil(total_tris / KZ_BIN_BLOCK) + 1] draw of each block's first triangle, the last draw
     uint32_t ndraws;
     uint32_t total_tris;
 
@@ -208,11 +209,13 @@ enum {
     KZ_CTRL_SPLITS = 2,    // split slots used
     KZ_CTRL_CHUNKS = 3,    // chunks of split tiles' lists
     KZ_CTRL_SPLIT_MAX = 4, // largest sub-tile list total of any batch so far (sizing)
+    KZ_CTRL_SPLIT_CURSOR = 5, // sub-tile list storage reserved so far this batch
     KZ_CTRL_WORDS = 64
 };
 
-// Entries of a split tile's list that one wave cuts into sub-tile lists.
-#define KZ_SPLIT_CHUNK 1024u
+// Entries of a split tile's list that one wave cuts into sub-tile lists
+// (small enough that a frame's split tiles keep every SIMD busy).
+#define KZ_SPLIT_CHUNK 256u
 
 // Radix binning: kz_bin_count adds the (triangle, tile) pairs of each block
 // of KZ_BIN_BLOCK triangles; pairs are generated KZ_PAIR_CHUNK triangles
@@ -222,6 +225,7 @@ enum {
 enum {
     KZ_BIN_PAIRS = 0,  // pairs generated and sorted this batch (<= list capacity)
     KZ_BIN_CUT = 1,    // first pair chunk whose pairs did not fit (chunks when all fit)
+    KZ_BIN_OVER = 2,   // 1 when chunks were cut
     KZ_BIN_WORDS = 4
 };
 

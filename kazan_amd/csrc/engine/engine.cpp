@@ -51,7 +51,7 @@ Engine::~Engine() {
                         pair_vals_[0], pair_vals_[1], radix_table_, radix_totals_})
         hipFree(p);
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
-                        chunk_tile_, chunk_counts_, split_lists_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
+                        chunk_tile_, chunk_counts_, split_lists_, plan_table_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
         hipFree(p);
     hipFree(tile_feat_);
     hipFree(tile_pred_);
@@ -200,6 +200,7 @@ hipError_t Engine::ensure_split_scratch(uint32_t tiles) {
         KZ_TRY(regrow(queue_, 16ull * t));
         KZ_TRY(regrow(split_counts_, 16ull * t));
         KZ_TRY(regrow(split_offsets_, 16ull * t));
+        KZ_TRY(regrow(plan_table_, kz_plan_table_words(t)));
         if (!ctrl_) {
             KZ_TRY(hipMalloc((void **)&ctrl_, KZ_CTRL_WORDS * sizeof(uint32_t)));
             KZ_TRY(kz_launch_fill_u32(stream_, ctrl_, KZ_CTRL_WORDS, 0u));
@@ -327,6 +328,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
     KZ_TRY(prepare_batch(op, shift));
     f.draws = op.d_draws;
     f.draw_begin = op.d_draw_begin;
+    f.block_draw = op.d_block_draw;
     f.tile_counts = tile_counts_;
     f.tile_offsets = tile_offsets_;
     f.tile_cursor = tile_cursor_;
@@ -403,7 +405,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         KZ_TRY(mark(KS_BIN_SCATTER, true));
         KZ_TRY(mark(KS_BIN_SCAN, false));
         KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, nullptr, ntiles, list_capacity_, nullptr,
-                              tile_flag_));
+                              tile_flag_, bin_ctrl_));
         KZ_TRY(mark(KS_BIN_SCAN, true));
         launches += 1 + 2 + 3 * kz_radix_passes(ntiles) + 1;
     } else {
@@ -417,7 +419,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         KZ_TRY(mark(KS_BIN_HIST, true));
         KZ_TRY(mark(KS_BIN_SCAN, false));
         KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_,
-                              nullptr));
+                              nullptr, nullptr));
         KZ_TRY(mark(KS_BIN_SCAN, true));
         KZ_TRY(mark(KS_BIN_SCATTER, false));
         KZ_TRY(kz_launch_bin_scatter(stream_, f));
@@ -453,14 +455,14 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
             KZ_TRY(kz_launch_cost_mlp(stream_, cost_model_d_, tile_feat_, ntiles, tile_pred_));
             launches += 1;
         }
-        KZ_TRY(kz_launch_plan(stream_, f, ntiles));
-        KZ_TRY(kz_launch_split(stream_, f, split_grid_));
+        KZ_TRY(kz_launch_plan(stream_, f, ntiles, plan_table_));
+        KZ_TRY(kz_launch_split(stream_, f, split_grid_, ntiles));
         KZ_TRY(mark(KS_PLAN, true));
         KZ_TRY(mark(KS_RASTER, false));
         const uint32_t grid = std::min(f.raster_slots, 16 * ntiles);
         KZ_TRY(hipModuleLaunchKernel(kern->raster_persistent, grid, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
         KZ_TRY(mark(KS_RASTER, true));
-        launches += 5;
+        launches += 7;
     }
     if (acc) {
         last_profile_tiles_ = ntiles;

@@ -82,10 +82,6 @@ class PipelineVariants {
     std::string err_[kMaxTileShift + 1];
 };
 
-// Autotuning of one batch (balanced form): its first submissions time
-// candidate (tile size, persistent-raster grid) pairs with HIP events, one
-// candidate per submission; then the fastest is kept. Pixels never depend
-// on the choice.
 // How a batch runs: tile size, the persistent raster's grid divisor, and,
 // in the balanced form, whether the plan uses the cost model and whether the
 // raster is the persistent one over the planned queue (else one workgroup
@@ -95,6 +91,9 @@ struct BatchConfig {
     bool model = true, persistent = true;
 };
 
+// Autotuning of one batch (balanced form): its first submissions time
+// candidate configurations with HIP events, one candidate per submission;
+// then the fastest is kept. Pixels never depend on the choice.
 struct BatchTuning {
     BatchTuning() = default;
     BatchTuning(const BatchTuning &) = delete;
@@ -125,6 +124,7 @@ struct Op {
     std::vector<KzDraw> draws;      // host copy, uploaded at end of recording
     KzDraw *d_draws = nullptr;      // device copy owned by the command buffer
     uint32_t *d_draw_begin = nullptr;  // the draws' first triangles, in the same allocation
+    uint32_t *d_block_draw = nullptr;  // then the draw of each 256-triangle block's first triangle
     // Copy: linear bytes; Copy2D: pitched rows
     const void *src = nullptr;
     void *dst = nullptr;
@@ -262,7 +262,7 @@ class Engine {
     // sized from the list capacity; sub-tile lists grown on demand
     uint32_t *tile_level_ = nullptr, *split_slot_ = nullptr, *split_info_ = nullptr, *queue_ = nullptr,
              *ctrl_ = nullptr, *split_counts_ = nullptr, *split_offsets_ = nullptr, *chunk_tile_ = nullptr,
-             *chunk_counts_ = nullptr, *split_lists_ = nullptr;
+             *chunk_counts_ = nullptr, *split_lists_ = nullptr, *plan_table_ = nullptr;
     uint32_t cap_split_tiles_ = 0, max_chunks_ = 0, split_capacity_ = 0;
     uint32_t split_grid_ = 1024;  // workgroups of the chunk kernels: 4 per CU
     // cost model and tile profile

@@ -10,11 +10,13 @@ hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width
                                    uint32_t h, uint32_t spp, uint32_t value);
 // Exclusive scan of counts[0, n) into offsets and cursor (or null); with
 // stats, adds the total to the list-entry statistic and flags an overflow
-// beyond `capacity`. Tiles flagged in tile_flag (or null) get offset
-// `capacity` and a count of at least 1, which sends them to the raster's
-// in-order scan of all pixel boxes; their flags are cleared.
+// beyond `capacity`. When radix binning cut pairs (bin_ctrl), tiles flagged
+// in tile_flag get offset `capacity` and a count of at least 1, which sends
+// them to the raster's in-order scan of all pixel boxes; their flags are
+// cleared. tile_flag and bin_ctrl are null for the other binning.
 hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag);
+                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag,
+                          const uint32_t *bin_ctrl);
 // Radix binning, after kz_bin_count wrote f.tri_rect and f.block_pairs.
 struct KzRadixScratch {
     uint32_t *chunk_offsets;  // [ceil(tris / KZ_PAIR_CHUNK)]
@@ -55,10 +57,12 @@ hipError_t kz_launch_gen_cluster(hipStream_t st, float *vertices, uint32_t n, ui
                                  uint32_t height, double sigma_frac, double radius, uint64_t seed);
 hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntiles);
 // Balanced mode: split levels, split slots and the ordered item queue
-// (one workgroup), then the sub-tile lists of split tiles on `grid`
-// workgroups.
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles);
-hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid);
+// (3 launches over one workgroup per 1,024 tiles; `table` holds
+// kz_plan_table_words(ntiles) words of per-workgroup counts), then the
+// sub-tile lists of split tiles on `grid` workgroups.
+uint32_t kz_plan_table_words(uint32_t ntiles);
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table);
+hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid, uint32_t ntiles);
 hipError_t kz_launch_stats_reduce(hipStream_t st, unsigned long long *stats);
 hipError_t kz_launch_resolve4(hipStream_t st, const uint32_t *src, uint32_t *dst, uint32_t width, int32_t x0,
                               int32_t y0, uint32_t w, uint32_t h);

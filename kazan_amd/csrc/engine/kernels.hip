@@ -50,11 +50,15 @@ constexpr uint32_t kAccThreads = 1024;
 constexpr uint32_t kHashLogAcc = 11;
 constexpr uint32_t kHashSizeAcc = 1u << kHashLogAcc;  // 64 KB: keys, values, 6 packed statistics words
 
-constexpr uint32_t kPlanThreads = 1024;
 constexpr uint32_t kSplitThreads = 256;
 
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// the statistics stripe of this workgroup
+__device__ __forceinline__ unsigned long long *kz_stats_stripe_of(const KzFrame &f) {
+    return f.stats + (size_t)(blockIdx.x & (KZ_STAT_STRIPES - 1)) * KZ_STAT_COUNT;
 }
 
 __device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
@@ -146,11 +150,12 @@ __global__ void kz_fill_rect_u32(uint32_t *dst, uint32_t width, int32_t x0, int3
 
 // Exclusive prefix sum of counts[0, n) into offsets and cursor. With stats,
 // adds the total to the list-entry statistic and, when `capacity` is
-// exceeded, counts an overflow. Flagged tiles (tile_flag) are sent to the
-// in-order scan: offset `capacity`, count at least 1; flags are cleared.
+// exceeded, counts an overflow. After a cut (bin_ctrl[KZ_BIN_OVER]), flagged
+// tiles (tile_flag) are sent to the in-order scan: offset `capacity`, count
+// at least 1; flags are cleared.
 __global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
                                                         uint32_t n, uint32_t capacity, unsigned long long *stats,
-                                                        uint32_t *tile_flag) {
+                                                        uint32_t *tile_flag, const uint32_t *bin_ctrl) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_carry;
     const uint32_t total = block_exclusive_scan(counts, offsets, cursor, n, s_wave, &s_carry);
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32
         atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], (unsigned long long)total);
         if (total > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
     }
-    if (tile_flag) {
+    if (tile_flag && bin_ctrl[KZ_BIN_OVER]) {
         // the scan's writes are behind its final barrier
         for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
             if (!tile_flag[i]) continue;
@@ -645,6 +650,7 @@ __global__ __launch_bounds__(kScanThreads) void kz_pair_scan(const uint32_t *blo
         const bool over = cut != ~0ull;
         ctrl[KZ_BIN_CUT] = over ? (uint32_t)(cut >> 32) : nchunks;
         ctrl[KZ_BIN_PAIRS] = over ? (uint32_t)cut : (uint32_t)pairs;
+        ctrl[KZ_BIN_OVER] = over ? 1u : 0u;
         if (stats) {
             atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], pairs);
             if (over) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
@@ -1014,9 +1020,6 @@ __device__ __forceinline__ uint32_t cost_bucket(float cost) {
     return 31u - __clz((uint32_t)c + 1u);
 }
 
-// One workgroup: split levels, split slots and their chunk ranges, and the
-// queue of all items ordered by bucket, most expensive first (order within
-// a bucket is arbitrary; pixels never depend on it).
 // Reserves `items` slots of LDS counter s[bucket] for every active lane and
 // returns the lane's first slot. Lanes with one item and the same bucket
 // share one atomic (a frame puts most tiles into a few buckets).
@@ -1032,100 +1035,153 @@ __device__ __forceinline__ uint32_t bucket_reserve(uint32_t *s, uint32_t bucket,
     return active ? atomicAdd(&s[bucket], items) : 0u;
 }
 
-// Tiles per thread whose loads one step of kz_plan issues together.
-constexpr uint32_t kPlanBatch = 8;
+// The plan runs on many workgroups, in three kernels over the same tile
+// partition (workgroup w: tiles [w * kPlanTiles, ...), kPlanPer per thread):
+//   kz_plan_cost   each workgroup's part of the frame's total cost;
+//   kz_plan_count  the fair share (every workgroup sums the parts in the
+//                  same order, so all agree on it), then per workgroup the
+//                  items of each LPT bucket, its split tiles and their chunks;
+//   kz_plan_queue  from those counts, each workgroup's place in every bucket
+//                  (buckets most expensive first) and its split slots and
+//                  chunks, then the queue, the levels and the split tables.
+// table: [workgroups * kPlanCols] counts, then [workgroups] float parts.
+constexpr uint32_t kPlanWg = 256;
+constexpr uint32_t kPlanPer = 1;
+constexpr uint32_t kPlanTiles = kPlanWg * kPlanPer;
+constexpr uint32_t kPlanCols = 34;  // 32 buckets, split tiles, chunks
+constexpr uint32_t kPlanGroups = kPlanWg / kPlanCols;  // threads per column when summing the table
 
-__device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t t0, uint32_t ntiles, uint32_t *n, float *pred) {
+__device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, uint32_t *t, uint32_t *n, float *pred) {
 #pragma unroll
-    for (uint32_t j = 0; j < kPlanBatch; ++j) {
-        const uint32_t t = t0 + j * kPlanThreads;
-        n[j] = t < ntiles ? f.tile_counts[t] : 0u;
-        pred[j] = t < ntiles && f.tile_pred ? f.tile_pred[t] : 0.0f;
+    for (uint32_t j = 0; j < kPlanPer; ++j) {
+        t[j] = blockIdx.x * kPlanTiles + j * kPlanWg + threadIdx.x;
+        n[j] = t[j] < ntiles ? f.tile_counts[t[j]] : 0u;
+        pred[j] = t[j] < ntiles && f.tile_pred ? f.tile_pred[t[j]] : 0.0f;
     }
 }
 
-// One workgroup: split levels, split slots and their chunk ranges, and the
-// queue of all items ordered by bucket, most expensive first (order within
-// a bucket is arbitrary; pixels never depend on it). Each pass over the
-// tiles takes kPlanBatch tiles per thread per step, loads first.
-__global__ __launch_bounds__(kPlanThreads) void kz_plan(KzFrame f, uint32_t ntiles) {
-    __shared__ uint32_t s_bucket[32], s_cursor[32], s_split, s_chunks;
-    __shared__ float s_total[kPlanThreads / 64];
-    const uint32_t tid = threadIdx.x;
-    if (tid < 32) s_bucket[tid] = 0;
-    if (tid == 0) s_split = s_chunks = 0;
-    const uint32_t step = kPlanThreads * kPlanBatch;
-    // the frame's total cost -> the fair share of one resident workgroup
+__global__ __launch_bounds__(kPlanWg) void kz_plan_cost(KzFrame f, uint32_t ntiles, uint32_t *table) {
+    __shared__ float s_part[kPlanWg / 64];
+    uint32_t t[kPlanPer], n[kPlanPer];
+    float pred[kPlanPer];
+    plan_loads(f, ntiles, t, n, pred);
     float part = 0.0f;
-    for (uint32_t t0 = tid; t0 < ntiles; t0 += step) {
-        uint32_t n[kPlanBatch];
-        float pred[kPlanBatch];
-        plan_loads(f, t0, ntiles, n, pred);
 #pragma unroll
-        for (uint32_t j = 0; j < kPlanBatch; ++j)
-            if (n[j]) part += tile_cost(f, n[j], pred[j]);
-    }
+    for (uint32_t j = 0; j < kPlanPer; ++j)
+        if (n[j]) part += tile_cost(f, n[j], pred[j]);
     for (uint32_t d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
-    if ((tid & 63) == 0) s_total[tid >> 6] = part;
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (uint32_t w = 0; w < kPlanWg / 64; ++w) s += s_part[w];
+        ((float *)(table + gridDim.x * kPlanCols))[blockIdx.x] = s;
+    }
+}
+
+// The fair share of one resident raster workgroup. Every workgroup sums
+// the parts by the same tree (lane-strided, then waves in order), so all
+// of them compute the same float.
+__device__ __forceinline__ float plan_share(const KzFrame &f, const uint32_t *table, float *s_wave) {
+    const float *part = (const float *)(table + gridDim.x * kPlanCols);
+    float v = 0.0f;
+    for (uint32_t w = threadIdx.x; w < gridDim.x; w += kPlanWg) v += part[w];
+    for (uint32_t d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = v;
     __syncthreads();
     float total = 0.0f;
-    for (uint32_t w = 0; w < kPlanThreads / 64; ++w) total += s_total[w];
-    const float share = fmaxf(total / (float)max(f.raster_slots, 1u), KZ_SPLIT_MIN);
-    // wave-uniform loops: bucket_reserve needs every lane
-    for (uint32_t b0 = 0; b0 < ntiles; b0 += step) {
-        const uint32_t t0 = b0 + tid;
-        uint32_t n[kPlanBatch];
-        float pred[kPlanBatch];
-        plan_loads(f, t0, ntiles, n, pred);
+    for (uint32_t w = 0; w < kPlanWg / 64; ++w) total += s_wave[w];
+    return fmaxf(total / (float)max(f.raster_slots, 1u), KZ_SPLIT_MIN);
+}
+
+__global__ __launch_bounds__(kPlanWg) void kz_plan_count(KzFrame f, uint32_t ntiles, uint32_t *table) {
+    __shared__ uint32_t s_cnt[kPlanCols];
+    __shared__ float s_wave[kPlanWg / 64];
+    if (threadIdx.x < kPlanCols) s_cnt[threadIdx.x] = 0;
+    uint32_t t[kPlanPer], n[kPlanPer];
+    float pred[kPlanPer];
+    plan_loads(f, ntiles, t, n, pred);
+    const float share = plan_share(f, table, s_wave);
 #pragma unroll
-        for (uint32_t j = 0; j < kPlanBatch; ++j) {
-            const uint32_t t = t0 + j * kPlanThreads;
-            float cost = 0.0f;
-            const uint32_t level = n[j] ? plan_level(f, t, n[j], pred[j], share, cost) : 0u;
-            const uint32_t items = 1u << (2 * level);
-            bucket_reserve(s_bucket, cost_bucket(cost / (float)items), items, n[j] != 0);
-            if (n[j]) f.tile_level[t] = level;
-            if (n[j] && level) {
-                const uint32_t slot = atomicAdd(&s_split, 1u);
-                f.split_slot[t] = slot;
-                f.split_info[2 * slot] = atomicAdd(&s_chunks, (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
-                f.split_info[2 * slot + 1] = t;
-                for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
-            }
+    for (uint32_t j = 0; j < kPlanPer; ++j) {
+        float cost = 0.0f;
+        const uint32_t level = n[j] ? plan_level(f, t[j], n[j], pred[j], share, cost) : 0u;
+        const uint32_t items = 1u << (2 * level);
+        bucket_reserve(s_cnt, cost_bucket(cost / (float)items), items, n[j] != 0);
+        if (n[j] && level) {
+            atomicAdd(&s_cnt[32], 1u);
+            atomicAdd(&s_cnt[33], (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK);
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        uint32_t s = 0;
+    if (threadIdx.x < kPlanCols) table[blockIdx.x * kPlanCols + threadIdx.x] = s_cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t ntiles, const uint32_t *table) {
+    __shared__ uint32_t s_cur[kPlanCols], s_tot[kPlanCols];
+    __shared__ uint32_t s_gb[kPlanGroups][kPlanCols], s_gt[kPlanGroups][kPlanCols];
+    __shared__ float s_wave[kPlanWg / 64];
+    uint32_t t[kPlanPer], n[kPlanPer];
+    float pred[kPlanPer];
+    plan_loads(f, ntiles, t, n, pred);
+    // this workgroup's start in every column: the columns' sums over the
+    // workgroups before it, and their totals over all (kPlanGroups threads
+    // per column, each taking every kPlanGroups-th workgroup)
+    if (threadIdx.x < kPlanGroups * kPlanCols) {
+        const uint32_t col = threadIdx.x % kPlanCols, grp = threadIdx.x / kPlanCols;
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = grp; w < gridDim.x; w += kPlanGroups) {
+            const uint32_t v = table[w * kPlanCols + col];
+            if (w < blockIdx.x) before += v;
+            total += v;
+        }
+        s_gb[grp][col] = before;
+        s_gt[grp][col] = total;
+    }
+    __syncthreads();
+    if (threadIdx.x < kPlanCols) {
+        uint32_t before = 0, total = 0;
+        for (uint32_t g = 0; g < kPlanGroups; ++g) {
+            before += s_gb[g][threadIdx.x];
+            total += s_gt[g][threadIdx.x];
+        }
+        s_cur[threadIdx.x] = before;
+        s_tot[threadIdx.x] = total;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t start = 0;  // buckets most expensive first
         for (int b = 31; b >= 0; --b) {
-            s_cursor[b] = s;
-            s += s_bucket[b];
+            s_cur[b] += start;
+            start += s_tot[b];
         }
-        f.ctrl[KZ_CTRL_HEAD] = 0;
-        f.ctrl[KZ_CTRL_ITEMS] = s;
-        f.ctrl[KZ_CTRL_SPLITS] = s_split;
-        f.ctrl[KZ_CTRL_CHUNKS] = min(s_chunks, f.max_chunks);
+        if (blockIdx.x == 0) {
+            f.ctrl[KZ_CTRL_HEAD] = 0;
+            f.ctrl[KZ_CTRL_SPLIT_CURSOR] = 0;
+            f.ctrl[KZ_CTRL_ITEMS] = start;
+            f.ctrl[KZ_CTRL_SPLITS] = s_tot[32];
+            f.ctrl[KZ_CTRL_CHUNKS] = min(s_tot[33], f.max_chunks);
+        }
     }
-    __syncthreads();
-    for (uint32_t b0 = 0; b0 < ntiles; b0 += step) {
-        const uint32_t t0 = b0 + tid;
-        uint32_t n[kPlanBatch];
-        float pred[kPlanBatch];
-        plan_loads(f, t0, ntiles, n, pred);
+    const float share = plan_share(f, table, s_wave);  // (its barrier also publishes s_cur)
 #pragma unroll
-        for (uint32_t j = 0; j < kPlanBatch; ++j) {
-            const uint32_t t = t0 + j * kPlanThreads;
-            float cost = 0.0f;
-            const uint32_t level = n[j] ? plan_level(f, t, n[j], pred[j], share, cost) : 0u;
-            const uint32_t items = 1u << (2 * level);
-            const uint32_t pos = bucket_reserve(s_cursor, cost_bucket(cost / (float)items), items, n[j] != 0);
-            if (!n[j]) continue;
-            for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t, level, s);
-            if (level) {
-                const uint32_t c0 = f.split_info[2 * f.split_slot[t]];
-                const uint32_t nc = (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
-                for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c) f.chunk_tile[c0 + c] = t;
-            }
+    for (uint32_t j = 0; j < kPlanPer; ++j) {
+        float cost = 0.0f;
+        const uint32_t level = n[j] ? plan_level(f, t[j], n[j], pred[j], share, cost) : 0u;
+        const uint32_t items = 1u << (2 * level);
+        const uint32_t pos = bucket_reserve(s_cur, cost_bucket(cost / (float)items), items, n[j] != 0);
+        if (!n[j]) continue;
+        f.tile_level[t[j]] = level;
+        for (uint32_t s = 0; s < items; ++s) f.queue[pos + s] = KZ_ITEM_PACK(t[j], level, s);
+        if (level) {
+            const uint32_t nc = (n[j] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+            const uint32_t slot = atomicAdd(&s_cur[32], 1u);
+            const uint32_t c0 = atomicAdd(&s_cur[33], nc);
+            f.split_slot[t[j]] = slot;
+            f.split_info[2 * slot] = c0;
+            f.split_info[2 * slot + 1] = t[j];
+            for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
+            for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c) f.chunk_tile[c0 + c] = t[j];
         }
     }
 }
@@ -1214,50 +1270,39 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     }
 }
 
-// One workgroup: the offsets of the sub-tile lists, slot after slot (each
-// slot's 16 lists consecutive), from the totals of kz_split_count.
-__global__ __launch_bounds__(kScanThreads) void kz_split_offsets(KzFrame f) {
-    __shared__ uint32_t s_wave[kScanThreads / 64];
-    __shared__ uint32_t s_carry;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// The offsets of the sub-tile lists, one thread per split slot (its 16
+// lists consecutive): a wave reserves its slots' storage with one atomic on
+// the batch's cursor (reset by kz_plan_queue), so slots land in any order.
+__global__ __launch_bounds__(kSplitThreads) void kz_split_offsets(KzFrame f) {
+    const uint32_t lane = lane_id();
     const uint32_t n_slots = f.ctrl[KZ_CTRL_SPLITS];
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t i0 = 0; i0 < n_slots; i0 += kScanThreads) {
-        const uint32_t i = i0 + tid;
-        uint32_t c[16], v = 0;
+    const uint32_t i = blockIdx.x * kSplitThreads + threadIdx.x;
+    if (blockIdx.x * kSplitThreads >= n_slots) return;
+    uint32_t c[16], v = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+        c[s] = i < n_slots ? f.split_counts[i * 16 + s] : 0u;
+        v += c[s];
+    }
+    uint32_t x = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(x, d, 64);
+        if (lane >= d) x += o;
+    }
+    const uint32_t total = readlane(x, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) {
+        base = atomicAdd(&f.ctrl[KZ_CTRL_SPLIT_CURSOR], total);
+        atomicMax(&f.ctrl[KZ_CTRL_SPLIT_MAX], base + total);
+        atomicAdd(&kz_stats_stripe_of(f)[KZ_STAT_SPLIT_ENTRIES], (unsigned long long)total);
+    }
+    uint32_t run = readlane(base, 63) + x - v;
+    if (i < n_slots) {
 #pragma unroll
         for (uint32_t s = 0; s < 16; ++s) {
-            c[s] = i < n_slots ? f.split_counts[i * 16 + s] : 0u;
-            v += c[s];
+            f.split_offsets[i * 16 + s] = run;
+            run += c[s];
         }
-        uint32_t x = v;
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(x, d, 64);
-            if (lane >= d) x += o;
-        }
-        if (lane == 63) s_wave[wave] = x;
-        __syncthreads();
-        uint32_t run = s_carry + x - v, total = 0;
-        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
-            const uint32_t t = s_wave[w];
-            if (w < wave) run += t;
-            total += t;
-        }
-        if (i < n_slots) {
-#pragma unroll
-            for (uint32_t s = 0; s < 16; ++s) {
-                f.split_offsets[i * 16 + s] = run;
-                run += c[s];
-            }
-        }
-        __syncthreads();
-        if (tid == 0) s_carry += total;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        atomicAdd(&f.stats[KZ_STAT_SPLIT_ENTRIES], (unsigned long long)s_carry);
-        atomicMax(&f.ctrl[KZ_CTRL_SPLIT_MAX], s_carry);
     }
 }
 
@@ -1272,9 +1317,12 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
         // lane s < 16: next slot of sub-tile s, after the entries of the
         // tile's earlier chunks (lanes 16 s'..16 s' + 15 sum every 4th chunk)
         uint32_t before = 0;
-        for (uint32_t j0 = 0; j0 < k.j; j0 += 4) {
-            const uint32_t j = j0 + (lane >> 4);
-            if (j < k.j) before += f.chunk_counts[(k.c0 + j) * 16 + (lane & 15)];
+        for (uint32_t j0 = 0; j0 < k.j; j0 += 16) {
+#pragma unroll
+            for (uint32_t u = 0; u < 16; u += 4) {
+                const uint32_t j = j0 + u + (lane >> 4);
+                if (j < k.j) before += f.chunk_counts[(k.c0 + j) * 16 + (lane & 15)];
+            }
         }
         before += __shfl_xor(before, 16, 64);
         before += __shfl_xor(before, 32, 64);
@@ -1772,8 +1820,9 @@ hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width
 }
 
 hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag) {
-    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats, tile_flag);
+                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag,
+                          const uint32_t *bin_ctrl) {
+    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats, tile_flag, bin_ctrl);
     return hipGetLastError();
 }
 
@@ -1887,14 +1936,21 @@ hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntile
     return hipGetLastError();
 }
 
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles) {
-    kz_plan<<<1, kPlanThreads, 0, st>>>(f, ntiles);
+uint32_t kz_plan_table_words(uint32_t ntiles) {
+    return ((ntiles + kPlanTiles - 1) / kPlanTiles) * (kPlanCols + 1);
+}
+
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table) {
+    const uint32_t grid = std::max<uint32_t>((ntiles + kPlanTiles - 1) / kPlanTiles, 1);
+    kz_plan_cost<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
+    kz_plan_count<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
+    kz_plan_queue<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     return hipGetLastError();
 }
 
-hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid) {
+hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid, uint32_t ntiles) {
     kz_split_count<<<grid, kSplitThreads, 0, st>>>(f);
-    kz_split_offsets<<<1, kScanThreads, 0, st>>>(f);
+    kz_split_offsets<<<(ntiles + kSplitThreads - 1) / kSplitThreads, kSplitThreads, 0, st>>>(f);
     kz_split_scatter<<<grid, kSplitThreads, 0, st>>>(f);
     return hipGetLastError();
 }

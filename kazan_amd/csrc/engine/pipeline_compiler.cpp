@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <mutex>
 #include <sstream>
@@ -82,6 +83,14 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
     os << "#define KZ_DST_ALPHA " << fs.dst_alpha << "\n";
     os << "#define KZ_ALPHA_OP " << fs.alpha_op << "\n";
     os << "#define KZ_WRITE_MASK " << fs.write_mask << "\n";
+    // setup records (KAZAN_SETUP_RECORDS=1, read as the engine reads it):
+    // compiled out otherwise, so that binning drops what only records need
+    const char *rec = std::getenv("KAZAN_SETUP_RECORDS");
+    os << "#define KZ_SETUP_RECORDS " << (rec && std::strcmp(rec, "0") != 0 ? 1 : 0) << "\n";
+    // KAZAN_RASTER_WAVES=n: ask the compiler for n resident raster waves per
+    // SIMD (registers beyond the budget spill); 0 = its own choice
+    const char *waves = std::getenv("KAZAN_RASTER_WAVES");
+    os << "#define KZ_RASTER_WAVES " << (waves ? std::max(0, std::min(8, std::atoi(waves))) : 0) << "\n";
     os << kz_src_device_common << "\n";
     os << "__device__ void kz_sample(const KzTexture &t, float u, float v, float out[4]);\n";
     os << shader_source << "\n";

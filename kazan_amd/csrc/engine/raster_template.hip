@@ -98,9 +98,15 @@ __device__ __forceinline__ unsigned long long kz_wave_sum_u64(unsigned long long
 // primitive assembly
 
 // The draw record of batch triangle g: binary search over the draws' first
-// triangles (a dense 4-byte array, so the probes of a wave share cache lines).
+// triangles (a dense 4-byte array, so the probes of a wave share cache
+// lines), between the draws of the first triangles of g's 256-triangle
+// block and of the next block.
 __device__ __forceinline__ uint32_t kz_find_draw(const KzFrame &f, uint32_t g) {
     uint32_t lo = 0, hi = f.ndraws - 1;
+    if (f.block_draw) {
+        lo = f.block_draw[g / KZ_BIN_BLOCK];
+        hi = f.block_draw[g / KZ_BIN_BLOCK + 1];
+    }
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
         if (f.draw_begin[mid] <= g) lo = mid;
@@ -447,7 +453,7 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
             if (t.px0 <= t.px1 && t.py0 <= t.py1) {
                 rect = (uint64_t)(uint32_t)t.px0 | ((uint64_t)(uint32_t)t.py0 << 16) |
                        ((uint64_t)(uint32_t)t.px1 << 32) | ((uint64_t)(uint32_t)t.py1 << 48);
-                if (f.tri_rec) kz_store_rec(f, g, t, vary);
+                if (KZ_SETUP_RECORDS && f.tri_rec) kz_store_rec(f, g, t, vary);
             }
             if (f.tri_z) {  // cost-model feature: the triangle's depth range
                 const float z1 = t.z0 + t.dz1, z2 = t.z0 + t.dz2;
@@ -652,7 +658,7 @@ __device__ __forceinline__ void kz_ahead_ids(KzAhead &a, const uint32_t *ids, ui
 __device__ __forceinline__ void kz_ahead_indices(KzAhead &a, const KzFrame &f) {
     a.draw = 0;
     a.idx[0] = a.idx[1] = a.idx[2] = 0;
-    if (!a.valid || f.tri_rec) return;
+    if (!a.valid || (KZ_SETUP_RECORDS && f.tri_rec)) return;
     a.draw = kz_draw_of(f, a.g);
     kz_triangle_indices(f, a.g, a.draw, a.idx);
 }
@@ -704,7 +710,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
                 k += 64;
                 continue;
             }
-            if (valid && !f.tri_rec) {
+            if (valid && !(KZ_SETUP_RECORDS && f.tri_rec)) {
                 draw = kz_draw_of(f, g);
                 kz_triangle_indices(f, g, draw, idx);
             }
@@ -738,7 +744,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             // every listed triangle was set up by kz_bin_count: its record,
             // or (without records) the vertex shader and setup once more
             bool ok = true;
-            if (f.tri_rec) {
+            if (KZ_SETUP_RECORDS && f.tri_rec) {
                 kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
             } else {
                 float p[3][4], vary[3][KZ_NVA];
@@ -1017,7 +1023,13 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 }
 
 // Static mode: one workgroup per screen tile, in tile order.
-extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
+#if KZ_RASTER_WAVES
+#define KZ_RASTER_ATTR __attribute__((amdgpu_waves_per_eu(KZ_RASTER_WAVES)))
+#else
+#define KZ_RASTER_ATTR
+#endif
+
+extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];
@@ -1049,7 +1061,7 @@ extern "C" __global__ __launch_bounds__(64) void kz_raster(KzFrame f) {
 // Balanced mode: persistent workgroups take work items (tiles or
 // sub-tiles) from the queue, most expensive first: item b in the first
 // round, then the next untaken item from a counter.
-extern "C" __global__ __launch_bounds__(64) void kz_raster_persistent(KzFrame f) {
+extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persistent(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];

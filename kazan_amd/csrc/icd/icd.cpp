@@ -902,18 +902,28 @@ VKAPI_ATTR VkResult VKAPI_CALL vkEndCommandBuffer(VkCommandBuffer commandBuffer)
     // upload each batch's draw records once; every submission reuses them
     for (Op &op : cb->ops) {
         if (op.kind != Op::Batch || op.draws.empty()) continue;
-        // the records, then their first triangles as a dense array
+        // the records, then their first triangles as a dense array, then
+        // the draw of the first triangle of every KZ_BIN_BLOCK-triangle
+        // block (and the last draw), which narrows the device's search
         const size_t n = op.draws.size(), rec_bytes = n * sizeof(KzDraw);
-        std::vector<char> host(rec_bytes + n * sizeof(uint32_t));
+        const size_t nblk = ((size_t)op.frame.total_tris + KZ_BIN_BLOCK - 1) / KZ_BIN_BLOCK + 1;
+        std::vector<char> host(rec_bytes + (n + nblk) * sizeof(uint32_t));
         std::memcpy(host.data(), op.draws.data(), rec_bytes);
         uint32_t *begin = (uint32_t *)(host.data() + rec_bytes);
         for (size_t i = 0; i < n; ++i) begin[i] = op.draws[i].tri_begin;
+        uint32_t *block_draw = begin + n;
+        for (size_t b = 0, d = 0; b < nblk; ++b) {
+            const uint64_t g = std::min<uint64_t>((uint64_t)b * KZ_BIN_BLOCK, op.frame.total_tris ? op.frame.total_tris - 1 : 0);
+            while (d + 1 < n && op.draws[d + 1].tri_begin <= g) ++d;
+            block_draw[b] = (uint32_t)d;
+        }
         void *p = nullptr;
         if (hipMalloc(&p, host.size()) != hipSuccess) return VK_ERROR_OUT_OF_DEVICE_MEMORY;
         cb->device_allocs.push_back(p);
         if (hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return VK_ERROR_DEVICE_LOST;
         op.d_draws = (KzDraw *)p;
         op.d_draw_begin = (uint32_t *)((char *)p + rec_bytes);
+        op.d_block_draw = op.d_draw_begin + n;
         op.frame.ndraws = (uint32_t)n;
     }
     cb->recording = false;
