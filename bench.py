@@ -17,7 +17,7 @@ split by binned counts only) and "plain" (one workgroup per tile). The first
 form is the reported value; config.forms holds all of them and
 config.<form>_vs_plain the throughput ratios.
 
-  python bench.py [--gpus 1] [--steps 50] [--warmup 12] [--workload grid_1080p]
+  python bench.py [--gpus 1] [--steps 50] [--warmup 16] [--workload grid_1080p]
                   [--forms learned,balanced_static,plain]
   python bench.py --copy-probe 1073741824   # HBM counter calibration only
 """
@@ -108,12 +108,13 @@ def measure(scene, form, args):
     r.render(n_prof)
     ktimes = ctx.kernel_times()
     ctx.set_kernel_timing(False)
-    ts, gd, cm, pr = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-    ctx.d.vkGetTuningKAZAN(ctx.device, ctypes.byref(ts), ctypes.byref(gd), ctypes.byref(cm), ctypes.byref(pr))
+    ts, gd, cm, pr, ml, fs = (ctypes.c_uint32() for _ in range(6))
+    ctx.d.vkGetTuningKAZAN(ctx.device, *(ctypes.byref(x) for x in (ts, gd, cm, pr, ml, fs)))
     return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r,
             "cost_model": has_model,
             "tuned": {"tile": ts.value, "grid_divisor": gd.value, "cost_model": bool(cm.value),
-                      "persistent_raster": bool(pr.value)}}
+                      "persistent_raster": bool(pr.value), "max_split_level": ml.value,
+                      "frag_weight_scale": fs.value}}
 
 
 def copy_probe(nbytes, reps=3):
@@ -137,10 +138,11 @@ def main():
                     help="pipeline forms to measure, first = the reported one (%s)" % ",".join(FORMS))
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    # the balanced form autotunes over its first 8 submissions (one timed
-    # candidate each), runs the choice eagerly on the 9th and captures the
-    # command buffer into a hipGraph on the 10th: warm up past that
-    ap.add_argument("--warmup", type=int, default=12)
+    # the balanced form autotunes over its first (at most) 12 submissions
+    # (one timed candidate each), runs the choice eagerly on the next and
+    # captures the command buffer into a hipGraph on the one after: warm up
+    # past that
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--workload", default="grid_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

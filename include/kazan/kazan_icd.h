@@ -158,11 +158,13 @@ typedef VkResult(VKAPI_PTR *PFN_vkFillClusterSceneKAZAN)(VkDevice device, VkBuff
  * vkGetGraphStatsKAZAN: graph captures and graph launches so far.
  * vkGetTuningKAZAN: what the most recently finished tuning chose: tile size
  * (8..64, 0 before any tuning finished), persistent-grid divisor, whether
- * the plan uses the cost model, whether the raster is the persistent one
- * (any pointer may be null). */
+ * the plan uses the cost model, whether the raster is the persistent one,
+ * the deepest split the plan may make (2: 16 sub-tiles, 1: 4, 0: none), the
+ * factor on the cost model's fragment weight (any pointer may be null). */
 typedef void(VKAPI_PTR *PFN_vkGetGraphStatsKAZAN)(VkDevice device, uint32_t *pCaptures, uint32_t *pLaunches);
 typedef void(VKAPI_PTR *PFN_vkGetTuningKAZAN)(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor,
-                                              uint32_t *pCostModel, uint32_t *pPersistentRaster);
+                                              uint32_t *pCostModel, uint32_t *pPersistentRaster,
+                                              uint32_t *pMaxSplitLevel, uint32_t *pFragWeightScale);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,8 @@ Training. The target is log2(1 + fragments); features are standardised;
 the driver's MFMA training kernel (kazan_cost_model_train) runs minibatch
 Adam from He-initialised weights. frag_weight, the list-entry equivalent of
 one fragment in raster time, is the ratio of the fragment and triangle
-coefficients of a least-squares fit of the measured cycles.
+coefficients of a non-negative least-squares fit of the measured cycles,
+clipped to [W_MIN, W_MAX].
 
 Evaluation. The last tenth of the frames is held out; the report gives the
 model's error on their measured fragment counts beside that of a
@@ -97,14 +98,25 @@ def collect(ctx, frames, log):
     return tuple(np.concatenate(c) for c in cols) + (frames + len(extra),)
 
 
+W_MIN, W_MAX = 0.02, 4.0
+
+
 def fit_frag_weight(feat, frags, cycles):
-    """cycles ~ c0 + c_tri * tris + c_frag * fragments; returns c_frag / c_tri."""
+    """cycles ~ c0 + c_tri * tris + c_frag * fragments with non-negative
+    coefficients (tile cycles cannot fall with more work; when fragments
+    and triangles are nearly proportional an unconstrained fit trades one
+    coefficient against the other); returns c_frag / c_tri, clipped to
+    [W_MIN, W_MAX] (W_MAX when only fragments explain the cycles)."""
+    from scipy.optimize import nnls
     tris = np.exp2(feat[:, 0].astype(np.float64)) - 1.0
     ok = (tris > 0) & (cycles > 0)
     A = np.stack([np.ones(ok.sum()), tris[ok], frags[ok].astype(np.float64)], 1)
-    coef, *_ = np.linalg.lstsq(A, cycles[ok].astype(np.float64), rcond=None)
-    w = coef[2] / coef[1] if coef[1] > 0 else 0.25
-    return float(np.clip(w, 0.02, 4.0)), [float(c) for c in coef]
+    scale = A.max(0)
+    scale[scale == 0] = 1.0
+    coef, _ = nnls(A / scale, cycles[ok].astype(np.float64))
+    coef = coef / scale
+    w = coef[2] / coef[1] if coef[1] > 0 else (W_MAX if coef[2] > 0 else W_MIN)
+    return float(np.clip(w, W_MIN, W_MAX)), [float(c) for c in coef]
 
 
 def train(driver, x, y, steps, batch, lr, seed, state=None, step0=0):

@@ -122,6 +122,8 @@ struct KzFrame {
     uint32_t *tile_cycles;      // [tiles]
     float frag_weight;
     uint32_t raster_slots;      // workgroups of the persistent raster (split threshold)
+    uint32_t split_max_level;   // 0..2: deepest split the plan may choose
+    uint32_t pad2;
 };
 
 // Per-tile pair statistics (tile_acc) over the (triangle, tile) pairs; the

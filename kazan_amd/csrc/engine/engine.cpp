@@ -355,6 +355,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         f.split_lists = split_lists_;
         f.max_chunks = max_chunks_;
         f.raster_slots = std::max(1u, kern->persistent_blocks / std::max(grid_div, 1u));
+        f.split_max_level = std::min(cfg.max_level, 2u);
     }
     // the cost model's inputs (and, in the static form, its labels) when a
     // model will run or tiles are being profiled
@@ -369,7 +370,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         f.tile_feat = tile_feat_;
         if (model && persistent) {
             f.tile_pred = tile_pred_;
-            f.frag_weight = frag_weight_;
+            f.frag_weight = frag_weight_ * (float)std::max(cfg.frag_scale, 1u);
         }
         if (labels) {
             f.tile_frags = tile_frags_;
@@ -512,6 +513,8 @@ BatchConfig Engine::default_config(const Op &op) const {
     c.grid_div = 1;
     c.model = true;
     c.persistent = balanced_;
+    c.max_level = 2;
+    c.frag_scale = 1;
     return c;
 }
 
@@ -528,7 +531,11 @@ int fastest(const BatchTuning &t) {
 // whose kernels fit (full grid, the plan with the cost model when there is
 // one); at the fastest size, a half and a quarter grid; at the fastest of
 // those, the plan without the model and the plain raster (one workgroup per
-// tile, no plan). The fastest candidate is kept.
+// tile, no plan); at the fastest planned one, splits of at most 4 sub-tiles
+// and none (many small triangles are set up once per sub-tile they touch);
+// with a cost model, the model's plan with its fragment weight times 4 and
+// 16 (a pipeline whose fragments cost more than the training pipelines').
+// The fastest candidate is kept.
 hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     cfg = default_config(op);
     timed = false;
@@ -538,7 +545,7 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
         op.tuning = std::make_shared<BatchTuning>();
         BatchTuning &t = *op.tuning;
         for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s)
-            if (op.variants->fits(s)) t.cand.push_back({s, 1u, model, true});
+            if (op.variants->fits(s)) t.cand.push_back({s, 1u, model, true, 2u});
         t.ms.assign(t.cand.size(), -1.0f);
         KZ_TRY(hipEventCreate(&t.ev[0]));
         KZ_TRY(hipEventCreate(&t.ev[1]));
@@ -556,16 +563,26 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     }
     // skip candidates whose kernels do not build
     while (t.next < t.cand.size() && !op.variants->get(t.cand[t.next].shift)) ++t.next;
-    if (t.next == t.cand.size() && t.phase < 2) {
+    while (t.next == t.cand.size() && t.phase < 4) {
         const int best = fastest(t);
         if (best >= 0) {
             const BatchConfig b = t.cand[best];
             if (t.phase == 0) {
-                t.cand.push_back({b.shift, 2u, b.model, true});
-                t.cand.push_back({b.shift, 4u, b.model, true});
-            } else {
-                if (b.model) t.cand.push_back({b.shift, b.grid_div, false, true});
-                t.cand.push_back({b.shift, 1u, false, false});
+                t.cand.push_back({b.shift, 2u, b.model, true, 2u, 1u});
+                t.cand.push_back({b.shift, 4u, b.model, true, 2u, 1u});
+            } else if (t.phase == 1) {
+                if (b.model) t.cand.push_back({b.shift, b.grid_div, false, true, 2u, 1u});
+                t.cand.push_back({b.shift, 1u, false, false, 2u, 1u});
+            } else if (t.phase == 2) {
+                if (b.persistent) {
+                    t.cand.push_back({b.shift, b.grid_div, b.model, true, 1u, 1u});
+                    t.cand.push_back({b.shift, b.grid_div, b.model, true, 0u, 1u});
+                }
+            } else if (b.persistent && model) {
+                // (splitting by triangle counts may not have paid; by
+                // weighted fragments it may)
+                t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 4u});
+                t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 16u});
             }
             t.ms.resize(t.cand.size(), -1.0f);
         }

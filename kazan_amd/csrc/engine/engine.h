@@ -89,6 +89,11 @@ class PipelineVariants {
 struct BatchConfig {
     uint32_t shift = 4, grid_div = 1;
     bool model = true, persistent = true;
+    uint32_t max_level = 2;  // deepest split (2: 16 sub-tiles, 1: 4, 0: none, LPT order only)
+    // multiplies the model's fragment weight: the weight is fitted on the
+    // training pipelines' fragment cost, and a pipeline whose fragments cost
+    // more (a heavier fragment shader, more samples) needs a larger one
+    uint32_t frag_scale = 1;
 };
 
 // Autotuning of one batch (balanced form): its first submissions time
@@ -100,7 +105,7 @@ struct BatchTuning {
     BatchTuning &operator=(const BatchTuning &) = delete;
     ~BatchTuning();
     bool done = false;
-    int phase = 0;                   // 0: tile sizes, 1: grid sizes, 2: plan and raster forms
+    int phase = 0;  // 0: tile sizes, 1: grid sizes, 2: plan and raster forms, 3: split depth, 4: fragment weight
     std::vector<BatchConfig> cand;
     std::vector<float> ms;           // per candidate, -1 = not timed
     uint32_t next = 0;               // next candidate to try
@@ -256,7 +261,7 @@ class Engine {
     bool autotune_ = true, graphs_ = true;
     uint64_t generation_ = 1;  // bumped whenever captured graphs go stale
     uint32_t graph_launches_ = 0, graph_captures_ = 0;
-    BatchConfig last_tuned_{0, 0, false, false};
+    BatchConfig last_tuned_{0, 0, false, false, 0, 0};
     uint64_t list_limit_ = 0;
     // balanced-mode scratch: up to 16 work items per tile; chunk tables
     // sized from the list capacity; sub-tile lists grown on demand

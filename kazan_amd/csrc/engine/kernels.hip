@@ -1009,6 +1009,7 @@ __device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uin
                                                float &cost) {
     cost = tile_cost(f, n, pred);
     uint32_t level = cost > 4.0f * share ? 2u : cost > share ? 1u : 0u;
+    level = min(level, f.split_max_level);
     while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px wide
     if (level > 0 && (uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
     return level;

@@ -423,12 +423,15 @@ VKAPI_ATTR void VKAPI_CALL vkGetGraphStatsKAZAN(VkDevice device, uint32_t *pCapt
 }
 
 VKAPI_ATTR void VKAPI_CALL vkGetTuningKAZAN(VkDevice device, uint32_t *pTileSize, uint32_t *pGridDivisor,
-                                            uint32_t *pCostModel, uint32_t *pPersistentRaster) {
+                                            uint32_t *pCostModel, uint32_t *pPersistentRaster,
+                                            uint32_t *pMaxSplitLevel, uint32_t *pFragWeightScale) {
     const kz::BatchConfig &c = from<Device>(device)->engine->last_tuned();
     if (pTileSize) *pTileSize = c.shift ? 1u << c.shift : 0u;
     if (pGridDivisor) *pGridDivisor = c.grid_div;
     if (pCostModel) *pCostModel = c.model ? 1u : 0u;
     if (pPersistentRaster) *pPersistentRaster = c.persistent ? 1u : 0u;
+    if (pMaxSplitLevel) *pMaxSplitLevel = c.max_level;
+    if (pFragWeightScale) *pFragWeightScale = c.frag_scale;
 }
 
 VKAPI_ATTR VkResult VKAPI_CALL vkFillClusterSceneKAZAN(VkDevice device, VkBuffer buffer, VkDeviceSize offset,

@@ -589,7 +589,7 @@ _SIGS = {
     "vkHasCostModelKAZAN": (c_uint32, [Handle]),
     "vkFillClusterSceneKAZAN": (c_int32, [Handle, Handle, c_uint64, c_void_p]),
     "vkGetGraphStatsKAZAN": (None, [Handle, P(c_uint32), P(c_uint32)]),
-    "vkGetTuningKAZAN": (None, [Handle, P(c_uint32), P(c_uint32), P(c_uint32), P(c_uint32)]),
+    "vkGetTuningKAZAN": (None, [Handle] + [P(c_uint32)] * 6),
 }
 
 KERNEL_NAMES = ["clear_counts", "bin_count", "bin_hist", "bin_scan", "bin_scatter", "bin_sort", "plan", "raster"]

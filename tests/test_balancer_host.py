@@ -58,6 +58,21 @@ def test_frag_weight_fit_recovers_coefficients():
     assert abs(w - 0.25) < 0.01, (w, coef)
 
 
+def test_frag_weight_fit_stays_non_negative():
+    # fragments proportional to triangles and cycles driven by fragments:
+    # an unconstrained fit can return a negative triangle coefficient; the
+    # fit keeps both coefficients non-negative and the weight at its bound
+    r = np.random.default_rng(3)
+    tris = r.integers(1, 5000, 4000).astype(np.float64)
+    frags = 1.3 * tris
+    cycles = 1000 + 25 * frags + r.normal(0, 200, 4000)
+    feat = np.zeros((4000, 16), np.float32)
+    feat[:, 0] = np.log2(1 + tris)
+    w, coef = T.fit_frag_weight(feat, frags.astype(np.float32), cycles.astype(np.float32))
+    assert min(coef) >= 0, coef
+    assert T.W_MIN <= w <= T.W_MAX
+
+
 def test_keep_tiles_keeps_every_busy_tile():
     r = np.random.default_rng(2)
     tris = np.concatenate([np.zeros(100), r.integers(1, 255, 2000), r.integers(256, 9000, 50)])

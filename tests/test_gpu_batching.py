@@ -45,21 +45,25 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     oc, od, _, ost = oracle.render(s)
     r = SceneRenderer(ctx, s)
     assert ctx.d.vkHasCostModelKAZAN(ctx.device) == 1  # the shipped model
-    for frame in range(14):
+    frames = 18
+    for frame in range(frames):
         print("frame %d" % frame, flush=True)
         st = r.render(1)
         c, d, _ = r.read_back()  # eager work between graph replays
         assert st["fragments"] == ost["fragments"], frame
         np.testing.assert_array_equal(d.view(np.uint32), od.view(np.uint32))
         assert np.abs(c.astype(np.int16) - oc.astype(np.int16)).max() <= COLOR_TOL_LSB, frame
-    # 8 timed trials (tile sizes 8..64, two grid sizes, the plan without the
-    # model, the plain raster), the choice, then a capture and graph launches
-    # for the rest
-    captures, launches = _graph_stats(ctx)
-    assert captures == 1 and launches == 5, (captures, launches)
-    ts, gd, cm, pr = (ctypes.c_uint32() for _ in range(4))
-    ctx.d.vkGetTuningKAZAN(ctx.device, *(ctypes.byref(x) for x in (ts, gd, cm, pr)))
+    ts, gd, cm, pr, ml, fs = (ctypes.c_uint32() for _ in range(6))
+    ctx.d.vkGetTuningKAZAN(ctx.device, *(ctypes.byref(x) for x in (ts, gd, cm, pr, ml, fs)))
     assert ts.value in (8, 16, 32, 64) and gd.value in (1, 2, 4) and cm.value in (0, 1) and pr.value in (0, 1)
+    assert ml.value in (0, 1, 2) and fs.value in (1, 4, 16)
+    # timed trials: tile sizes 8..64, two grid sizes, the plan without the
+    # model and the plain raster; when a planned raster is fastest, two split
+    # depths and two fragment weights (with the model); then the choice, a
+    # capture and graph launches
+    trials = 8 + (4 if pr.value else 0)
+    captures, launches = _graph_stats(ctx)
+    assert captures == 1 and launches == frames - trials - 1, (captures, launches, trials)
 
 
 def test_rerecording_recaptures(driver, monkeypatch, oracle):
