@@ -75,8 +75,9 @@ struct KzFrame {
 
     // binning scratch (device memory owned by the engine)
     uint32_t *tile_counts;   // [tiles]
-    uint32_t *tile_offsets;  // [tiles] exclusive prefix of counts
+    uint32_t *tile_offsets;  // [tiles] first entry of the tile's list (any value when empty)
     uint32_t *tile_cursor;   // [tiles] scatter cursors
+    uint32_t *tile_flag;     // [tiles] nonzero: cut by the list capacity (radix binning), or null
     uint32_t *tile_lists;    // [list_capacity] triangle ids, per tile, sorted
     uint32_t *sort_tmp;      // [list_capacity] merge scratch for long lists
     uint32_t list_capacity;
@@ -233,7 +234,7 @@ enum {
 
 // Statistics are accumulated in KZ_STAT_STRIPES copies (a workgroup adds to
 // copy blockIdx % KZ_STAT_STRIPES) so that no single address takes every
-// workgroup's atomics; kz_stats_reduce folds them at the end of a submission.
+// workgroup's atomics; kz_stats_reduce folds them when the host waits.
 #define KZ_STAT_STRIPES 256
 
 // Per-triangle pixel bounding box (inclusive, clipped to the scissor),
@@ -252,3 +253,19 @@ enum {
 // Balanced mode: a tile is split when its cost (list entries) exceeds the
 // frame's cost per resident raster workgroup, and never below this.
 #define KZ_SPLIT_MIN 512.0f
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// Tile t's list: its offset and count. A tile flagged by a cut (binning
+// dropped the pairs of triangles past the list capacity, some of them in
+// this tile) reads as offset list_capacity with at least one entry: the
+// raster then takes its triangles from the in-order scan of all pixel
+// boxes, and the plan never splits it.
+__device__ __forceinline__ void kz_tile_list(const KzFrame &f, uint32_t t, uint32_t &off, uint32_t &n) {
+    off = f.tile_offsets[t];
+    n = f.tile_counts[t];
+    if (f.tile_flag && f.tile_flag[t]) {
+        off = f.list_capacity;
+        n = n > 1u ? n : 1u;
+    }
+}
+#endif

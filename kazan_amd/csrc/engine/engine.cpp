@@ -232,8 +232,7 @@ hipError_t Engine::ensure_scratch(uint32_t tiles, uint32_t tris) {
         KZ_TRY(regrow(tile_counts_, t));
         KZ_TRY(regrow(tile_offsets_, t));
         if (radix_) {
-            KZ_TRY(regrow(tile_flag_, t));
-            KZ_TRY(kz_launch_fill_u32(stream_, tile_flag_, t, 0u));  // kz_scan clears what it reads
+            KZ_TRY(regrow(tile_flag_, t));  // cleared with the counts before each batch
         } else {
             KZ_TRY(regrow(tile_cursor_, t));
         }
@@ -332,6 +331,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
     f.tile_counts = tile_counts_;
     f.tile_offsets = tile_offsets_;
     f.tile_cursor = tile_cursor_;
+    f.tile_flag = radix_ ? tile_flag_ : nullptr;
     f.tile_lists = tile_lists_;
     f.sort_tmp = sort_tmp_;
     f.list_capacity = list_capacity_;
@@ -383,13 +383,13 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
 
     // binning: pixel boxes, then the per-tile lists in primitive order
     KZ_TRY(mark(KS_CLEAR_COUNTS, false));
-    KZ_TRY(kz_launch_fill_u32(stream_, tile_counts_, ntiles, 0u));
+    KZ_TRY(kz_launch_fill_tiles(stream_, tile_counts_, tile_offsets_, radix_ ? tile_flag_ : nullptr, ntiles));
     if (acc) KZ_TRY(kz_launch_fill_u32(stream_, tile_acc_, (uint64_t)KZ_ACC * ntiles, 0u));
     KZ_TRY(mark(KS_CLEAR_COUNTS, true));
     launches += acc ? 2 : 1;
     if (radix_) {
-        // pairs in primitive order, a stable radix sort by tile (whose last
-        // pass writes the lists and counts), offsets
+        // pairs in primitive order, a stable radix sort by tile whose last
+        // pass writes the lists, counts and offsets
         f.block_pairs = block_pairs_;
         KzRadixScratch rs{chunk_offsets_, bin_ctrl_, tile_flag_, {pair_keys_[0], pair_keys_[1]},
                           {pair_vals_[0], pair_vals_[1]}, radix_table_, radix_totals_,
@@ -404,11 +404,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         KZ_TRY(mark(KS_BIN_SCATTER, false));
         KZ_TRY(kz_launch_bin_radix(stream_, f, rs, ntiles));
         KZ_TRY(mark(KS_BIN_SCATTER, true));
-        KZ_TRY(mark(KS_BIN_SCAN, false));
-        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, nullptr, ntiles, list_capacity_, nullptr,
-                              tile_flag_, bin_ctrl_));
-        KZ_TRY(mark(KS_BIN_SCAN, true));
-        launches += 1 + 2 + 3 * kz_radix_passes(ntiles) + 1;
+        launches += 1 + 2 + 3 * kz_radix_passes(ntiles);
     } else {
         // per-tile counts, offsets, an unordered scatter, a sort of each list
         KZ_TRY(mark(KS_BIN_COUNT, false));
@@ -419,8 +415,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         KZ_TRY(acc ? kz_launch_bin_hist_acc(stream_, f) : kz_launch_bin_hist(stream_, f));
         KZ_TRY(mark(KS_BIN_HIST, true));
         KZ_TRY(mark(KS_BIN_SCAN, false));
-        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_,
-                              nullptr, nullptr));
+        KZ_TRY(kz_launch_scan(stream_, tile_counts_, tile_offsets_, tile_cursor_, ntiles, list_capacity_, stats_d_));
         KZ_TRY(mark(KS_BIN_SCAN, true));
         KZ_TRY(mark(KS_BIN_SCATTER, false));
         KZ_TRY(kz_launch_bin_scatter(stream_, f));
@@ -534,8 +529,10 @@ int fastest(const BatchTuning &t) {
 // tile, no plan); at the fastest planned one, splits of at most 4 sub-tiles
 // and none (many small triangles are set up once per sub-tile they touch);
 // with a cost model, the model's plan with its fragment weight times 4 and
-// 16 (a pipeline whose fragments cost more than the training pipelines').
-// The fastest candidate is kept.
+// 16 (a pipeline whose fragments cost more than the training pipelines');
+// at the fastest planned one, the other two grid sizes again (the grid was
+// chosen with the deepest splits; with fewer, a larger grid may pay). The
+// fastest candidate is kept.
 hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     cfg = default_config(op);
     timed = false;
@@ -563,7 +560,7 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     }
     // skip candidates whose kernels do not build
     while (t.next < t.cand.size() && !op.variants->get(t.cand[t.next].shift)) ++t.next;
-    while (t.next == t.cand.size() && t.phase < 4) {
+    while (t.next == t.cand.size() && t.phase < 5) {
         const int best = fastest(t);
         if (best >= 0) {
             const BatchConfig b = t.cand[best];
@@ -578,11 +575,16 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
                     t.cand.push_back({b.shift, b.grid_div, b.model, true, 1u, 1u});
                     t.cand.push_back({b.shift, b.grid_div, b.model, true, 0u, 1u});
                 }
-            } else if (b.persistent && model) {
-                // (splitting by triangle counts may not have paid; by
-                // weighted fragments it may)
-                t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 4u});
-                t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 16u});
+            } else if (t.phase == 3) {
+                if (b.persistent && model) {
+                    // (splitting by triangle counts may not have paid; by
+                    // weighted fragments it may)
+                    t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 4u});
+                    t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 16u});
+                }
+            } else if (b.persistent) {
+                for (uint32_t d : {1u, 2u, 4u})
+                    if (d != b.grid_div) t.cand.push_back({b.shift, d, b.model, true, b.max_level, b.frag_scale});
             }
             t.ms.resize(t.cand.size(), -1.0f);
         }
@@ -715,16 +717,18 @@ hipError_t Engine::launch_ops_one(const Op &op) {
 
 hipError_t Engine::end_submit(hipEvent_t fence_event) {
     KZ_TRY(hipEventRecord(ev_end_, stream_));
-    KZ_TRY(kz_launch_stats_reduce(stream_, stats_d_));
-    KZ_TRY(hipMemcpyAsync(stats_h_, stats_d_ + (size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT,
-                          KZ_STAT_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          stream_));
     if (fence_event) KZ_TRY(hipEventRecord(fence_event, stream_));
     return hipSuccess;
 }
 
 void Engine::collect_stats() {
     if (!submit_open_) return;
+    // the striped counters are folded and read once per wait, not once per
+    // submission (they accumulate over the submissions in between)
+    if (kz_launch_stats_reduce(stream_, stats_d_) != hipSuccess ||
+        hipMemcpyAsync(stats_h_, stats_d_ + (size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT,
+                       KZ_STAT_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_) != hipSuccess)
+        std::memset(stats_h_, 0, KZ_STAT_COUNT * sizeof(unsigned long long));
     hipEventSynchronize(ev_end_);
     hipStreamSynchronize(stream_);
     std::memcpy(last_.counters, stats_h_, sizeof(last_.counters));

@@ -6,22 +6,22 @@
 #include "device_common.h"
 
 hipError_t kz_launch_fill_u32(hipStream_t st, uint32_t *dst, uint64_t n, uint32_t value);
+// Per-tile list state before a batch's binning: counts 0, offsets all ones,
+// cut flags 0 (flags may be null).
+hipError_t kz_launch_fill_tiles(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *flags, uint32_t n);
 hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width, int32_t x0, int32_t y0, uint32_t w,
                                    uint32_t h, uint32_t spp, uint32_t value);
 // Exclusive scan of counts[0, n) into offsets and cursor (or null); with
 // stats, adds the total to the list-entry statistic and flags an overflow
-// beyond `capacity`. When radix binning cut pairs (bin_ctrl), tiles flagged
-// in tile_flag get offset `capacity` and a count of at least 1, which sends
-// them to the raster's in-order scan of all pixel boxes; their flags are
-// cleared. tile_flag and bin_ctrl are null for the other binning.
+// beyond `capacity` (sort binning; radix binning's last pass writes the
+// offsets itself).
 hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag,
-                          const uint32_t *bin_ctrl);
+                          uint32_t capacity, unsigned long long *stats);
 // Radix binning, after kz_bin_count wrote f.tri_rect and f.block_pairs.
 struct KzRadixScratch {
     uint32_t *chunk_offsets;  // [ceil(tris / KZ_PAIR_CHUNK)]
     uint32_t *ctrl;           // KZ_BIN_WORDS
-    uint32_t *tile_flag;      // [tiles], zero between batches
+    uint32_t *tile_flag;      // [tiles], cleared with the counts before a batch
     uint32_t *keys[2], *vals[2];  // [list capacity] each
     uint32_t *table;          // [kz_radix_digits_max() * tstride]
     uint32_t *totals;         // [kz_radix_digits_max()]

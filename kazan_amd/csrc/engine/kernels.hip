@@ -150,12 +150,9 @@ __global__ void kz_fill_rect_u32(uint32_t *dst, uint32_t width, int32_t x0, int3
 
 // Exclusive prefix sum of counts[0, n) into offsets and cursor. With stats,
 // adds the total to the list-entry statistic and, when `capacity` is
-// exceeded, counts an overflow. After a cut (bin_ctrl[KZ_BIN_OVER]), flagged
-// tiles (tile_flag) are sent to the in-order scan: offset `capacity`, count
-// at least 1; flags are cleared.
+// exceeded, counts an overflow.
 __global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
-                                                        uint32_t n, uint32_t capacity, unsigned long long *stats,
-                                                        uint32_t *tile_flag, const uint32_t *bin_ctrl) {
+                                                        uint32_t n, uint32_t capacity, unsigned long long *stats) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_carry;
     const uint32_t total = block_exclusive_scan(counts, offsets, cursor, n, s_wave, &s_carry);
@@ -163,14 +160,15 @@ __global__ __launch_bounds__(kScanThreads) void kz_scan(uint32_t *counts, uint32
         atomicAdd(&stats[KZ_STAT_LIST_ENTRIES], (unsigned long long)total);
         if (total > capacity) atomicAdd(&stats[KZ_STAT_LIST_OVERFLOW], 1ull);
     }
-    if (tile_flag && bin_ctrl[KZ_BIN_OVER]) {
-        // the scan's writes are behind its final barrier
-        for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
-            if (!tile_flag[i]) continue;
-            offsets[i] = capacity;
-            counts[i] = max(counts[i], 1u);
-            tile_flag[i] = 0;
-        }
+}
+
+// A batch's per-tile list state before binning: counts 0, offsets all ones
+// (lowered by radix binning's last pass), cut flags 0 (flags may be null).
+__global__ void kz_fill_tiles(uint32_t *counts, uint32_t *offsets, uint32_t *flags, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        counts[i] = 0u;
+        offsets[i] = 0xFFFFFFFFu;
+        if (flags) flags[i] = 0u;
     }
 }
 
@@ -661,8 +659,8 @@ __global__ __launch_bounds__(kScanThreads) void kz_pair_scan(const uint32_t *blo
 // Writes the pairs of chunk c = blockIdx.x (triangles [c * KZ_PAIR_CHUNK,
 // ...)) from its offset on: the chunk's q-th pair is the k-th tile of the
 // triangle whose prefix range holds q. A cut chunk instead flags every tile
-// its triangles' boxes touch; kz_scan sends those tiles to the in-order
-// scan of all pixel boxes.
+// its triangles' boxes touch; kz_tile_list reads a flagged tile as one for
+// the in-order scan of all pixel boxes.
 __global__ __launch_bounds__(kPairThreads) void kz_pair_write(const uint64_t *rect, uint32_t ntris, uint32_t sh,
                                                               uint32_t tiles_x, const uint32_t *chunk_offsets,
                                                               const uint32_t *ctrl, uint32_t *keys, uint32_t *vals,
@@ -850,11 +848,14 @@ struct RadixAcc {
 // by a scan of those counts, blocks by the scanned table, digits by the
 // scanned totals. The block is regrouped by digit in LDS so that each
 // digit's run is written contiguously. The last pass (lists != null)
-// writes only the triangle ids, into the tile lists, and adds each tile's
-// run lengths to tile_counts (and, with acc, the pair statistics).
+// writes only the triangle ids, into the tile lists, adds each tile's run
+// lengths to tile_counts and lowers tile_offsets (all ones before) to the
+// runs' first positions, so that the lists need no scan of the counts
+// (and, with acc, adds the pair statistics).
 __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t *keys_in, const uint32_t *vals_in,
                                                                   uint32_t *keys_out, uint32_t *vals_out,
                                                                   uint32_t *lists, uint32_t *tile_counts,
+                                                                  uint32_t *tile_offsets,
                                                                   const uint32_t *ctrl, const uint32_t *table,
                                                                   uint32_t tstride, const uint32_t *totals,
                                                                   uint32_t shift, uint32_t bits, RadixAcc acc) {
@@ -928,9 +929,10 @@ __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t
         const uint32_t j = j0 + tid;
         const bool ok = j < nitems;
         const uint32_t k = ok ? s_key[j] : kNone, v = ok ? s_val[j] : 0u;
+        uint32_t g = 0;
         if (ok) {
             const uint32_t d = (k >> shift) & dmask;
-            const uint32_t g = s_gbase[d] + (j - s_lstart[d]);  // < npairs by construction
+            g = s_gbase[d] + (j - s_lstart[d]);  // < npairs by construction
             if (g < npairs) {
                 if (lists) {
                     lists[g] = v;
@@ -943,7 +945,10 @@ __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t
         if (!lists) continue;
         // the last pass's block is sorted by tile: one update per run of a tile
         const Run run = run_of(k, lane);
-        if (run.leader) atomicAdd(&tile_counts[k], run.len);
+        if (run.leader) {
+            atomicAdd(&tile_counts[k], run.len);
+            atomicMin(&tile_offsets[k], g);
+        }
         if (!acc.acc) continue;
         uint32_t area = 0, lo = 0, hi = 0, zinv = 0, zmax = 0;
         if (ok) {
@@ -1011,7 +1016,11 @@ __device__ __forceinline__ uint32_t plan_level(const KzFrame &f, uint32_t t, uin
     uint32_t level = cost > 4.0f * share ? 2u : cost > share ? 1u : 0u;
     level = min(level, f.split_max_level);
     while (level > 0 && f.tile_shift < level + 1) --level;  // sub-tiles at least 2 px wide
-    if (level > 0 && (uint64_t)f.tile_offsets[t] + n > (uint64_t)f.list_capacity) level = 0;
+    if (level > 0) {
+        uint32_t off, cnt;
+        kz_tile_list(f, t, off, cnt);
+        if ((uint64_t)off + cnt > (uint64_t)f.list_capacity) level = 0;
+    }
     return level;
 }
 
@@ -1056,7 +1065,9 @@ __device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, ui
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j) {
         t[j] = blockIdx.x * kPlanTiles + j * kPlanWg + threadIdx.x;
-        n[j] = t[j] < ntiles ? f.tile_counts[t[j]] : 0u;
+        uint32_t off = 0;
+        n[j] = 0;
+        if (t[j] < ntiles) kz_tile_list(f, t[j], off, n[j]);
         pred[j] = t[j] < ntiles && f.tile_pred ? f.tile_pred[t[j]] : 0.0f;
     }
 }
@@ -1812,6 +1823,12 @@ hipError_t kz_launch_fill_u32(hipStream_t st, uint32_t *dst, uint64_t n, uint32_
     return hipGetLastError();
 }
 
+hipError_t kz_launch_fill_tiles(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *flags, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    kz_fill_tiles<<<grid_for(n, 256), 256, 0, st>>>(counts, offsets, flags, n);
+    return hipGetLastError();
+}
+
 hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width, int32_t x0, int32_t y0, uint32_t w,
                                    uint32_t h, uint32_t spp, uint32_t value) {
     const uint64_t n = (uint64_t)w * h * spp;
@@ -1821,9 +1838,8 @@ hipError_t kz_launch_fill_rect_u32(hipStream_t st, uint32_t *dst, uint32_t width
 }
 
 hipError_t kz_launch_scan(hipStream_t st, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint32_t n,
-                          uint32_t capacity, unsigned long long *stats, uint32_t *tile_flag,
-                          const uint32_t *bin_ctrl) {
-    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats, tile_flag, bin_ctrl);
+                          uint32_t capacity, unsigned long long *stats) {
+    kz_scan<<<1, kScanThreads, 0, st>>>(counts, offsets, cursor, n, capacity, stats);
     return hipGetLastError();
 }
 
@@ -1865,7 +1881,8 @@ hipError_t kz_launch_bin_radix(hipStream_t st, const KzFrame &f, const KzRadixSc
         kz_radix_scan<<<1u << pb, kScanThreads, 0, st>>>(s.table, s.tstride, s.ctrl, s.totals);
         kz_radix_scatter<<<s.tstride, kRadixThreads, 0, st>>>(
             s.keys[in], s.vals[in], last ? nullptr : s.keys[out], last ? nullptr : s.vals[out],
-            last ? f.tile_lists : nullptr, last ? f.tile_counts : nullptr, s.ctrl, s.table, s.tstride, s.totals,
+            last ? f.tile_lists : nullptr, last ? f.tile_counts : nullptr, last ? f.tile_offsets : nullptr, s.ctrl,
+            s.table, s.tstride, s.totals,
             shift, pb, last ? acc : RadixAcc{nullptr, nullptr, nullptr, 0, 0});
     }
     return hipGetLastError();

@@ -87,10 +87,12 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
     // compiled out otherwise, so that binning drops what only records need
     const char *rec = std::getenv("KAZAN_SETUP_RECORDS");
     os << "#define KZ_SETUP_RECORDS " << (rec && std::strcmp(rec, "0") != 0 ? 1 : 0) << "\n";
-    // KAZAN_RASTER_WAVES=n: ask the compiler for n resident raster waves per
-    // SIMD (registers beyond the budget spill); 0 = its own choice
+    // KAZAN_RASTER_WAVES=n: ask the compiler for at least n resident raster
+    // waves per SIMD (registers beyond the budget spill); 0 = its own
+    // choice. Default 3: binds only the heaviest shaders (an 8-light Phong
+    // raster would otherwise take a few registers more than 3 waves allow)
     const char *waves = std::getenv("KAZAN_RASTER_WAVES");
-    os << "#define KZ_RASTER_WAVES " << (waves ? std::max(0, std::min(8, std::atoi(waves))) : 0) << "\n";
+    os << "#define KZ_RASTER_WAVES " << (waves ? std::max(0, std::min(8, std::atoi(waves))) : 3) << "\n";
     os << kz_src_device_common << "\n";
     os << "__device__ void kz_sample(const KzTexture &t, float u, float v, float out[4]);\n";
     os << shader_source << "\n";

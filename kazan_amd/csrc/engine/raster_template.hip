@@ -134,7 +134,12 @@ struct KzTri {
 
 // 0 = dropped (clip-rejected, degenerate or culled), 1 = set up,
 // 2 = unsupported (needs z or guard-band clipping; dropped and counted).
+// Straight-line: every field of t is written whatever the outcome (the
+// checks pick the status at the end), so that a wave whose lanes end in
+// different cases takes no branches.
 __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], KzTri &t) {
+    // trivially rejected: all three vertices outside one clip plane
+    bool reject = false;
 #pragma unroll
     for (int plane = 0; plane < 6; ++plane) {
         int out = 0;
@@ -151,15 +156,15 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
             }
             out += o ? 1 : 0;
         }
-        if (out == 3) return 0;
+        reject |= out == 3;
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (!(p[k][3] > 0.0f) || !(p[k][2] >= 0.0f) || !(p[k][2] <= p[k][3])) return 2;
+    // needs clipping: w <= 0, z outside [0, w], or outside the guard band
+    bool unsup = false;
     int32_t sx[3], sy[3];
     float zf[3], iw[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+        unsup |= !(p[k][3] > 0.0f) || !(p[k][2] >= 0.0f) || !(p[k][2] <= p[k][3]);
         // x / 1 == x exactly: skip the IEEE divisions when w is exactly one
         const bool w1 = p[k][3] == 1.0f;
         const float xd = w1 ? p[k][0] : p[k][0] / p[k][3];
@@ -168,9 +173,11 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
         const float xf = xd * f.vp_hw + f.vp_ox;
         const float yf = yd * f.vp_hh + f.vp_oy;
         zf[k] = zd * f.vp_dz + f.vp_zmin;
-        if (!(xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f)) return 2;
-        sx[k] = (int32_t)__builtin_rintf(xf * 16.0f);
-        sy[k] = (int32_t)__builtin_rintf(yf * 16.0f);
+        const bool band = xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f;
+        unsup |= !band;
+        // (outside the band, or NaN: a stand-in that converts; the status discards it)
+        sx[k] = (int32_t)__builtin_rintf(band ? xf * 16.0f : 0.0f);
+        sy[k] = (int32_t)__builtin_rintf(band ? yf * 16.0f : 0.0f);
         iw[k] = w1 ? 1.0f : 1.0f / p[k][3];
     }
     t.x0 = sx[0];
@@ -180,10 +187,8 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
     t.d2x = sx[2] - sx[0];
     t.d2y = sy[2] - sy[0];
     const int64_t S = (int64_t)t.d1x * t.d2y - (int64_t)t.d2x * t.d1y;
-    if (S == 0) return 0;
     const bool front = (KZ_FRONT_FACE == 0) ? (S < 0) : (S > 0);
-    if ((KZ_CULL_MODE & 1) && front) return 0;
-    if ((KZ_CULL_MODE & 2) && !front) return 0;
+    const bool culled = ((KZ_CULL_MODE & 1) && front) || ((KZ_CULL_MODE & 2) && !front);
     t.S = S;
     t.invS = 1.0f / (float)S;
     t.z0 = zf[0];
@@ -198,7 +203,7 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
     t.px1 = min(xmax >> 4, f.sc_x1);
     t.py0 = max(ymin >> 4, f.sc_y0);
     t.py1 = min(ymax >> 4, f.sc_y1);
-    return 1;
+    return reject ? 0 : unsup ? 2 : (S == 0 || culled) ? 0 : 1;
 }
 
 // Coverage constants: the interior-positive orientation and the top-left
@@ -246,6 +251,29 @@ __device__ __forceinline__ void kz_edges(int32_t x0, int32_t y0, int32_t d1x, in
     e12 = S - e01 - e20;
 }
 
+// kz_edges' values converted to float. `narrow` (wave-uniform): the triangle
+// has short edges (kz_small_deltas), so the values fit 32 bits and the
+// int32 -> float conversion rounds them as the int64 one does (to nearest
+// even), without the 64-bit products and conversions.
+__device__ __forceinline__ void kz_edges_f(bool narrow, int32_t x0, int32_t y0, int32_t d1x, int32_t d1y,
+                                           int32_t d2x, int32_t d2y, int64_t S, int32_t sx, int32_t sy, float &f01,
+                                           float &f20, float &f12) {
+    if (narrow) {
+        const int32_t rx = sx - x0, ry = sy - y0;
+        const int32_t a = __mul24(d1x, ry) - __mul24(d1y, rx);
+        const int32_t b = __mul24(d2y, rx) - __mul24(d2x, ry);
+        f01 = (float)a;
+        f20 = (float)b;
+        f12 = (float)((int32_t)S - a - b);
+        return;
+    }
+    int64_t e01, e20, e12;
+    kz_edges(x0, y0, d1x, d1y, d2x, d2y, S, sx, sy, e01, e20, e12);
+    f01 = (float)e01;
+    f20 = (float)e20;
+    f12 = (float)e12;
+}
+
 __device__ __forceinline__ bool kz_inside(int64_t e01, int64_t e20, int64_t e12, int32_t flip, int32_t th01,
                                           int32_t th12, int32_t th20) {
     if (flip) {
@@ -286,6 +314,59 @@ __device__ __forceinline__ uint32_t kz_cover_mask(const KzTri &t, const KzCov &c
         if (kz_inside(e01, e20, e12, c.flip, c.th01, c.th12, c.th20)) m |= 1u << s;
     }
     return m;
+}
+
+// Incremental coverage of a triangle with c.small, walked pixel by pixel
+// (row-major) over a box of its pixel box. Per edge E01, E12, E20 it keeps
+// the value at the current pixel's first sample with the orientation applied
+// and the top-left threshold subtracted, so a sample is inside iff its three
+// values are >= 0, i.e. iff their bitwise OR is. E = A * rx + B * ry + C with
+// (A, B, C) = (-d1y, d1x, 0) for E01, (d2y, -d2x, 0) for E20 and
+// (d1y - d2y, d2x - d1x, S) for E12; a step to the next pixel of a row adds
+// 16 * A, from a row's last pixel (box width w) to the next row's first
+// 16 * (B - (w - 1) * A); another sample of the pixel adds a constant
+// offset. Every value is an edge value at a point of the pixel box, so it
+// fits 32 bits (c.small) and equals the one kz_cover_mask computes.
+struct KzWalk {
+    int32_t e[3];                // at the current pixel
+    int32_t ax[3], wy[3];        // step to the next pixel, to the next row
+    int32_t o[3][KZ_SAMPLES];    // sample s relative to sample 0
+};
+
+// the walk of the box of width w whose first pixel is (px, py)
+__device__ __forceinline__ void kz_walk_init(KzWalk &w, const KzTri &t, const KzCov &c, int32_t px, int32_t py,
+                                             int32_t width) {
+    const int32_t sg = c.flip ? -1 : 1;
+    const int32_t A[3] = {-t.d1y * sg, (t.d1y - t.d2y) * sg, t.d2y * sg};
+    const int32_t B[3] = {t.d1x * sg, (t.d2x - t.d1x) * sg, -t.d2x * sg};
+    const int32_t C[3] = {0, (int32_t)t.S * sg, 0};
+    const int32_t th[3] = {c.th01, c.th12, c.th20};
+    const int32_t rx = px * 16 + kz_sample_x[0] - t.x0, ry = py * 16 + kz_sample_y[0] - t.y0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w.e[k] = __mul24(A[k], rx) + __mul24(B[k], ry) + C[k] - th[k];
+        w.ax[k] = A[k] * 16;
+        w.wy[k] = (B[k] - __mul24(width - 1, A[k])) * 16;
+#pragma unroll
+        for (int s = 0; s < KZ_SAMPLES; ++s)
+            w.o[k][s] = __mul24(A[k], kz_sample_x[s] - kz_sample_x[0]) + __mul24(B[k], kz_sample_y[s] - kz_sample_y[0]);
+    }
+}
+
+__device__ __forceinline__ uint32_t kz_walk_mask(const KzWalk &w) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < KZ_SAMPLES; ++s) {
+        const int32_t v = (w.e[0] + w.o[0][s]) | (w.e[1] + w.o[1][s]) | (w.e[2] + w.o[2][s]);
+        m |= (v >= 0 ? 1u : 0u) << s;
+    }
+    return m;
+}
+
+// to the next pixel; `wrap`: the current one ends its row
+__device__ __forceinline__ void kz_walk_step(KzWalk &w, bool wrap) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w.e[k] += wrap ? w.wy[k] : w.ax[k];
 }
 
 // The draw of batch triangle g: the per-triangle table binning wrote, the
@@ -437,14 +518,33 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
             if (t.px0 <= t.px1 && t.py0 <= t.py1 && (t.px1 - t.px0 + 1) * (t.py1 - t.py0 + 1) <= KZ_TIGHT_CAND) {
                 const KzCov c = kz_cov_setup(t);
                 int32_t cx0 = t.px1 + 1, cy0 = t.py1 + 1, cx1 = t.px0 - 1, cy1 = t.py0 - 1;
-                for (int32_t py = t.py0; py <= t.py1; ++py)
-                    for (int32_t px = t.px0; px <= t.px1; ++px)
-                        if (kz_cover_mask(t, c, px, py)) {
+                if (c.small) {
+                    KzWalk wk;
+                    kz_walk_init(wk, t, c, t.px0, t.py0, t.px1 - t.px0 + 1);
+                    int32_t px = t.px0, py = t.py0;
+                    const int32_t nc = (t.px1 - t.px0 + 1) * (t.py1 - t.py0 + 1);
+                    for (int32_t j = 0; j < nc; ++j) {
+                        if (kz_walk_mask(wk)) {
                             cx0 = min(cx0, px);
                             cx1 = max(cx1, px);
                             cy0 = min(cy0, py);
                             cy1 = max(cy1, py);
                         }
+                        const bool wrap = px == t.px1;
+                        kz_walk_step(wk, wrap);
+                        py += wrap ? 1 : 0;
+                        px = wrap ? t.px0 : px + 1;
+                    }
+                } else {
+                    for (int32_t py = t.py0; py <= t.py1; ++py)
+                        for (int32_t px = t.px0; px <= t.px1; ++px)
+                            if (kz_cover_mask(t, c, px, py)) {
+                                cx0 = min(cx0, px);
+                                cx1 = max(cx1, px);
+                                cy0 = min(cy0, py);
+                                cy1 = max(cy1, py);
+                            }
+                }
                 t.px0 = cx0;
                 t.px1 = cx1;
                 t.py0 = cy0;
@@ -495,6 +595,8 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
 
 __device__ __forceinline__ uint32_t kz_wrap(int32_t i, uint32_t n, uint32_t mode) {
     if (mode == 0) {  // REPEAT
+        // a power-of-two size (wave-uniform): the low bits are the residue
+        if ((n & (n - 1)) == 0) return (uint32_t)i & (n - 1);
         const int32_t r = i % (int32_t)n;
         return (uint32_t)(r < 0 ? r + (int32_t)n : r);
     }
@@ -663,6 +765,125 @@ __device__ __forceinline__ void kz_ahead_indices(KzAhead &a, const KzFrame &f) {
     kz_triangle_indices(f, a.g, a.draw, a.idx);
 }
 
+// Resolves the fragment queue s_frag[0, total) of the batch held by the
+// wave's lanes (L), 64 fragments at a time: depth test and write, varyings,
+// fragment shader and blend, in primitive order per pixel. NARROW: every
+// triangle of the batch has short edges (32-bit edge values).
+template <bool NARROW>
+__device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L, int32_t X0, int32_t Y0,
+                                           uint32_t total, uint32_t lane, uint32_t *s_color, float *s_depth,
+                                           uint32_t *s_claim, const uint32_t *s_frag, unsigned long long &nfrag,
+                                           unsigned long long &npass) {
+    for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+        const uint32_t qi = c0 + lane;
+        const bool active = qi < total;
+        const uint32_t e = active ? s_frag[qi] : 0u;
+        const uint32_t pix = e >> 10, seq = (e >> 4) & 63u, mask = e & 15u;
+        // the owning triangle's data, from lane `seq`
+        const int32_t x0 = kz_bperm_i(L.t.x0, seq), y0 = kz_bperm_i(L.t.y0, seq);
+        const int32_t d1x = kz_bperm_i(L.t.d1x, seq), d1y = kz_bperm_i(L.t.d1y, seq);
+        const int32_t d2x = kz_bperm_i(L.t.d2x, seq), d2y = kz_bperm_i(L.t.d2y, seq);
+        const int32_t s_lo = kz_bperm_i((int32_t)(uint32_t)(uint64_t)L.t.S, seq);
+        const int32_t s_hi = NARROW ? (s_lo >> 31) : kz_bperm_i((int32_t)((uint64_t)L.t.S >> 32), seq);
+        const int64_t S = (int64_t)(((uint64_t)(uint32_t)s_lo) | ((uint64_t)(uint32_t)s_hi << 32));
+        const float invS = kz_bperm_f(L.t.invS, seq);
+        const float z0 = kz_bperm_f(L.t.z0, seq), dz1 = kz_bperm_f(L.t.dz1, seq), dz2 = kz_bperm_f(L.t.dz2, seq);
+        const int32_t px = X0 + (int32_t)(pix & (KZ_TS - 1)), py = Y0 + (int32_t)(pix >> KZ_TILE_SHIFT);
+
+        float zs[KZ_SAMPLES];
+        uint32_t nf = 0;
+#pragma unroll
+        for (int s = 0; s < KZ_SAMPLES; ++s) {
+            zs[s] = 0.0f;
+            if ((mask >> s) & 1u) {
+                ++nf;
+                float e01, e20, e12;
+                kz_edges_f(NARROW, x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + kz_sample_x[s],
+                           py * 16 + kz_sample_y[s], e01, e20, e12);
+                const float l1 = e20 * invS, l2 = e01 * invS;
+                const float z = (z0 + l1 * dz1) + l2 * dz2;
+                zs[s] = fminf(fmaxf(z, f.z_lo), f.z_hi);
+            }
+        }
+        nfrag += nf;
+
+        // The samples that can still pass the depth test: those that pass
+        // against the depth the tile holds now. The round's earlier
+        // fragments only move a pixel's depth away from passing (or leave
+        // it), so the others fail after them too; only NOT_EQUAL with depth
+        // writes is not monotone. The fragment shader runs for the rest.
+        uint32_t maybe = active ? mask : 0u;
+#if KZ_DEPTH_TEST && !(KZ_DEPTH_OP == 5 && KZ_DEPTH_WRITE)
+        {
+            uint32_t m = 0;
+#pragma unroll
+            for (int s = 0; s < KZ_SAMPLES; ++s)
+                if (((maybe >> s) & 1u) && kz_depth_cmp(zs[s], s_depth[pix * KZ_SAMPLES + s])) m |= 1u << s;
+            maybe = m;
+        }
+#endif
+
+        // varyings at the pixel centre and the fragment shader
+        float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            float a0[KZ_NVA], da1[KZ_NVA], da2[KZ_NVA];
+#pragma unroll
+            for (int i = 0; i < KZ_NV; ++i) {
+                a0[i] = kz_bperm_f(L.a0[i], seq);
+                da1[i] = kz_bperm_f(L.da1[i], seq);
+                da2[i] = kz_bperm_f(L.da2[i], seq);
+            }
+            const float iw0 = kz_bperm_f(L.t.iw0, seq), iw1 = kz_bperm_f(L.t.iw1, seq),
+                        iw2 = kz_bperm_f(L.t.iw2, seq);
+            if (maybe) {
+                float e01, e20, e12;
+                kz_edges_f(NARROW, x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + 8, py * 16 + 8, e01, e20, e12);
+                const float l0 = e12 * invS, l1 = e20 * invS, l2 = e01 * invS;
+                const float q = ((l0 * iw0) + (l1 * iw1)) + (l2 * iw2);
+                const float rq = 1.0f / q;
+                const float b1 = (l1 * iw1) * rq, b2 = (l2 * iw2) * rq;
+                float vary[KZ_NVA];
+#pragma unroll
+                for (int i = 0; i < KZ_NV; ++i) vary[i] = (a0[i] + b1 * da1[i]) + b2 * da2[i];
+                const float zc = fminf(fmaxf((z0 + l1 * dz1) + l2 * dz2, f.z_lo), f.z_hi);
+                const float frag_coord[4] = {(float)px + 0.5f, (float)py + 0.5f, zc, q};
+                kz_fragment(f, vary, frag_coord, col);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) col[c] = kz_clamp01(col[c]);
+            }
+        }
+
+        // depth test and write, blend and write, in primitive order per
+        // pixel: the lowest triangle of each pixel claims it, then the next
+        uint32_t pass = 0;
+        bool act = maybe != 0;
+        while (kz_ballot(act)) {
+            if (act) atomicMin(&s_claim[pix], seq);
+            __syncthreads();
+            const bool win = act && s_claim[pix] == seq;
+            if (win) {
+#pragma unroll
+                for (int s = 0; s < KZ_SAMPLES; ++s) {
+                    if (!((maybe >> s) & 1u)) continue;
+#if KZ_DEPTH_TEST
+                    float &dd = s_depth[pix * KZ_SAMPLES + s];
+                    if (!kz_depth_cmp(zs[s], dd)) continue;
+#if KZ_DEPTH_WRITE
+                    dd = zs[s];
+#endif
+#endif
+                    pass |= 1u << s;
+                    s_color[pix * KZ_SAMPLES + s] = kz_rop(s_color[pix * KZ_SAMPLES + s], col);
+                }
+                s_claim[pix] = KZ_NO_CLAIM;
+                act = false;
+            }
+            __syncthreads();
+        }
+        npass += (uint32_t)__popc(pass);
+    }
+}
+
 // Rasterises n triangles into the square of side 1 << RS at (X0, Y0) (a
 // whole tile, or a sub-tile of the balanced mode), in primitive order. The
 // triangles are ids[0, n) (ids = null: triangles 0 .. n-1); with `filter`,
@@ -728,47 +949,48 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 
         // --- per-lane triangle: vertex shader + setup -------------------
         KzLaneTri L;
-        L.live = false;
-        L.cx0 = L.cy0 = 0;
-        L.cx1 = L.cy1 = -1;
-        L.c.flip = 0;
-        L.c.th01 = L.c.th12 = L.c.th20 = 0;
-        L.c.small = 1;
-        L.t.x0 = L.t.y0 = L.t.d1x = L.t.d1y = L.t.d2x = L.t.d2y = 0;
-        L.t.S = 0;
-        L.t.invS = L.t.z0 = L.t.dz1 = L.t.dz2 = 0.0f;
-        L.t.iw0 = L.t.iw1 = L.t.iw2 = 0.0f;
+        // every listed triangle was set up by kz_bin_count: its record, or
+        // (without records) the vertex shader and setup once more. Without
+        // records every lane runs them, a lane without a triangle on vertex
+        // indices 0 of draw 0 (discarded), so that the code is straight-line.
+        bool ok;
+        if (KZ_SETUP_RECORDS && f.tri_rec) {
+            L.t.x0 = L.t.y0 = L.t.d1x = L.t.d1y = L.t.d2x = L.t.d2y = 0;
+            L.t.S = 0;
+            L.t.invS = L.t.z0 = L.t.dz1 = L.t.dz2 = 0.0f;
+            L.t.iw0 = L.t.iw1 = L.t.iw2 = 0.0f;
+            L.t.px0 = L.t.py0 = 0;
+            L.t.px1 = L.t.py1 = -1;
 #pragma unroll
-        for (int i = 0; i < KZ_NVA; ++i) L.a0[i] = L.da1[i] = L.da2[i] = 0.0f;
-        if (valid) {
-            // every listed triangle was set up by kz_bin_count: its record,
-            // or (without records) the vertex shader and setup once more
-            bool ok = true;
-            if (KZ_SETUP_RECORDS && f.tri_rec) {
-                kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
-            } else {
-                float p[3][4], vary[3][KZ_NVA];
-                kz_fetch_vertices(f, draw, idx, p, vary);
-                ok = kz_setup(f, p, L.t) == 1;
+            for (int i = 0; i < KZ_NVA; ++i) L.a0[i] = L.da1[i] = L.da2[i] = 0.0f;
+            if (valid) kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
+            ok = valid;
+        } else {
+            float p[3][4], vary[3][KZ_NVA];
+            kz_fetch_vertices(f, draw, idx, p, vary);
+            const int st = kz_setup(f, p, L.t);
+            ok = valid && st == 1;
 #pragma unroll
-                for (int i = 0; i < KZ_NV; ++i) {
-                    L.a0[i] = vary[0][i];
-                    L.da1[i] = vary[1][i] - vary[0][i];
-                    L.da2[i] = vary[2][i] - vary[0][i];
-                }
-            }
-            if (ok) {
-                L.cx0 = max(L.t.px0, X0);
-                L.cx1 = min(L.t.px1, X0 + RW - 1);
-                L.cy0 = max(L.t.py0, Y0);
-                L.cy1 = min(L.t.py1, Y0 + RW - 1);
-                L.live = L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
-                L.c = kz_cov_setup(L.t);
+            for (int i = 0; i < KZ_NVA; ++i) {
+                L.a0[i] = i < KZ_NV ? vary[0][i] : 0.0f;
+                L.da1[i] = i < KZ_NV ? vary[1][i] - vary[0][i] : 0.0f;
+                L.da2[i] = i < KZ_NV ? vary[2][i] - vary[0][i] : 0.0f;
             }
         }
+        L.cx0 = max(L.t.px0, X0);
+        L.cx1 = min(L.t.px1, X0 + RW - 1);
+        L.cy0 = max(L.t.py0, Y0);
+        L.cy1 = min(L.t.py1, Y0 + RW - 1);
+        L.live = ok && L.cx0 <= L.cx1 && L.cy0 <= L.cy1;
+        L.c = kz_cov_setup(L.t);
         const int32_t cw = L.cx1 - L.cx0 + 1;
         const int32_t cand = L.live ? cw * (L.cy1 - L.cy0 + 1) : 0;
-        const bool small = cand <= KZ_SMALL_CAND;
+        // a triangle with few candidate pixels here and short edges: one lane
+        // walks its candidates (the others: the whole wave, one at a time)
+        const bool small = cand <= KZ_SMALL_CAND && L.c.small;
+        // every triangle of the batch that can have fragments here has short
+        // edges: the resolve evaluates edges in 32 bits
+        const bool narrow = kz_ballot(L.live && !L.c.small) == 0;
 
         // --- phase 1: covered pixels per triangle ------------------------
         // (small triangles keep their sample masks, KZ_SAMPLES bits per
@@ -777,17 +999,18 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         uint64_t cmask = 0;
         {
             const int32_t maxc = kz_wave_max(small ? cand : 0, lane);
-            int32_t cx = L.cx0, cy = L.cy0;
+            KzWalk wk;
+            if (small) kz_walk_init(wk, L.t, L.c, L.cx0, L.cy0, cw);
+            int32_t dx = 0;
             for (int32_t j = 0; j < maxc; ++j) {
                 if (small && j < cand) {
-                    const uint32_t m = kz_cover_mask(L.t, L.c, cx, cy);
+                    const uint32_t m = kz_walk_mask(wk);
+                    const bool wrap = dx == cw - 1;
+                    kz_walk_step(wk, wrap);
+                    dx = wrap ? 0 : dx + 1;
                     if (m) {
                         ++cnt;
                         cmask |= (uint64_t)m << (j * KZ_SAMPLES);
-                    }
-                    if (++cx > L.cx1) {
-                        cx = L.cx0;
-                        ++cy;
                     }
                 }
             }
@@ -832,16 +1055,20 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         if (!filter && nx.k != k + take) kz_ahead_ids(nx, ids, k + take, n, lane);
 
         // --- phase 2: write the fragment queue in primitive order ---------
-        if (taken && small && cmask) {
+        if (taken && small) {  // the covered candidates, in row-major order
+            // candidate j of the box is pixel (j % cw, j / cw), and
+            // floor((j + 0.5) / cw) == j / cw for j < 64, cw <= 64 even with
+            // the reciprocal's rounding
+            const float rcw = __builtin_amdgcn_rcpf((float)cw);
             uint32_t w = excl;
-            int32_t cx = L.cx0, cy = L.cy0;
-            for (int32_t j = 0; j < cand; ++j) {
-                const uint32_t m = (uint32_t)(cmask >> (j * KZ_SAMPLES)) & ((1u << KZ_SAMPLES) - 1u);
-                if (m) s_frag[w++] = kz_pack_frag((uint32_t)((cy - Y0) * KZ_TS + (cx - X0)), lane, m);
-                if (++cx > L.cx1) {
-                    cx = L.cx0;
-                    ++cy;
-                }
+            uint64_t bits = cmask;
+            while (bits) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(bits) / KZ_SAMPLES;
+                const uint32_t m = (uint32_t)(bits >> (j * KZ_SAMPLES)) & ((1u << KZ_SAMPLES) - 1u);
+                bits &= ~((uint64_t)((1u << KZ_SAMPLES) - 1u) << (j * KZ_SAMPLES));
+                const int32_t dy = (int32_t)(((float)j + 0.5f) * rcw);
+                const int32_t cx = L.cx0 + (int32_t)j - dy * cw, cy = L.cy0 + dy;
+                s_frag[w++] = kz_pack_frag((uint32_t)((cy - Y0) * KZ_TS + (cx - X0)), lane, m);
             }
         }
         bigm = kz_ballot(taken && L.live && !small);
@@ -885,116 +1112,10 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         if (!filter) kz_ahead_indices(nx, f);
 
         // --- resolve the queue 64 fragments at a time --------------------
-        for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-            const uint32_t qi = c0 + lane;
-            const bool active = qi < total;
-            const uint32_t e = active ? s_frag[qi] : 0u;
-            const uint32_t pix = e >> 10, seq = (e >> 4) & 63u, mask = e & 15u;
-            // the owning triangle's data, from lane `seq`
-            const int32_t x0 = kz_bperm_i(L.t.x0, seq), y0 = kz_bperm_i(L.t.y0, seq);
-            const int32_t d1x = kz_bperm_i(L.t.d1x, seq), d1y = kz_bperm_i(L.t.d1y, seq);
-            const int32_t d2x = kz_bperm_i(L.t.d2x, seq), d2y = kz_bperm_i(L.t.d2y, seq);
-            const int64_t S =
-                (int64_t)(((uint64_t)(uint32_t)kz_bperm_i((int32_t)(uint32_t)(uint64_t)L.t.S, seq)) |
-                          ((uint64_t)(uint32_t)kz_bperm_i((int32_t)((uint64_t)L.t.S >> 32), seq) << 32));
-            const float invS = kz_bperm_f(L.t.invS, seq);
-            const float z0 = kz_bperm_f(L.t.z0, seq), dz1 = kz_bperm_f(L.t.dz1, seq), dz2 = kz_bperm_f(L.t.dz2, seq);
-            const int32_t px = X0 + (int32_t)(pix & (KZ_TS - 1)), py = Y0 + (int32_t)(pix >> KZ_TILE_SHIFT);
-
-            float zs[KZ_SAMPLES];
-            uint32_t nf = 0;
-#pragma unroll
-            for (int s = 0; s < KZ_SAMPLES; ++s) {
-                zs[s] = 0.0f;
-                if ((mask >> s) & 1u) {
-                    ++nf;
-                    int64_t e01, e20, e12;
-                    kz_edges(x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + kz_sample_x[s], py * 16 + kz_sample_y[s], e01,
-                             e20, e12);
-                    const float l1 = (float)e20 * invS, l2 = (float)e01 * invS;
-                    const float z = (z0 + l1 * dz1) + l2 * dz2;
-                    zs[s] = fminf(fmaxf(z, f.z_lo), f.z_hi);
-                }
-            }
-            nfrag += nf;
-
-            // depth test + write, in primitive order per pixel
-            uint32_t pass = 0;
-#if KZ_DEPTH_TEST
-            bool act = active && mask != 0;
-            while (kz_ballot(act)) {
-                if (act) atomicMin(&s_claim[pix], seq);
-                __syncthreads();
-                const bool win = act && s_claim[pix] == seq;
-                if (win) {
-#pragma unroll
-                    for (int s = 0; s < KZ_SAMPLES; ++s) {
-                        if ((mask >> s) & 1u) {
-                            float &dd = s_depth[pix * KZ_SAMPLES + s];
-                            if (kz_depth_cmp(zs[s], dd)) {
-                                pass |= 1u << s;
-#if KZ_DEPTH_WRITE
-                                dd = zs[s];
-#endif
-                            }
-                        }
-                    }
-                    s_claim[pix] = KZ_NO_CLAIM;
-                    act = false;
-                }
-                __syncthreads();
-            }
-#else
-            pass = active ? mask : 0u;
-#endif
-            npass += (uint32_t)__popc(pass);
-
-            // varyings at the pixel centre and the fragment shader
-            float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            {
-                float a0[KZ_NVA], da1[KZ_NVA], da2[KZ_NVA];
-#pragma unroll
-                for (int i = 0; i < KZ_NV; ++i) {
-                    a0[i] = kz_bperm_f(L.a0[i], seq);
-                    da1[i] = kz_bperm_f(L.da1[i], seq);
-                    da2[i] = kz_bperm_f(L.da2[i], seq);
-                }
-                const float iw0 = kz_bperm_f(L.t.iw0, seq), iw1 = kz_bperm_f(L.t.iw1, seq),
-                            iw2 = kz_bperm_f(L.t.iw2, seq);
-                if (pass) {
-                    int64_t e01, e20, e12;
-                    kz_edges(x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + 8, py * 16 + 8, e01, e20, e12);
-                    const float l0 = (float)e12 * invS, l1 = (float)e20 * invS, l2 = (float)e01 * invS;
-                    const float q = ((l0 * iw0) + (l1 * iw1)) + (l2 * iw2);
-                    const float rq = 1.0f / q;
-                    const float b1 = (l1 * iw1) * rq, b2 = (l2 * iw2) * rq;
-                    float vary[KZ_NVA];
-#pragma unroll
-                    for (int i = 0; i < KZ_NV; ++i) vary[i] = (a0[i] + b1 * da1[i]) + b2 * da2[i];
-                    const float zc = fminf(fmaxf((z0 + l1 * dz1) + l2 * dz2, f.z_lo), f.z_hi);
-                    const float frag_coord[4] = {(float)px + 0.5f, (float)py + 0.5f, zc, q};
-                    kz_fragment(f, vary, frag_coord, col);
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) col[c] = kz_clamp01(col[c]);
-                }
-            }
-
-            // colour: blend + write, in primitive order per pixel
-            bool act2 = pass != 0;
-            while (kz_ballot(act2)) {
-                if (act2) atomicMin(&s_claim[pix], seq);
-                __syncthreads();
-                const bool win = act2 && s_claim[pix] == seq;
-                if (win) {
-#pragma unroll
-                    for (int s = 0; s < KZ_SAMPLES; ++s)
-                        if ((pass >> s) & 1u) s_color[pix * KZ_SAMPLES + s] = kz_rop(s_color[pix * KZ_SAMPLES + s], col);
-                    s_claim[pix] = KZ_NO_CLAIM;
-                    act2 = false;
-                }
-                __syncthreads();
-            }
-        }
+        if (narrow)
+            kz_resolve<true>(f, L, X0, Y0, total, lane, s_color, s_depth, s_claim, s_frag, nfrag, npass);
+        else
+            kz_resolve<false>(f, L, X0, Y0, total, lane, s_color, s_depth, s_claim, s_frag, nfrag, npass);
         k += take;
         __syncthreads();
     }
@@ -1037,7 +1158,8 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFram
     const uint32_t tile = kz_xcd_remap(blockIdx.x, gridDim.x);
     const int32_t X0 = (int32_t)((tile % f.tiles_x) << KZ_TILE_SHIFT);
     const int32_t Y0 = (int32_t)((tile / f.tiles_x) << KZ_TILE_SHIFT);
-    const uint32_t off = f.tile_offsets[tile], n = f.tile_counts[tile];
+    uint32_t off, n;
+    kz_tile_list(f, tile, off, n);
     const uint64_t t0 = f.tile_cycles ? clock64() : 0;
     unsigned long long frags = 0;
     // entries past list_capacity were dropped by the scatter: such a tile
@@ -1075,7 +1197,8 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persis
         const uint32_t RS = KZ_TILE_SHIFT - level;
         const int32_t X0 = (int32_t)(((tile % f.tiles_x) << KZ_TILE_SHIFT) + ((sub & ((1u << level) - 1)) << RS));
         const int32_t Y0 = (int32_t)(((tile / f.tiles_x) << KZ_TILE_SHIFT) + ((sub >> level) << RS));
-        const uint32_t toff = f.tile_offsets[tile], tn = f.tile_counts[tile];
+        uint32_t toff, tn;
+        kz_tile_list(f, tile, toff, tn);
         if (level == 0) {
             if ((uint64_t)toff + tn <= (uint64_t)f.list_capacity)
                 kz_raster_rect(f, X0, Y0, RS, f.tile_lists + toff, tn, false, s_color, s_depth, s_claim,

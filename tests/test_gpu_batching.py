@@ -59,9 +59,9 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     assert ml.value in (0, 1, 2) and fs.value in (1, 4, 16)
     # timed trials: tile sizes 8..64, two grid sizes, the plan without the
     # model and the plain raster; when a planned raster is fastest, two split
-    # depths and two fragment weights (with the model); then the choice, a
-    # capture and graph launches
-    trials = 8 + (4 if pr.value else 0)
+    # depths, two fragment weights (with the model) and the two other grid
+    # sizes; then the choice, a capture and graph launches
+    trials = 8 + (6 if pr.value else 0)
     captures, launches = _graph_stats(ctx)
     assert captures == 1 and launches == frames - trials - 1, (captures, launches, trials)
 

@@ -73,6 +73,20 @@ def test_grid_small_textured_blended(mctx, oracle):
     check_parity(mctx, oracle, g.jittered_grid(64, 128, 320, 240, 64))
 
 
+@pytest.mark.parametrize("tex_size", [64, 48])
+@pytest.mark.parametrize("filt,addr", [(0, 0), (1, 0), (0, 2), (1, 2)])
+def test_texture_filter_and_addressing(mctx, oracle, tex_size, filt, addr):
+    """NEAREST (0) and LINEAR (1) filtering with REPEAT (0) and
+    CLAMP_TO_EDGE (2) addressing, on a power-of-two texture and another
+    size, with texture coordinates in [-1.5, 2.5] so that both wrap."""
+    s = g.jittered_grid(48, 64, 200, 150, tex_size)
+    s.vertices[:, 3:5] = s.vertices[:, 3:5] * np.float32(4.0) - np.float32(1.5)
+    s.tex_filter = filt
+    s.tex_addr = (addr, addr)
+    s.name += "_f%d_a%d" % (filt, addr)
+    check_parity(mctx, oracle, s)
+
+
 def test_grid_1080p_full(mctx, oracle):
     st, ost, n_off = check_parity(mctx, oracle, g.jittered_grid(512, 1024, 1920, 1080, 1024))
     assert ost["fragments"] > 2_000_000
