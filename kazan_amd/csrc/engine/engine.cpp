@@ -83,6 +83,11 @@ bool Engine::init(int hip_device, std::string &err) {
     if (const char *e = std::getenv("KAZAN_GRAPHS")) graphs_ = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("KAZAN_SETUP_RECORDS")) setup_records_ = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("KAZAN_BINNING")) radix_ = std::strcmp(e, "sort") != 0;
+    if (const char *e = std::getenv("KAZAN_GRID_DIVISOR")) {
+        const int d = std::atoi(e);
+        default_grid_div_ = d == 2 || d == 4 ? (uint32_t)d : 1u;
+    }
+    if (const char *e = std::getenv("KAZAN_SPLIT_LEVEL")) default_max_level_ = (uint32_t)std::min(std::max(std::atoi(e), 0), 2);
     if (radix_ &&
         ((e = hipMalloc((void **)&bin_ctrl_, KZ_BIN_WORDS * sizeof(uint32_t))) != hipSuccess ||
          (e = hipMalloc((void **)&radix_totals_, kz_radix_digits_max() * sizeof(uint32_t))) != hipSuccess)) {
@@ -505,10 +510,10 @@ hipError_t Engine::regrow(T *&p, uint64_t count) {
 BatchConfig Engine::default_config(const Op &op) const {
     BatchConfig c;
     c.shift = op.tile_shift;
-    c.grid_div = 1;
+    c.grid_div = default_grid_div_;
     c.model = true;
     c.persistent = balanced_;
-    c.max_level = 2;
+    c.max_level = default_max_level_;
     c.frag_scale = 1;
     return c;
 }

@@ -260,6 +260,9 @@ class Engine {
     bool kernel_timing_ = false;
     bool balanced_ = true;
     bool autotune_ = true, graphs_ = true;
+    // the untuned configuration's grid divisor and deepest split
+    // (KAZAN_GRID_DIVISOR, KAZAN_SPLIT_LEVEL)
+    uint32_t default_grid_div_ = 1, default_max_level_ = 2;
     uint64_t generation_ = 1;  // bumped whenever captured graphs go stale
     uint32_t graph_launches_ = 0, graph_captures_ = 0;
     BatchConfig last_tuned_{0, 0, false, false, 0, 0};
