@@ -124,7 +124,13 @@ struct KzFrame {
     float frag_weight;
     uint32_t raster_slots;      // workgroups of the persistent raster (split threshold)
     uint32_t split_max_level;   // 0..2: deepest split the plan may choose
-    uint32_t pad2;
+    // The render pass's clears folded into this batch (bit 0: colour, bit
+    // 1: depth; recorded just before it, over the whole framebuffer): the
+    // raster starts every tile from these values instead of loading it, and
+    // writes every tile, also those without triangles.
+    uint32_t clear_mask;
+    uint32_t clear_color;       // packed RGBA8
+    float clear_depth;
 };
 
 // Per-tile pair statistics (tile_acc) over the (triangle, tile) pairs; the

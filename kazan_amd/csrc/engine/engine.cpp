@@ -535,9 +535,10 @@ int fastest(const BatchTuning &t) {
 // and none (many small triangles are set up once per sub-tile they touch);
 // with a cost model, the model's plan with its fragment weight times 4 and
 // 16 (a pipeline whose fragments cost more than the training pipelines');
-// at the fastest planned one, the other two grid sizes again (the grid was
-// chosen with the deepest splits; with fewer, a larger grid may pay). The
-// fastest candidate is kept.
+// at the fastest planned one, the other two grid sizes and the other tile
+// sizes again (they were chosen with the deepest splits and the model; with
+// fewer splits a larger grid or smaller tiles may pay). The fastest
+// candidate is kept.
 hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     cfg = default_config(op);
     timed = false;
@@ -590,6 +591,9 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
             } else if (b.persistent) {
                 for (uint32_t d : {1u, 2u, 4u})
                     if (d != b.grid_div) t.cand.push_back({b.shift, d, b.model, true, b.max_level, b.frag_scale});
+                for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s)
+                    if (s != b.shift && op.variants->fits(s))
+                        t.cand.push_back({s, b.grid_div, b.model, true, b.max_level, b.frag_scale});
             }
             t.ms.resize(t.cand.size(), -1.0f);
         }

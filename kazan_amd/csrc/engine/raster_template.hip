@@ -894,7 +894,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
                                              const uint32_t *ids, uint32_t n, bool filter, uint32_t *s_color,
                                              float *s_depth, uint32_t *s_claim, uint32_t *s_frag) {
     const uint32_t lane = kz_lane();
-    if (n == 0) return 0;
+    if (n == 0 && !f.clear_mask) return 0;  // (with folded clears the square is written regardless)
     const int32_t RW = 1 << RS;
     const uint32_t *gcolor = (const uint32_t *)f.color;
 
@@ -909,8 +909,8 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
 #pragma unroll
             for (int s = 0; s < KZ_SAMPLES; ++s) {
-                s_color[i * KZ_SAMPLES + s] = gcolor[gp + s];
-                s_depth[i * KZ_SAMPLES + s] = f.depth[gp + s];
+                s_color[i * KZ_SAMPLES + s] = (f.clear_mask & 1u) ? f.clear_color : gcolor[gp + s];
+                s_depth[i * KZ_SAMPLES + s] = (f.clear_mask & 2u) ? f.clear_depth : f.depth[gp + s];
             }
         }
         s_claim[i] = KZ_NO_CLAIM;
@@ -1164,8 +1164,10 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFram
     unsigned long long frags = 0;
     // entries past list_capacity were dropped by the scatter: such a tile
     // takes its triangles from an in-order scan of all pixel boxes
-    if (n == 0)
-        frags = 0;
+    if (n == 0)  // nothing to draw; with folded clears the tile is still written
+        frags = f.clear_mask ? kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, nullptr, 0, false, s_color, s_depth, s_claim,
+                                              s_frag)
+                             : 0;
     else if ((uint64_t)off + n <= (uint64_t)f.list_capacity)
         frags = kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, f.tile_lists + off, n, false, s_color, s_depth, s_claim,
                                s_frag);
@@ -1182,7 +1184,9 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFram
 
 // Balanced mode: persistent workgroups take work items (tiles or
 // sub-tiles) from the queue, most expensive first: item b in the first
-// round, then the next untaken item from a counter.
+// round, then the next untaken item from a counter. With folded clears,
+// the tiles without triangles (they have no items) follow, strided over
+// the workgroups. One call site of kz_raster_rect serves every case.
 extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persistent(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
@@ -1190,35 +1194,56 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persis
     __shared__ uint32_t s_frag[KZ_FRAG_CAP];
     const uint32_t lane = kz_lane();
     const uint32_t n_items = f.ctrl[KZ_CTRL_ITEMS];
+    const uint32_t ntiles = f.tiles_x * f.tiles_y;
     uint32_t q = kz_xcd_remap(blockIdx.x, gridDim.x);  // first round: one item per workgroup
-    while (q < n_items) {
-        const uint32_t geo = f.queue[q];
-        const uint32_t tile = KZ_ITEM_TILE(geo), level = KZ_ITEM_LEVEL(geo), sub = KZ_ITEM_SUB(geo);
-        const uint32_t RS = KZ_TILE_SHIFT - level;
-        const int32_t X0 = (int32_t)(((tile % f.tiles_x) << KZ_TILE_SHIFT) + ((sub & ((1u << level) - 1)) << RS));
-        const int32_t Y0 = (int32_t)(((tile / f.tiles_x) << KZ_TILE_SHIFT) + ((sub >> level) << RS));
-        uint32_t toff, tn;
-        kz_tile_list(f, tile, toff, tn);
-        if (level == 0) {
-            if ((uint64_t)toff + tn <= (uint64_t)f.list_capacity)
-                kz_raster_rect(f, X0, Y0, RS, f.tile_lists + toff, tn, false, s_color, s_depth, s_claim,
-                               s_frag);
-            else
-                kz_raster_rect(f, X0, Y0, RS, nullptr, f.total_tris, true, s_color, s_depth, s_claim, s_frag);
+    uint32_t e = blockIdx.x;                           // next tile to check for emptiness
+    for (;;) {
+        int32_t X0, Y0;
+        uint32_t RS, n;
+        const uint32_t *ids;
+        bool filter = false;
+        const bool item = q < n_items;
+        if (item) {
+            const uint32_t geo = f.queue[q];
+            const uint32_t tile = KZ_ITEM_TILE(geo), level = KZ_ITEM_LEVEL(geo), sub = KZ_ITEM_SUB(geo);
+            RS = KZ_TILE_SHIFT - level;
+            X0 = (int32_t)(((tile % f.tiles_x) << KZ_TILE_SHIFT) + ((sub & ((1u << level) - 1)) << RS));
+            Y0 = (int32_t)(((tile / f.tiles_x) << KZ_TILE_SHIFT) + ((sub >> level) << RS));
+            uint32_t toff, tn;
+            kz_tile_list(f, tile, toff, tn);
+            if (level == 0) {
+                filter = (uint64_t)toff + tn > (uint64_t)f.list_capacity;
+                ids = filter ? nullptr : f.tile_lists + toff;
+                n = filter ? f.total_tris : tn;
+            } else {
+                // a split tile's list always fits; its sub-tile lists may
+                // not, and then the sub-tile filters the tile's list
+                const uint32_t i = f.split_slot[tile] * 16 + sub;
+                const uint32_t soff = f.split_offsets[i], sn = f.split_counts[i];
+                filter = (uint64_t)soff + sn > (uint64_t)f.split_capacity;
+                ids = filter ? f.tile_lists + toff : f.split_lists + soff;
+                n = filter ? tn : sn;
+            }
         } else {
-            // a split tile's list always fits; its sub-tile lists may not,
-            // and then the sub-tile filters the tile's list
-            const uint32_t i = f.split_slot[tile] * 16 + sub;
-            const uint32_t soff = f.split_offsets[i], sn = f.split_counts[i];
-            if ((uint64_t)soff + sn <= (uint64_t)f.split_capacity)
-                kz_raster_rect(f, X0, Y0, RS, f.split_lists + soff, sn, false, s_color, s_depth, s_claim,
-                               s_frag);
-            else
-                kz_raster_rect(f, X0, Y0, RS, f.tile_lists + toff, tn, true, s_color, s_depth, s_claim,
-                               s_frag);
+            if (!f.clear_mask) break;
+            uint32_t off = 0, cnt = 1;
+            for (; e < ntiles; e += gridDim.x) {
+                kz_tile_list(f, e, off, cnt);
+                if (cnt == 0) break;
+            }
+            if (e >= ntiles) break;
+            RS = KZ_TILE_SHIFT;
+            X0 = (int32_t)((e % f.tiles_x) << KZ_TILE_SHIFT);
+            Y0 = (int32_t)((e / f.tiles_x) << KZ_TILE_SHIFT);
+            ids = nullptr;
+            n = 0;
+            e += gridDim.x;
         }
-        uint32_t next = 0;
-        if (lane == 0) next = atomicAdd(&f.ctrl[KZ_CTRL_HEAD], 1u);
-        q = gridDim.x + (uint32_t)kz_readlane_i((int32_t)next, 0);
+        kz_raster_rect(f, X0, Y0, RS, ids, n, filter, s_color, s_depth, s_claim, s_frag);
+        if (item) {
+            uint32_t next = 0;
+            if (lane == 0) next = atomicAdd(&f.ctrl[KZ_CTRL_HEAD], 1u);
+            q = gridDim.x + (uint32_t)kz_readlane_i((int32_t)next, 0);
+        }
     }
 }

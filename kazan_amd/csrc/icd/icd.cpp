@@ -1129,6 +1129,27 @@ static void record_draw(CommandBuffer *cb, uint32_t count, uint32_t instances, u
             batch = &last;
     }
     if (!batch) {
+        // The render pass's clears of this batch's colour and depth, recorded
+        // just before it over the whole framebuffer, fold into the batch: its
+        // raster starts every tile from the clear values instead of a fill
+        // kernel and a load.
+        while (!cb->ops.empty() && cb->ops.back().kind == Op::Fill) {
+            const Op &o = cb->ops.back();
+            const bool whole = o.x0 == 0 && o.y0 == 0 && o.w == f.width && o.h == f.height &&
+                               o.fill_width == f.width && o.fill_spp == fx.samples;
+            const bool col = whole && o.fill_ptr == (const uint32_t *)f.color && !(f.clear_mask & 1u);
+            const bool dep = whole && o.fill_ptr == (const uint32_t *)f.depth && !(f.clear_mask & 2u);
+            if (col) {
+                f.clear_mask |= 1u;
+                f.clear_color = o.fill_value;
+            } else if (dep) {
+                f.clear_mask |= 2u;
+                std::memcpy(&f.clear_depth, &o.fill_value, 4);
+            } else {
+                break;
+            }
+            cb->ops.pop_back();
+        }
         Op op;
         op.kind = Op::Batch;
         op.variants = &p->variants;
