@@ -17,7 +17,7 @@ split by binned counts only) and "plain" (one workgroup per tile). The first
 form is the reported value; config.forms holds all of them and
 config.<form>_vs_plain the throughput ratios.
 
-  python bench.py [--gpus 1] [--steps 50] [--warmup 24] [--workload grid_1080p]
+  python bench.py [--gpus 1] [--steps 50] [--warmup 30] [--workload grid_1080p]
                   [--forms learned,balanced_static,plain]
   python bench.py --copy-probe 1073741824   # HBM counter calibration only
 """
@@ -142,7 +142,7 @@ def main():
     # (one timed candidate each), runs the choice eagerly on the next and
     # captures the command buffer into a hipGraph on the one after: warm up
     # past that
-    ap.add_argument("--warmup", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="grid_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

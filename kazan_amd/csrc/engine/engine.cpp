@@ -535,10 +535,10 @@ int fastest(const BatchTuning &t) {
 // and none (many small triangles are set up once per sub-tile they touch);
 // with a cost model, the model's plan with its fragment weight times 4 and
 // 16 (a pipeline whose fragments cost more than the training pipelines');
-// at the fastest planned one, the other two grid sizes and the other tile
-// sizes again (they were chosen with the deepest splits and the model; with
-// fewer splits a larger grid or smaller tiles may pay). The fastest
-// candidate is kept.
+// at the fastest planned one, every other pair of tile size and grid size
+// (both were chosen with the deepest splits and the model; with fewer
+// splits smaller tiles on a larger grid may pay). The fastest candidate is
+// kept.
 hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     cfg = default_config(op);
     timed = false;
@@ -589,11 +589,10 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
                     t.cand.push_back({b.shift, b.grid_div, true, true, 2u, 16u});
                 }
             } else if (b.persistent) {
-                for (uint32_t d : {1u, 2u, 4u})
-                    if (d != b.grid_div) t.cand.push_back({b.shift, d, b.model, true, b.max_level, b.frag_scale});
                 for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s)
-                    if (s != b.shift && op.variants->fits(s))
-                        t.cand.push_back({s, b.grid_div, b.model, true, b.max_level, b.frag_scale});
+                    for (uint32_t d : {1u, 2u, 4u})
+                        if ((s != b.shift || d != b.grid_div) && op.variants->fits(s))
+                            t.cand.push_back({s, d, b.model, true, b.max_level, b.frag_scale});
             }
             t.ms.resize(t.cand.size(), -1.0f);
         }

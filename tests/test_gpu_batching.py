@@ -45,7 +45,7 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     oc, od, _, ost = oracle.render(s)
     r = SceneRenderer(ctx, s)
     assert ctx.d.vkHasCostModelKAZAN(ctx.device) == 1  # the shipped model
-    frames = 22
+    frames = 28
     for frame in range(frames):
         print("frame %d" % frame, flush=True)
         st = r.render(1)
@@ -59,10 +59,10 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     assert ml.value in (0, 1, 2) and fs.value in (1, 4, 16)
     # timed trials: the 4 tile sizes 8..64, two grid sizes, the plan without
     # the model and the plain raster; when a planned raster is fastest, two
-    # split depths, two fragment weights (with the model), the two other grid
-    # sizes and the 3 other tile sizes; then the choice, a capture and graph
+    # split depths, two fragment weights (with the model) and the 11 other
+    # pairs of tile size and grid size; then the choice, a capture and graph
     # launches
-    trials = 8 + (9 if pr.value else 0)
+    trials = 8 + (15 if pr.value else 0)
     captures, launches = _graph_stats(ctx)
     assert captures == 1 and launches == frames - trials - 1, (captures, launches, trials)
 
