@@ -106,7 +106,7 @@ struct BatchTuning {
     ~BatchTuning();
     bool done = false;
     int phase = 0;  // 0: tile sizes, 1: grid sizes, 2: plan and raster forms, 3: split depth, 4: fragment weight,
-                    // 5: every tile size x grid size again
+                    // 5: every tile size x grid size again, 6: the three fastest again
     std::vector<BatchConfig> cand;
     std::vector<float> ms;           // per candidate, -1 = not timed
     uint32_t next = 0;               // next candidate to try

@@ -57,7 +57,7 @@ hipError_t kz_launch_gen_cluster(hipStream_t st, float *vertices, uint32_t n, ui
                                  uint32_t height, double sigma_frac, double radius, uint64_t seed);
 hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntiles);
 // Balanced mode: split levels, split slots and the ordered item queue
-// (3 launches over one workgroup per 1,024 tiles; `table` holds
+// (3 launches over one workgroup per 256 tiles; `table` holds
 // kz_plan_table_words(ntiles) words of per-workgroup counts), then the
 // sub-tile lists of split tiles on `grid` workgroups.
 uint32_t kz_plan_table_words(uint32_t ntiles);
