@@ -633,13 +633,17 @@ __device__ __forceinline__ void kz_sample(const KzTexture &t, float u, float v, 
     const uint32_t *row_a = (const uint32_t *)(t.data + (size_t)ya * t.w * 4);
     const uint32_t *row_b = (const uint32_t *)(t.data + (size_t)yb * t.w * 4);
     const uint32_t t00 = row_a[xa], t10 = row_a[xb], t01 = row_b[xa], t11 = row_b[xb];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const uint32_t sh = 8 * c;
-        const uint32_t top = ((t00 >> sh) & 255u) * (16u - ax) + ((t10 >> sh) & 255u) * ax;
-        const uint32_t bot = ((t01 >> sh) & 255u) * (16u - ax) + ((t11 >> sh) & 255u) * ax;
-        out[c] = (float)(top * (16u - ay) + bot * ay) * (1.0f / 65280.0f);
-    }
+    // two channels per 32-bit lerp: R and B (then G and A) in 16-bit fields;
+    // the weights of each lerp sum to 16, so a field stays <= 255 * 16 * 16
+    // = 65280 and never carries into the next (the same integers per channel)
+    const uint32_t m = 0x00FF00FFu, wx = 16u - ax, wy = 16u - ay;
+    const uint32_t rb = ((t00 & m) * wx + (t10 & m) * ax) * wy + ((t01 & m) * wx + (t11 & m) * ax) * ay;
+    const uint32_t ga = (((t00 >> 8) & m) * wx + ((t10 >> 8) & m) * ax) * wy +
+                        (((t01 >> 8) & m) * wx + ((t11 >> 8) & m) * ax) * ay;
+    out[0] = (float)(rb & 0xFFFFu) * (1.0f / 65280.0f);
+    out[1] = (float)(ga & 0xFFFFu) * (1.0f / 65280.0f);
+    out[2] = (float)(rb >> 16) * (1.0f / 65280.0f);
+    out[3] = (float)(ga >> 16) * (1.0f / 65280.0f);
 }
 
 __device__ __forceinline__ float kz_clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
@@ -718,6 +722,14 @@ __device__ __forceinline__ uint32_t kz_rop(uint32_t dst, const float src[4]) {
 // ---------------------------------------------------------------------------
 // the tile rasteriser
 
+// Sample masks of a small triangle's candidate pixels (KZ_SAMPLES bits for
+// each of at most KZ_SMALL_CAND pixels)
+#if KZ_SAMPLES * KZ_SMALL_CAND <= 32
+typedef uint32_t KzCandMask;
+#else
+typedef uint64_t KzCandMask;
+#endif
+
 struct KzLaneTri {
     KzTri t;
     KzCov c;
@@ -791,6 +803,7 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
         const int32_t px = X0 + (int32_t)(pix & (KZ_TS - 1)), py = Y0 + (int32_t)(pix >> KZ_TILE_SHIFT);
 
         float zs[KZ_SAMPLES];
+        float c01 = 0.0f, c20 = 0.0f, c12 = 0.0f;  // one sample per pixel: it is the pixel centre
         uint32_t nf = 0;
 #pragma unroll
         for (int s = 0; s < KZ_SAMPLES; ++s) {
@@ -800,6 +813,11 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
                 float e01, e20, e12;
                 kz_edges_f(NARROW, x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + kz_sample_x[s],
                            py * 16 + kz_sample_y[s], e01, e20, e12);
+                if (KZ_SAMPLES == 1) {
+                    c01 = e01;
+                    c20 = e20;
+                    c12 = e12;
+                }
                 const float l1 = e20 * invS, l2 = e01 * invS;
                 const float z = (z0 + l1 * dz1) + l2 * dz2;
                 zs[s] = fminf(fmaxf(z, f.z_lo), f.z_hi);
@@ -836,8 +854,9 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
             const float iw0 = kz_bperm_f(L.t.iw0, seq), iw1 = kz_bperm_f(L.t.iw1, seq),
                         iw2 = kz_bperm_f(L.t.iw2, seq);
             if (maybe) {
-                float e01, e20, e12;
-                kz_edges_f(NARROW, x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + 8, py * 16 + 8, e01, e20, e12);
+                float e01 = c01, e20 = c20, e12 = c12;
+                if (KZ_SAMPLES != 1)
+                    kz_edges_f(NARROW, x0, y0, d1x, d1y, d2x, d2y, S, px * 16 + 8, py * 16 + 8, e01, e20, e12);
                 const float l0 = e12 * invS, l1 = e20 * invS, l2 = e01 * invS;
                 const float q = ((l0 * iw0) + (l1 * iw1)) + (l2 * iw2);
                 const float rq = 1.0f / q;
@@ -996,7 +1015,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         // (small triangles keep their sample masks, KZ_SAMPLES bits per
         // candidate pixel, for phase 2: at most 16 x 4 bits)
         uint32_t cnt = 0;
-        uint64_t cmask = 0;
+        KzCandMask cmask = 0;
         {
             const int32_t maxc = kz_wave_max(small ? cand : 0, lane);
             KzWalk wk;
@@ -1010,7 +1029,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
                     dx = wrap ? 0 : dx + 1;
                     if (m) {
                         ++cnt;
-                        cmask |= (uint64_t)m << (j * KZ_SAMPLES);
+                        cmask |= (KzCandMask)m << (j * KZ_SAMPLES);
                     }
                 }
             }
@@ -1061,11 +1080,11 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             // the reciprocal's rounding
             const float rcw = __builtin_amdgcn_rcpf((float)cw);
             uint32_t w = excl;
-            uint64_t bits = cmask;
+            KzCandMask bits = cmask;
             while (bits) {
-                const uint32_t j = (uint32_t)__builtin_ctzll(bits) / KZ_SAMPLES;
+                const uint32_t j = (uint32_t)__builtin_ctzll((uint64_t)bits) / KZ_SAMPLES;
                 const uint32_t m = (uint32_t)(bits >> (j * KZ_SAMPLES)) & ((1u << KZ_SAMPLES) - 1u);
-                bits &= ~((uint64_t)((1u << KZ_SAMPLES) - 1u) << (j * KZ_SAMPLES));
+                bits &= ~((KzCandMask)((1u << KZ_SAMPLES) - 1u) << (j * KZ_SAMPLES));
                 const int32_t dy = (int32_t)(((float)j + 0.5f) * rcw);
                 const int32_t cx = L.cx0 + (int32_t)j - dy * cw, cy = L.cy0 + dy;
                 s_frag[w++] = kz_pack_frag((uint32_t)((cy - Y0) * KZ_TS + (cx - X0)), lane, m);
