@@ -52,6 +52,7 @@ class Scene:
         self.cull_mode, self.front_face = 0, 0
         self.viewport = (0.0, 0.0, float(width), float(height), 0.0, 1.0)
         self.scissor = (0, 0, width, height)
+        self.render_area = None  # (x, y, w, h) of vkCmdBeginRenderPass; None: the whole framebuffer
         self.texture = None  # (H, W, 4) uint8
         self.tex_filter = 1  # LINEAR
         self.tex_addr = (2, 2)  # CLAMP_TO_EDGE

@@ -339,8 +339,9 @@ class SceneRenderer:
         for k in range(4):
             clears[0].color.float32[k] = s.clear_color[k]
         clears[1].depthStencil.depth = s.clear_depth
+        ra = s.render_area or (0, 0, s.width, s.height)
         rbi = vk.VkRenderPassBeginInfo(vk.ST_RENDER_PASS_BEGIN_INFO, None, self.rp, self.fb,
-                                       vk.VkRect2D(vk.VkOffset2D(0, 0), vk.VkExtent2D(s.width, s.height)), 2, clears)
+                                       vk.VkRect2D(vk.VkOffset2D(ra[0], ra[1]), vk.VkExtent2D(ra[2], ra[3])), 2, clears)
         d.vkCmdBeginRenderPass(cb, _ref(rbi), 0)
         d.vkCmdBindPipeline(cb, 0, self.pipeline)
         d.vkCmdBindVertexBuffers(cb, 0, 1, (vk.Handle * 1)(self.vbuf[0]), (ctypes.c_uint64 * 1)(0))

@@ -191,6 +191,35 @@ def test_depth_greater_and_scissor(mctx, oracle):
     check_parity(mctx, oracle, s)
 
 
+def test_partial_render_area_keeps_the_rest(mctx, oracle):
+    """A render pass over part of the framebuffer clears and draws only
+    there (its clear is a fill of the area, not folded into the raster,
+    which then loads the tiles); the pixels outside keep the previous
+    frame's values."""
+    import copy
+    from kazan_amd.scenes.renderer import SceneRenderer
+    s = g.random_triangles(300, 96, 64, seed=11)
+    r = SceneRenderer(mctx, s)
+    r.render(1)
+    c1, d1, _ = r.read_back()
+    oc1, od1, _, _ = oracle.render(s)
+    np.testing.assert_array_equal(d1.view(np.uint32), od1.view(np.uint32))
+    s.clear_color = (0.25, 0.5, 0.75, 1.0)
+    s.render_area = (16, 8, 48, 40)
+    r._record()
+    r.render(1)
+    c2, d2, _ = r.read_back()
+    s2 = copy.copy(s)
+    s2.scissor = (16, 8, 48, 40)  # the area bounds the draws too
+    oc2, od2, _, _ = oracle.render(s2)
+    inside = np.zeros((s.height, s.width), bool)
+    inside[8:48, 16:64] = True
+    np.testing.assert_array_equal(d2[inside].view(np.uint32), od2[inside].view(np.uint32))
+    assert np.abs(c2[inside].astype(np.int16) - oc2[inside].astype(np.int16)).max() <= COLOR_TOL_LSB
+    np.testing.assert_array_equal(c2[~inside], c1[~inside])
+    np.testing.assert_array_equal(d2[~inside].view(np.uint32), d1[~inside].view(np.uint32))
+
+
 def test_msaa4_triangles(mctx, oracle):
     s = g.random_triangles(400, 128, 96, seed=7)
     s.samples = 4
