@@ -102,7 +102,8 @@ struct KzFrame {
     uint32_t *split_slot;       // [tiles] split slot of a split tile
     uint32_t *split_info;       // [2 * split slots] (first chunk, tile) of a slot
     uint32_t *chunk_tile;       // [max_chunks] tile of each KZ_SPLIT_CHUNK-entry chunk
-    uint32_t *chunk_counts;     // [chunks * 16] entries per sub-tile, then write bases
+    uint32_t *chunk_counts;     // [chunks * 16] entries per sub-tile, then bases within the slot's lists
+    uint16_t *split_mask;       // [max_chunks * KZ_SPLIT_CHUNK] sub-tiles each chunk entry overlaps
     uint32_t *split_counts;     // [split slots * 16]
     uint32_t *split_offsets;    // [split slots * 16]
     uint32_t *split_lists;      // [split_capacity]

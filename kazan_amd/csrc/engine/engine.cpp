@@ -53,6 +53,7 @@ Engine::~Engine() {
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
                         chunk_tile_, chunk_counts_, split_lists_, plan_table_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
         hipFree(p);
+    hipFree(split_mask_);
     hipFree(tile_feat_);
     hipFree(tile_pred_);
     hipFree(cost_model_d_);
@@ -217,6 +218,7 @@ hipError_t Engine::ensure_split_scratch(uint32_t tiles) {
                                               0x0FFFFFFFull);
         KZ_TRY(regrow(chunk_tile_, c));
         KZ_TRY(regrow(chunk_counts_, 16 * c));
+        KZ_TRY(regrow(split_mask_, (uint64_t)KZ_SPLIT_CHUNK * c));
         max_chunks_ = (uint32_t)c;
     }
     if (want_split > split_capacity_) {
@@ -355,6 +357,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         f.split_info = split_info_;
         f.chunk_tile = chunk_tile_;
         f.chunk_counts = chunk_counts_;
+        f.split_mask = split_mask_;
         f.split_counts = split_counts_;
         f.split_offsets = split_offsets_;
         f.split_lists = split_lists_;

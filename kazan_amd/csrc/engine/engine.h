@@ -272,6 +272,7 @@ class Engine {
     uint32_t *tile_level_ = nullptr, *split_slot_ = nullptr, *split_info_ = nullptr, *queue_ = nullptr,
              *ctrl_ = nullptr, *split_counts_ = nullptr, *split_offsets_ = nullptr, *chunk_tile_ = nullptr,
              *chunk_counts_ = nullptr, *split_lists_ = nullptr, *plan_table_ = nullptr;
+    uint16_t *split_mask_ = nullptr;  // [max_chunks * KZ_SPLIT_CHUNK]
     uint32_t cap_split_tiles_ = 0, max_chunks_ = 0, split_capacity_ = 0;
     uint32_t split_grid_ = 1024;  // workgroups of the chunk kernels: 4 per CU
     // cost model and tile profile

@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t nti
             f.split_slot[t[j]] = slot;
             f.split_info[2 * slot] = c0;
             f.split_info[2 * slot + 1] = t[j];
-            for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_count adds
+            for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_totals writes
             for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c) f.chunk_tile[c0 + c] = t[j];
         }
     }
@@ -1237,8 +1237,9 @@ __device__ __forceinline__ uint32_t sub_mask(uint64_t r, const Chunk &k) {
     return m;
 }
 
-// Entries per sub-tile of every chunk (one wave per chunk), kept per chunk
-// and added to the sub-tile totals of its split slot (zeroed by kz_plan).
+// Entries per sub-tile of every chunk (one wave per chunk), kept per chunk,
+// and each entry's sub-tile mask (split_mask), so that the scatter reads
+// the masks instead of gathering pixel boxes again.
 constexpr uint32_t kSplitRounds = KZ_SPLIT_CHUNK / 64;
 
 // The sub-tile masks of a chunk's entries (lane l: entries r * 64 + l): all
@@ -1270,15 +1271,59 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
 #pragma unroll
         for (uint32_t r = 0; r < kSplitRounds; ++r) {
             if (r * 64 >= k.n) break;
+            f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + r * 64 + lane] = (uint16_t)m[r];
             for (uint32_t s = 0; s < k.nsub; ++s) {
                 const uint32_t c_s = (uint32_t)__popcll(__ballot((m[r] >> s) & 1u));
                 if (lane == s) cnt += c_s;
             }
         }
-        if (lane < 16) {
-            f.chunk_counts[c * 16 + lane] = cnt;
-            if (cnt) atomicAdd(&f.split_counts[k.slot * 16 + lane], cnt);
+        if (lane < 16) f.chunk_counts[c * 16 + lane] = cnt;
+    }
+}
+
+// Per split slot (one wave each): its 16 sub-tile totals (split_counts),
+// and each of its chunks' counts (chunk_counts) turned into the position of
+// the chunk's first entry within each sub-tile list, 64 chunks at a time
+// (lane l: chunk j0 + l, its 16 counts loaded at once; per sub-tile a scan
+// over the wave; lane s < 16 carries sub-tile s's running total).
+__global__ __launch_bounds__(kSplitThreads) void kz_split_totals(KzFrame f) {
+    const uint32_t lane = lane_id();
+    const uint32_t n_slots = f.ctrl[KZ_CTRL_SPLITS];
+    const uint32_t waves = gridDim.x * (kSplitThreads / 64);
+    for (uint32_t i = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); i < n_slots; i += waves) {
+        const uint32_t c0 = f.split_info[2 * i], t = f.split_info[2 * i + 1];
+        const uint32_t nch = (f.tile_counts[t] + KZ_SPLIT_CHUNK - 1) / KZ_SPLIT_CHUNK;
+        uint32_t run = 0;
+        for (uint32_t j0 = 0; j0 < nch; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            uint4 *row = (uint4 *)(f.chunk_counts + (size_t)(c0 + (j < nch ? j : 0)) * 16);
+            uint32_t v[16];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint4 w = j < nch ? row[q] : make_uint4(0u, 0u, 0u, 0u);
+                v[4 * q] = w.x;
+                v[4 * q + 1] = w.y;
+                v[4 * q + 2] = w.z;
+                v[4 * q + 3] = w.w;
+            }
+#pragma unroll
+            for (uint32_t s16 = 0; s16 < 16; ++s16) {
+                uint32_t y = v[s16];
+                for (uint32_t d = 1; d < 64; d <<= 1) {
+                    const uint32_t o = __shfl_up(y, d, 64);
+                    if (lane >= d) y += o;
+                }
+                const uint32_t b = readlane(run, s16);
+                const uint32_t tot = readlane(y, 63);
+                v[s16] = b + y - v[s16];
+                if (lane == s16) run += tot;
+            }
+            if (j < nch) {
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) row[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+            }
         }
+        if (lane < 16) f.split_counts[i * 16 + lane] = run;
     }
 }
 
@@ -1326,21 +1371,16 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
     const uint32_t waves = gridDim.x * (kSplitThreads / 64);
     for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
         const Chunk k = chunk_at(f, c);
-        // lane s < 16: next slot of sub-tile s, after the entries of the
-        // tile's earlier chunks (lanes 16 s'..16 s' + 15 sum every 4th chunk)
-        uint32_t before = 0;
-        for (uint32_t j0 = 0; j0 < k.j; j0 += 16) {
-#pragma unroll
-            for (uint32_t u = 0; u < 16; u += 4) {
-                const uint32_t j = j0 + u + (lane >> 4);
-                if (j < k.j) before += f.chunk_counts[(k.c0 + j) * 16 + (lane & 15)];
-            }
-        }
-        before += __shfl_xor(before, 16, 64);
-        before += __shfl_xor(before, 32, 64);
-        uint32_t base = lane < 16 ? f.split_offsets[k.slot * 16 + lane] + before : 0u;
+        // lane s < 16: next position in sub-tile s's list (the slot's list
+        // offset plus the entries of the tile's earlier chunks)
+        uint32_t base = lane < 16 ? f.split_offsets[k.slot * 16 + lane] + f.chunk_counts[c * 16 + lane] : 0u;
         uint32_t id[kSplitRounds], m[kSplitRounds];
-        chunk_masks(f, k, lane, id, m);
+#pragma unroll
+        for (uint32_t r = 0; r < kSplitRounds; ++r) {
+            const uint32_t e = r * 64 + lane;
+            id[r] = e < k.n ? k.ids[e] : kNone;
+            m[r] = e < k.n ? f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + e] : 0u;
+        }
 #pragma unroll
         for (uint32_t r = 0; r < kSplitRounds; ++r) {
             if (r * 64 >= k.n) break;
@@ -1968,6 +2008,7 @@ hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uin
 
 hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid, uint32_t ntiles) {
     kz_split_count<<<grid, kSplitThreads, 0, st>>>(f);
+    kz_split_totals<<<grid, kSplitThreads, 0, st>>>(f);
     kz_split_offsets<<<(ntiles + kSplitThreads - 1) / kSplitThreads, kSplitThreads, 0, st>>>(f);
     kz_split_scatter<<<grid, kSplitThreads, 0, st>>>(f);
     return hipGetLastError();
