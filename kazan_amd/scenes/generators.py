@@ -217,9 +217,10 @@ def zipf_draws(n_draws=4096, total=4_194_304, width=2560, height=1440, seed=rng.
 
 
 def gaussian_cluster(n_tris=4_194_304, width=1920, height=1080, cluster_frac=0.9, sigma_frac=0.05, radius=0.92,
-                     seed=rng.DEFAULT_SEED):
+                     seed=rng.DEFAULT_SEED, centre=(0.5, 0.5)):
     """Skewed workload: cluster_frac of the triangles centred ~ N(screen
-    centre, sigma = sigma_frac * (W, H)) and the rest ~ U(screen); vertices at
+    centre (or `centre`, fractions of W and H), sigma = sigma_frac * (W, H))
+    and the rest ~ U(screen); vertices at
     per-vertex offsets ~ U[-radius, radius]^2 px from the centre (radius
     0.92 px gives ~30x overdraw at the cluster peak: peak density n_c /
     (2 pi sx sy) triangles per px^2 times the mean area 11/144 * (2r)^2).
@@ -230,8 +231,8 @@ def gaussian_cluster(n_tris=4_194_304, width=1920, height=1080, cluster_frac=0.9
     cy = np.empty(n_tris)
     # Box-Muller for the cluster, uniform for the rest
     r = np.sqrt(-2.0 * np.log(np.maximum(u[:nc, 0], 1e-12)))
-    cx[:nc] = width / 2 + sigma_frac * width * r * np.cos(2 * np.pi * u[:nc, 1])
-    cy[:nc] = height / 2 + sigma_frac * height * r * np.sin(2 * np.pi * u[:nc, 1])
+    cx[:nc] = width * centre[0] + sigma_frac * width * r * np.cos(2 * np.pi * u[:nc, 1])
+    cy[:nc] = height * centre[1] + sigma_frac * height * r * np.sin(2 * np.pi * u[:nc, 1])
     cx[nc:] = u[nc:, 2] * width
     cy[nc:] = u[nc:, 3] * height
     w = rng.uniform_f32(seed + 3, n_tris * 6).reshape(n_tris, 3, 2).astype(np.float64) * 2 - 1
@@ -246,6 +247,21 @@ def gaussian_cluster(n_tris=4_194_304, width=1920, height=1080, cluster_frac=0.9
     v[:, 4:8] = rng.uniform_f32(seed + 6, nv * 4).reshape(nv, 4)
     s = Scene("cluster_%d_%dx%d" % (n_tris, width, height), width, height, VS_POS4_COLOR4, FS_COLOR, v)
     s.blend = (1, BLEND_SRC_ALPHA, BLEND_ONE_MINUS_SRC_ALPHA, 0, BLEND_ONE, BLEND_ONE_MINUS_SRC_ALPHA, 0)
+    return s
+
+
+def mixed_clusters(n_fine=3_000_000, n_coarse=30_000, width=1920, height=1080, seed=rng.DEFAULT_SEED):
+    """Two Gaussian clusters side by side: n_fine triangles of radius 0.92 px
+    (the skewed workload's size, ~0.3 fragments each) around (0.3 W, 0.5 H)
+    and n_coarse of radius 10 px (~30 fragments each) around (0.7 W, 0.5 H).
+    A tile in the fine cluster holds ten times the triangles of one in the
+    coarse cluster but fewer fragments: binned triangle counts misjudge its
+    raster cost, fragment counts do not."""
+    a = gaussian_cluster(n_fine, width, height, 1.0, 0.05, 0.92, seed, (0.3, 0.5))
+    b = gaussian_cluster(n_coarse, width, height, 1.0, 0.04, 10.0, seed + 7, (0.7, 0.5))
+    s = Scene("mixed_%d_%d_%dx%d" % (n_fine, n_coarse, width, height), width, height, VS_POS4_COLOR4, FS_COLOR,
+              np.concatenate([a.vertices, b.vertices]))
+    s.blend = a.blend
     return s
 
 
@@ -284,5 +300,6 @@ WORKLOADS = {
     "grid_1080p": lambda seed=rng.DEFAULT_SEED: jittered_grid(512, 1024, 1920, 1080, 1024, seed),
     "zipf_4096_1440p": lambda seed=rng.DEFAULT_SEED: zipf_draws(4096, 4_194_304, 2560, 1440, seed),
     "cluster_4m_1080p": lambda seed=rng.DEFAULT_SEED: gaussian_cluster(4_194_304, 1920, 1080, seed=seed),
+    "mixed_1080p": lambda seed=rng.DEFAULT_SEED: mixed_clusters(3_000_000, 30_000, 1920, 1080, seed),
     "stress_4k_msaa4": lambda seed=rng.DEFAULT_SEED: stress_4k(2048, 8_388_608, 3840, 2160, 4, seed),
 }

@@ -277,6 +277,14 @@ def test_gaussian_cluster_skew(mctx, oracle):
     assert ost["fragments"] > 10000
 
 
+def test_mixed_triangle_sizes(mctx, oracle):
+    # a fine cluster (40,000 ~1 px triangles) beside a coarse one (400 of
+    # ~30 px): the balanced form's plan splits by predicted fragments here
+    s = g.mixed_clusters(40_000, 400, 320, 180)
+    st, ost, _ = check_parity(mctx, oracle, s)
+    assert ost["fragments"] > 20000
+
+
 def test_gaussian_cluster_long_lists(mctx, oracle):
     # a tighter cluster: centre tiles exceed one LDS sort run (4,096 ids)
     s = g.gaussian_cluster(300_000, 256, 256, sigma_frac=0.02)
