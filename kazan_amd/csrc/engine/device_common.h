@@ -262,6 +262,14 @@ enum {
 #define KZ_SPLIT_MIN 512.0f
 
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// Fragment-shader arithmetic at Vulkan's precision, not IEEE's: a division
+// within 2.5 ULP (reciprocal then product) and a square root within 1 ULP
+// (v_rcp_f32, v_sqrt_f32), where correctly rounded ones take ~10
+// instructions each. Fragment shaders only feed colour (RGBA8); vertex
+// shaders, which place the samples and depths, keep the IEEE operations.
+__device__ __forceinline__ float kz_fs_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float kz_fs_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Tile t's list: its offset and count. A tile flagged by a cut (binning
 // dropped the pairs of triangles past the list capacity, some of them in
 // this tile) reads as offset list_capacity with at least one entry: the

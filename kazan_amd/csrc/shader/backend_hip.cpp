@@ -169,10 +169,17 @@ struct StageGen {
             }
             return true;
         }
+        case Op::FDiv:
+            if (is_fs) {  // fragment shaders: Vulkan's 2.5 ULP division (kz_fs_div)
+                for (int c = 0; c < lanes(res()); ++c)
+                    os << "    " << lhs(res(), c) << " = kz_fs_div(" << comp(ins.args[0], c) << ", " << comp(ins.args[1], c)
+                       << ");\n";
+                return true;
+            }
+            [[fallthrough]];
         case Op::FAdd:
         case Op::FSub:
         case Op::FMul:
-        case Op::FDiv:
         case Op::Add: {
             const char *o = ins.op == Op::FSub ? " - " : ins.op == Op::FMul ? " * " : ins.op == Op::FDiv ? " / " : " + ";
             for (int c = 0; c < lanes(res()); ++c)
@@ -195,7 +202,8 @@ struct StageGen {
             return true;
         case Op::Sqrt:
             for (int c = 0; c < lanes(res()); ++c)
-                os << "    " << lhs(res(), c) << " = __builtin_sqrtf(" << comp(ins.args[0], c) << ");\n";
+                os << "    " << lhs(res(), c) << (is_fs ? " = kz_fs_sqrt(" : " = __builtin_sqrtf(") << comp(ins.args[0], c)
+                   << ");\n";
             return true;
         case Op::Dot: {
             const int n = lanes(ins.args[0]);
