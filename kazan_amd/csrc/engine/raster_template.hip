@@ -22,15 +22,20 @@
 //
 // Kernels:
 //   kz_bin_count  one thread per triangle: vertex fetch + position part of
-//                 the vertex shader + setup; writes the triangle's tile
-//                 rectangle and bumps the per-tile counters.
+//                 the vertex shader + setup; writes the triangle's pixel box
+//                 (the box of the pixels it covers when the box is small,
+//                 empty when it covers no sample) and the pair count of its
+//                 256-triangle block, from which the prebuilt kernels build
+//                 the per-tile lists in primitive order.
 //   kz_raster     one wave64 per screen tile: walks the tile's triangle list
-//                 (sorted in primitive order by kz_bin_sort) in batches of
-//                 64, one triangle per lane; coverage is compacted with
-//                 ballot + prefix sums into an LDS fragment queue; depth test
-//                 and blending are applied in primitive order per pixel
-//                 against the tile's colour and depth held in LDS; the tile
-//                 is written back once.
+//                 in batches of 64, one triangle per lane; coverage is
+//                 compacted with ballot + prefix sums into an LDS fragment
+//                 queue; depth test and blending are applied in primitive
+//                 order per pixel against the tile's colour and depth held
+//                 in LDS; the tile is written back once.
+//   kz_raster_persistent  the same per work item (a tile or a sub-tile of a
+//                 split tile), resident workgroups taking items from the
+//                 plan's queue, most expensive first.
 
 #define KZ_TS (1 << KZ_TILE_SHIFT)
 #define KZ_TP (KZ_TS * KZ_TS)
