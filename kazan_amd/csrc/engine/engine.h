@@ -43,6 +43,11 @@ struct PipelineKernels {
 // Builds the full kernel source (prelude + shared structs + generated shader
 // functions + the raster template) and compiles it to a gfx950 code object,
 // going through the on-disk code-object cache. Needs no GPU.
+// The code-object cache: directory, and the file name of a pipeline
+// variant's code object (the FNV-1a hash of its complete source, the
+// compile options and the hiprtc version).
+std::string pipeline_code_dir();
+std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source);
 bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
                            std::string &log, bool *cache_hit);
 std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source);

@@ -100,18 +100,24 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
     return os.str();
 }
 
-bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
-                           std::string &log, bool *cache_hit) {
-    const std::string src = pipeline_full_source(fs, shader_source);
-    std::string key_text = src;
+std::string pipeline_code_dir() { return cache_dir(); }
+
+std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source) {
+    std::string key_text = pipeline_full_source(fs, shader_source);
     for (int i = 0; i < kNumOptions; ++i) key_text += std::string("\n") + kCompileOptions[i];
     int rtc_major = 0, rtc_minor = 0;
     hiprtcVersion(&rtc_major, &rtc_minor);
     key_text += "\nhiprtc " + std::to_string(rtc_major) + "." + std::to_string(rtc_minor);
     char name[32];
-    std::snprintf(name, sizeof name, "%016llx", (unsigned long long)fnv1a(key_text));
+    std::snprintf(name, sizeof name, "%016llx.hsaco", (unsigned long long)fnv1a(key_text));
+    return name;
+}
+
+bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
+                           std::string &log, bool *cache_hit) {
+    const std::string src = pipeline_full_source(fs, shader_source);
     const std::string dir = cache_dir();
-    const std::string path = dir + "/" + name + ".hsaco";
+    const std::string path = dir + "/" + pipeline_code_name(fs, shader_source);
     if (cache_hit) *cache_hit = false;
     {
         std::ifstream in(path, std::ios::binary);

@@ -7,6 +7,9 @@
 // the device's HIP stream.
 #include <hip/hip_runtime_api.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -622,15 +625,155 @@ VKAPI_ATTR void VKAPI_CALL vkDestroyShaderModule(VkDevice, VkShaderModule module
     delete from<ShaderModule>(module);
 }
 
-VKAPI_ATTR VkResult VKAPI_CALL vkCreatePipelineCache(VkDevice, const VkPipelineCacheCreateInfo *,
+// Pipeline caches (the reference leaves vkCreatePipelineCache a stub,
+// vulkan-driver/src/api_impl.rs:3979). Code objects live in the on-disk
+// cache keyed by content hash; a VkPipelineCache carries them between
+// processes: its data holds the code objects of the pipelines created with
+// it, and creating a cache from such data (same vendor, device and
+// pipelineCacheUUID) puts them into the on-disk cache, where pipeline
+// creation finds them instead of compiling.
+namespace {
+constexpr size_t kCacheHeader = 32, kCacheName = 32;
+
+void cache_add(PipelineCache *c, const std::string &name, std::vector<char> code) {
+    for (const auto &e : c->entries)
+        if (e.first == name) return;
+    c->entries.emplace_back(name, std::move(code));
+}
+
+bool read_file(const std::string &path, std::vector<char> &out) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    out.clear();
+    char buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) out.insert(out.end(), buf, buf + n);
+    std::fclose(f);
+    return !out.empty();
+}
+
+void write_if_absent(const std::string &dir, const std::string &name, const std::vector<char> &code) {
+    const std::string path = dir + "/" + name;
+    if (FILE *f = std::fopen(path.c_str(), "rb")) {
+        std::fclose(f);
+        return;
+    }
+    mkdir(dir.c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string(getpid());
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    std::fclose(f);
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+void cache_header(const PhysicalDevice &pd, uint8_t out[kCacheHeader]) {
+    const uint32_t w[4] = {(uint32_t)kCacheHeader, 1u /* VK_PIPELINE_CACHE_HEADER_VERSION_ONE */, pd.props.vendorID,
+                           pd.props.deviceID};
+    std::memcpy(out, w, 16);
+    std::memcpy(out + 16, pd.props.pipelineCacheUUID, 16);
+}
+
+// Entries of data written by vkGetPipelineCacheData for this device; other
+// data (another device, driver version or format) is ignored, as Vulkan allows.
+void cache_load(const PhysicalDevice &pd, PipelineCache *c, const uint8_t *data, size_t size) {
+    uint8_t want[kCacheHeader];
+    cache_header(pd, want);
+    if (size < kCacheHeader + 4 || std::memcmp(data, want, kCacheHeader) != 0) return;
+    uint32_t count;
+    std::memcpy(&count, data + kCacheHeader, 4);
+    size_t at = kCacheHeader + 4;
+    const std::string dir = kz::pipeline_code_dir();
+    for (uint32_t i = 0; i < count; ++i) {
+        if (size - at < kCacheName + 8) return;
+        char name[kCacheName + 1] = {};
+        std::memcpy(name, data + at, kCacheName);
+        uint64_t n;
+        std::memcpy(&n, data + at + kCacheName, 8);
+        at += kCacheName + 8;
+        if (n == 0 || n > size - at) return;
+        const std::string nm(name);
+        if (nm.empty() || nm.find('/') != std::string::npos || nm.find("..") != std::string::npos) return;
+        std::vector<char> code(data + at, data + at + n);
+        at += n;
+        write_if_absent(dir, nm, code);
+        cache_add(c, nm, std::move(code));
+    }
+}
+}  // namespace
+
+VKAPI_ATTR VkResult VKAPI_CALL vkCreatePipelineCache(VkDevice device, const VkPipelineCacheCreateInfo *ci,
                                                      const VkAllocationCallbacks *, VkPipelineCache *pCache) {
-    // code objects are cached on disk by content hash; the handle is a token
-    *pCache = to<VkPipelineCache>(new (std::nothrow) int(0));
+    auto *c = new (std::nothrow) PipelineCache();
+    if (!c) return VK_ERROR_OUT_OF_HOST_MEMORY;
+    if (ci && ci->initialDataSize && ci->pInitialData)
+        cache_load(*from<Device>(device)->physical, c, (const uint8_t *)ci->pInitialData, ci->initialDataSize);
+    *pCache = to<VkPipelineCache>(c);
     return VK_SUCCESS;
 }
 
 VKAPI_ATTR void VKAPI_CALL vkDestroyPipelineCache(VkDevice, VkPipelineCache cache, const VkAllocationCallbacks *) {
-    delete from<int>(cache);
+    delete from<PipelineCache>(cache);
+}
+
+// Vulkan's two-call form; with a short buffer, the header and the entries
+// that fit (still valid initial data) and VK_INCOMPLETE.
+VKAPI_ATTR VkResult VKAPI_CALL vkGetPipelineCacheData(VkDevice device, VkPipelineCache cache, size_t *pDataSize,
+                                                      void *pData) {
+    PipelineCache *c = from<PipelineCache>(cache);
+    if (!c->pending.empty()) {  // variants compiled since their pipeline was created (tile-size tuning)
+        const std::string dir = kz::pipeline_code_dir();
+        std::vector<std::string> still;
+        for (const std::string &name : c->pending) {
+            std::vector<char> code;
+            if (read_file(dir + "/" + name, code))
+                cache_add(c, name, std::move(code));
+            else
+                still.push_back(name);
+        }
+        c->pending.swap(still);
+    }
+    size_t total = kCacheHeader + 4;
+    for (const auto &e : c->entries) total += kCacheName + 8 + e.second.size();
+    if (!pData) {
+        *pDataSize = total;
+        return VK_SUCCESS;
+    }
+    if (*pDataSize < kCacheHeader + 4) {
+        *pDataSize = 0;
+        return VK_INCOMPLETE;
+    }
+    uint8_t *out = (uint8_t *)pData;
+    cache_header(*from<Device>(device)->physical, out);
+    size_t at = kCacheHeader + 4;
+    uint32_t count = 0;
+    for (const auto &e : c->entries) {
+        const size_t need = kCacheName + 8 + e.second.size();
+        if (*pDataSize - at < need) break;
+        std::memset(out + at, 0, kCacheName);
+        std::memcpy(out + at, e.first.data(), std::min(e.first.size(), kCacheName));
+        const uint64_t n = e.second.size();
+        std::memcpy(out + at + kCacheName, &n, 8);
+        std::memcpy(out + at + kCacheName + 8, e.second.data(), n);
+        at += need;
+        ++count;
+    }
+    std::memcpy(out + kCacheHeader, &count, 4);
+    const bool all = count == c->entries.size();
+    *pDataSize = at;
+    return all ? VK_SUCCESS : VK_INCOMPLETE;
+}
+
+VKAPI_ATTR VkResult VKAPI_CALL vkMergePipelineCaches(VkDevice, VkPipelineCache dst, uint32_t n,
+                                                     const VkPipelineCache *srcs) {
+    PipelineCache *d = from<PipelineCache>(dst);
+    for (uint32_t i = 0; i < n; ++i) {
+        const PipelineCache *s = from<PipelineCache>(srcs[i]);
+        if (s == d) continue;
+        for (const auto &e : s->entries) cache_add(d, e.first, e.second);
+        d->pending.insert(d->pending.end(), s->pending.begin(), s->pending.end());
+    }
+    return VK_SUCCESS;
 }
 
 VKAPI_ATTR VkResult VKAPI_CALL vkCreateDescriptorSetLayout(VkDevice, const VkDescriptorSetLayoutCreateInfo *ci,
@@ -797,7 +940,7 @@ VKAPI_ATTR void VKAPI_CALL vkDestroyFramebuffer(VkDevice, VkFramebuffer fb, cons
 // ===========================================================================
 // pipelines
 
-VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipelineCache, uint32_t count,
+VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipelineCache cache, uint32_t count,
                                                          const VkGraphicsPipelineCreateInfo *ci,
                                                          const VkAllocationCallbacks *, VkPipeline *pPipelines) {
     Device *d = from<Device>(device);
@@ -813,6 +956,20 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipe
             continue;
         }
         pPipelines[i] = to<VkPipeline>(p);
+        if (cache) {  // the pipeline's code objects (each tile size's variant; those compiled later at data time)
+            PipelineCache *c = from<PipelineCache>(cache);
+            const std::string dir = kz::pipeline_code_dir();
+            for (uint32_t s = kz::kMinTileShift; s <= kz::kMaxTileShift; ++s) {
+                FixedState fs = p->fixed;
+                fs.tile_shift = s;
+                const std::string name = kz::pipeline_code_name(fs, p->shader_source);
+                std::vector<char> code;
+                if (read_file(dir + "/" + name, code))
+                    cache_add(c, name, std::move(code));
+                else
+                    c->pending.push_back(name);
+            }
+        }
     }
     return result;
 }
@@ -1431,6 +1588,8 @@ std::unordered_map<std::string, Entry> &entries() {
         KZ_D(vkDestroyShaderModule);
         KZ_D(vkCreatePipelineCache);
         KZ_D(vkDestroyPipelineCache);
+        KZ_D(vkGetPipelineCacheData);
+        KZ_D(vkMergePipelineCaches);
         KZ_D(vkCreateDescriptorSetLayout);
         KZ_D(vkDestroyDescriptorSetLayout);
         KZ_D(vkCreateDescriptorPool);

@@ -149,6 +149,15 @@ struct Pipeline {
     uint32_t subpass = 0;
 };
 
+// A VkPipelineCache: the code objects (by cache file name) of the pipeline
+// variants created with it or loaded into it. Its data is a Vulkan pipeline
+// cache header (32 bytes) followed by a u32 entry count and, per entry, the
+// file name (32 bytes, zero-padded), a u64 size and the code object.
+struct PipelineCache {
+    std::vector<std::pair<std::string, std::vector<char>>> entries;
+    std::vector<std::string> pending;  // variants not compiled yet when their pipeline was created
+};
+
 struct CommandPool {
     std::vector<struct CommandBuffer *> buffers;
 };

@@ -50,6 +50,7 @@ class Context:
         pds = (vk.Handle * n.value)()
         d.check("vkEnumeratePhysicalDevices", self.instance, _ref(n), pds)
         self.physical = vk.Handle(pds[0])
+        self.pipeline_cache = None  # VkPipelineCache handle passed to vkCreateGraphicsPipelines, if set
         self.props = vk.VkPhysicalDeviceProperties()
         d.vkGetPhysicalDeviceProperties(self.physical, _ref(self.props))
         prio = ctypes.c_float(1.0)
@@ -329,7 +330,8 @@ class SceneRenderer:
                                               ctypes.pointer(rs), ctypes.pointer(ms), ctypes.pointer(ds),
                                               ctypes.pointer(cbs), None, self.layout, self.rp, 0, None, -1)
         self.pipeline = vk.Handle()
-        d.check("vkCreateGraphicsPipelines", dev, None, 1, _ref(gci), None, _ref(self.pipeline))
+        d.check("vkCreateGraphicsPipelines", dev, self.ctx.pipeline_cache, 1, _ref(gci), None,
+                _ref(self.pipeline))
 
     def _record(self):
         d, s = self.ctx.d, self.s

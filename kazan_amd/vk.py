@@ -43,6 +43,7 @@ ST_BUFFER_CREATE_INFO = 12
 ST_IMAGE_CREATE_INFO = 14
 ST_IMAGE_VIEW_CREATE_INFO = 15
 ST_SHADER_MODULE_CREATE_INFO = 16
+ST_PIPELINE_CACHE_CREATE_INFO = 17
 ST_PIPELINE_SHADER_STAGE_CREATE_INFO = 18
 ST_PIPELINE_VERTEX_INPUT_STATE_CREATE_INFO = 19
 ST_PIPELINE_INPUT_ASSEMBLY_STATE_CREATE_INFO = 20
@@ -258,6 +259,11 @@ class VkSamplerCreateInfo(Structure):
 class VkShaderModuleCreateInfo(Structure):
     _fields_ = [("sType", c_int32), ("pNext", c_void_p), ("flags", c_uint32), ("codeSize", c_size_t),
                 ("pCode", c_void_p)]
+
+
+class VkPipelineCacheCreateInfo(Structure):
+    _fields_ = [("sType", c_int32), ("pNext", c_void_p), ("flags", c_uint32), ("initialDataSize", c_size_t),
+                ("pInitialData", c_void_p)]
 
 
 class VkDescriptorSetLayoutBinding(Structure):
@@ -491,7 +497,7 @@ class VkFenceCreateInfo(Structure):
 
 
 # Sizes the C compiler gives the same structs on LP64 (checked by the tests).
-ABI_STRUCTS = [VkApplicationInfo, VkInstanceCreateInfo, VkPhysicalDeviceLimits, VkPhysicalDeviceProperties,
+ABI_STRUCTS = [VkApplicationInfo, VkPipelineCacheCreateInfo, VkInstanceCreateInfo, VkPhysicalDeviceLimits, VkPhysicalDeviceProperties,
                VkPhysicalDeviceMemoryProperties, VkDeviceQueueCreateInfo, VkDeviceCreateInfo, VkMemoryAllocateInfo,
                VkMemoryRequirements, VkBufferCreateInfo, VkImageCreateInfo, VkImageViewCreateInfo,
                VkSamplerCreateInfo, VkShaderModuleCreateInfo, VkDescriptorSetLayoutBinding,
@@ -547,6 +553,10 @@ _SIGS = {
     "vkAllocateDescriptorSets": (c_int32, [Handle, P(VkDescriptorSetAllocateInfo), P(Handle)]),
     "vkUpdateDescriptorSets": (None, [Handle, c_uint32, P(VkWriteDescriptorSet), c_uint32, c_void_p]),
     "vkCreatePipelineLayout": (c_int32, [Handle, P(VkPipelineLayoutCreateInfo), c_void_p, P(Handle)]),
+    "vkCreatePipelineCache": (c_int32, [Handle, P(VkPipelineCacheCreateInfo), c_void_p, P(Handle)]),
+    "vkDestroyPipelineCache": (None, [Handle, Handle, c_void_p]),
+    "vkGetPipelineCacheData": (c_int32, [Handle, Handle, P(c_size_t), c_void_p]),
+    "vkMergePipelineCaches": (c_int32, [Handle, Handle, c_uint32, P(Handle)]),
     "vkDestroyPipelineLayout": (None, [Handle, Handle, c_void_p]),
     "vkCreateRenderPass": (c_int32, [Handle, P(VkRenderPassCreateInfo), c_void_p, P(Handle)]),
     "vkDestroyRenderPass": (None, [Handle, Handle, c_void_p]),
