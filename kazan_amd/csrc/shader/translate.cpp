@@ -843,6 +843,35 @@ bool Translator::translate_function(uint32_t fn) {
                 res = emit(Op::FMul, {a[0]->ir[0], emit(Op::Splat, {rl}, rt)}, rt);
                 break;
             }
+            case 4: res = emit(Op::FMax, {a[0]->ir[0], emit(Op::FNeg, {a[0]->ir[0]}, rt)}, rt); break;    // FAbs
+            case 46: {                                                                                  // FMix: x + (y - x) * a
+                const uint32_t d = emit(Op::FSub, {a[1]->ir[0], a[0]->ir[0]}, rt);
+                res = emit(Op::FAdd, {a[0]->ir[0], emit(Op::FMul, {d, a[2]->ir[0]}, rt)}, rt);
+                break;
+            }
+            case 67: {                                                                                  // Distance
+                const Type vt = ir_type(a[0]->type);
+                const uint32_t d = emit(Op::FSub, {a[0]->ir[0], a[1]->ir[0]}, vt);
+                const uint32_t dd = vt.lanes > 1 ? emit(Op::Dot, {d, d}, Type::f32()) : emit(Op::FMul, {d, d}, vt);
+                res = emit(Op::Sqrt, {dd}, rt);
+                break;
+            }
+            case 68: {                                                                                  // Cross: x.yzx*y.zxy - x.zxy*y.yzx
+                const uint32_t x = a[0]->ir[0], y = a[1]->ir[0];
+                auto sw = [&](uint32_t v, uint32_t i, uint32_t j, uint32_t k) { return emit(Op::Shuffle, {v, v}, rt, {i, j, k}); };
+                const uint32_t p = emit(Op::FMul, {sw(x, 1, 2, 0), sw(y, 2, 0, 1)}, rt);
+                const uint32_t q = emit(Op::FMul, {sw(x, 2, 0, 1), sw(y, 1, 2, 0)}, rt);
+                res = emit(Op::FSub, {p, q}, rt);
+                break;
+            }
+            case 71: {                                                                                  // Reflect: I - 2 dot(N, I) N
+                const uint32_t i = a[0]->ir[0], n = a[1]->ir[0];
+                const uint32_t d = rt.lanes > 1 ? emit(Op::Dot, {n, i}, Type::f32()) : emit(Op::FMul, {n, i}, rt);
+                const uint32_t d2 = emit(Op::FMul, {m.new_const_f32(2.0f), d}, Type::f32());
+                const uint32_t s2 = rt.lanes > 1 ? emit(Op::Splat, {d2}, rt) : d2;
+                res = emit(Op::FSub, {i, emit(Op::FMul, {s2, n}, rt)}, rt);
+                break;
+            }
             default: return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
             }
             m.value(res).name = value_name(o[1]);

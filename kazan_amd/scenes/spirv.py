@@ -102,6 +102,11 @@ Dim2D = 1
 FunctionControlNone = 0
 
 # GLSL.std.450 extended instructions
+GLSL_FAbs = 4
+GLSL_FMix = 46
+GLSL_Distance = 67
+GLSL_Cross = 68
+GLSL_Reflect = 71
 GLSL_FMin = 37
 GLSL_FMax = 40
 GLSL_FClamp = 43
