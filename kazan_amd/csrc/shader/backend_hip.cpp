@@ -205,6 +205,12 @@ struct StageGen {
                 os << "    " << lhs(res(), c) << (is_fs ? " = kz_fs_sqrt(" : " = __builtin_sqrtf(") << comp(ins.args[0], c)
                    << ");\n";
             return true;
+        case Op::Floor:
+        case Op::Ceil:
+            for (int c = 0; c < lanes(res()); ++c)
+                os << "    " << lhs(res(), c) << (ins.op == Op::Floor ? " = __builtin_floorf(" : " = __builtin_ceilf(")
+                   << comp(ins.args[0], c) << ");\n";
+            return true;
         case Op::Dot: {
             const int n = lanes(ins.args[0]);
             std::string e = comp(ins.args[0], 0) + " * " + comp(ins.args[1], 0);

@@ -45,6 +45,8 @@ const char *op_text(Op op) {
     case Op::FMin: return "fmin";
     case Op::FClamp: return "fclamp";
     case Op::Sqrt: return "sqrt";
+    case Op::Floor: return "floor";
+    case Op::Ceil: return "ceil";
     case Op::Dot: return "dot";
     case Op::Splat: return "splat";
     case Op::MakeVector: return "make_vector";

@@ -61,6 +61,8 @@ enum class Op : uint8_t {
     FMin,
     FClamp,   // [x, lo, hi]
     Sqrt,
+    Floor,
+    Ceil,
     Dot,      // ((a0*b0 + a1*b1) + a2*b2) + a3*b3
     Splat,    // scalar -> vector
     MakeVector,

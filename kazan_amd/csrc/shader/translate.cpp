@@ -843,6 +843,11 @@ bool Translator::translate_function(uint32_t fn) {
                 res = emit(Op::FMul, {a[0]->ir[0], emit(Op::Splat, {rl}, rt)}, rt);
                 break;
             }
+            case 8: res = emit(Op::Floor, {a[0]->ir[0]}, rt); break;                                   // Floor
+            case 9: res = emit(Op::Ceil, {a[0]->ir[0]}, rt); break;                                    // Ceil
+            case 10:                                                                                    // Fract: x - floor(x)
+                res = emit(Op::FSub, {a[0]->ir[0], emit(Op::Floor, {a[0]->ir[0]}, rt)}, rt);
+                break;
             case 4: res = emit(Op::FMax, {a[0]->ir[0], emit(Op::FNeg, {a[0]->ir[0]}, rt)}, rt); break;    // FAbs
             case 46: {                                                                                  // FMix: x + (y - x) * a
                 const uint32_t d = emit(Op::FSub, {a[1]->ir[0], a[0]->ir[0]}, rt);

@@ -103,6 +103,9 @@ FunctionControlNone = 0
 
 # GLSL.std.450 extended instructions
 GLSL_FAbs = 4
+GLSL_Floor = 8
+GLSL_Ceil = 9
+GLSL_Fract = 10
 GLSL_FMix = 46
 GLSL_Distance = 67
 GLSL_Cross = 68
