@@ -45,10 +45,19 @@ def algorithmic_bytes(scene, fragments):
     return geo + tex + fragments * per_frag, per_frag
 
 
+def host_threads():
+    """Host cores this process may use: the CPU affinity set, bounded by the
+    share the machine grants one GPU's job when it states one
+    (OMP_NUM_THREADS); returns (threads, visible cores)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(visible, int(share)) if share.isdigit() and int(share) > 0 else visible
+    return max(1, threads), visible
+
+
 def cpu_baseline(scene, fragments, seconds):
     from oracle import bridge  # the CPU restatement: measured beside the GPU, never the product
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    threads, visible = host_threads()
     times = []
     t_end = time.perf_counter() + seconds
     while time.perf_counter() < t_end or len(times) < 2:
@@ -60,7 +69,8 @@ def cpu_baseline(scene, fragments, seconds):
     assert st["fragments"] == fragments, "CPU and GPU fragment counts differ"
     med = statistics.median(times)
     return {"value": fragments / med, "unit": "fragments/s", "cores": threads, "kind": "port",
-            "frame_ms": med * 1e3,
+            "label": "build CPU restatement, not Kazan (Kazan has no CPU draw path)",
+            "host_cores_visible": visible, "frame_ms": med * 1e3,
             "sample": "full %s frame (%d fragments), %d repetitions, median" % (scene.name, fragments, len(times))}
 
 
@@ -91,8 +101,24 @@ def measure(scene, form, args):
     r = SceneRenderer(ctx, scene)
     print("bench[%s]: %s on %s, %d triangles" % (form, scene.name, ctx.device_name, scene.num_triangles),
           file=sys.stderr, flush=True)
+    # steady state first: the balanced form times one tuning candidate per
+    # submission, and every form runs its first submissions eagerly before
+    # the command buffer is captured into a hipGraph. Submit until the
+    # engine reports the graph captured (so tuning is over), then warm up.
+    caps, launches = ctypes.c_uint32(), ctypes.c_uint32()
+    tuning_frames = 0
+    while tuning_frames < 200:
+        ctx.d.vkGetGraphStatsKAZAN(ctx.device, ctypes.byref(caps), ctypes.byref(launches))
+        if caps.value >= 1:
+            break
+        r.render(1)
+        tuning_frames += 1
+    steady = caps.value >= 1
     if args.warmup:
         r.render(args.warmup)
+    ctx.stats()  # drops the counters of the tuning and warm-up frames
+    ctx.d.vkGetGraphStatsKAZAN(ctx.device, ctypes.byref(caps), ctypes.byref(launches))
+    caps0 = caps.value
     # timed region: K frames back to back, bracketed by queue waits
     ctx.wait_idle()
     t0 = time.perf_counter()
@@ -101,6 +127,8 @@ def measure(scene, form, args):
     ctx.wait_idle()
     t1 = time.perf_counter()
     st = ctx.stats()
+    ctx.d.vkGetGraphStatsKAZAN(ctx.device, ctypes.byref(caps), ctypes.byref(launches))
+    recaptures = caps.value - caps0
     print("bench[%s]: %d frames in %.3f s, gpu %.3f ms total" % (form, args.steps, t1 - t0, st["gpu_ms"]),
           file=sys.stderr, flush=True)
     ctx.set_kernel_timing(True)
@@ -111,10 +139,32 @@ def measure(scene, form, args):
     ts, gd, cm, pr, ml, fs = (ctypes.c_uint32() for _ in range(6))
     ctx.d.vkGetTuningKAZAN(ctx.device, *(ctypes.byref(x) for x in (ts, gd, cm, pr, ml, fs)))
     return {"stats": st, "wall": t1 - t0, "ktimes": ktimes, "n_prof": n_prof, "ctx": ctx, "renderer": r,
-            "cost_model": has_model,
+            "cost_model": has_model, "tuning_frames": tuning_frames, "steady": steady,
+            "recaptures_in_timed_region": recaptures,
             "tuned": {"tile": ts.value, "grid_divisor": gd.value, "cost_model": bool(cm.value),
                       "persistent_raster": bool(pr.value), "max_split_level": ml.value,
                       "frag_weight_scale": fs.value}}
+
+
+def skewed_summary(scene, order, args):
+    """Frame time and fragments/s of each form on a skewed workload (the
+    case the balanced form and its cost model exist for), and the ratios to
+    the plain form."""
+    res = {}
+    for form in order:
+        m = measure(scene, form, args)
+        res[form] = {"ms_per_frame": m["wall"] * 1e3 / args.steps,
+                     "fragments_per_s": m["stats"]["fragments"] / m["wall"],
+                     "fragments_per_frame": m["stats"]["fragments"] // args.steps,
+                     "tuning_frames": m["tuning_frames"], "tuned": m["tuned"],
+                     "recaptures_in_timed_region": m["recaptures_in_timed_region"]}
+        del m
+    out = {"workload": scene.name, "triangles": scene.num_triangles, "forms": res}
+    if "plain" in res:
+        for f in res:
+            if f != "plain":
+                out["%s_vs_plain" % f] = res[f]["fragments_per_s"] / res["plain"]["fragments_per_s"]
+    return out
 
 
 def copy_probe(nbytes, reps=3):
@@ -144,6 +194,9 @@ def main():
     # past that
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="grid_1080p")
+    ap.add_argument("--skewed", default="cluster_4m_1080p",
+                    help="skewed workload whose frame times and ratios per form go into config.skewed "
+                         "('none' skips it)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -213,6 +266,9 @@ def main():
                           "split_entries_per_frame": v["stats"]["split_entries"] / args.steps,
                           "cost_model": v["cost_model"],
                           "tuned": v["tuned"],
+                          "tuning_frames": v["tuning_frames"],
+                          "steady_state": v["steady"],
+                          "recaptures_in_timed_region": v["recaptures_in_timed_region"],
                           "kernel_ms_per_frame": {k: t[0] / v["n_prof"] for k, t in v["ktimes"].items()}}
                       for f, v in forms.items()},
         },
@@ -234,6 +290,8 @@ def main():
         for f in forms:
             if f != "plain":
                 out["config"]["%s_vs_plain" % f] = rate(f) / rate("plain")
+    if args.skewed != "none" and args.skewed != args.workload:
+        out["config"]["skewed"] = skewed_summary(g.WORKLOADS[args.skewed](), order, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, frags_per_frame, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -48,8 +48,17 @@ struct PipelineKernels {
 // compile options and the hiprtc version).
 std::string pipeline_code_dir();
 std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source);
+// With read_cache false the on-disk copy is ignored and replaced (a file
+// that failed to load). A cached file that is not a gfx950 code object is
+// never returned.
 bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
-                           std::string &log, bool *cache_hit);
+                           std::string &log, bool *cache_hit, bool read_cache = true);
+// FNV-1a (64-bit) of a byte range: cache file names and cache-data entry
+// checksums.
+uint64_t fnv1a64(const void *data, size_t n, uint64_t h = 1469598103934665603ull);
+// An ELF64 object for AMDGPU (e_machine 224): the shape of every code object
+// hiprtc writes; anything else in a cache is damaged or foreign.
+bool is_amdgpu_code_object(const char *data, size_t n);
 std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source);
 // Loads the code object on the current device.
 bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, std::string &err);
@@ -76,6 +85,12 @@ class PipelineVariants {
     // null when the tile size does not fit the workgroup's LDS or the
     // compile failed (err says why)
     const PipelineKernels *get(uint32_t tile_shift, std::string *err = nullptr);
+    // A code object for one tile size supplied by a VkPipelineCache: get()
+    // loads it before asking the on-disk cache or the compiler, and falls
+    // back to them when it does not load.
+    void preload(uint32_t tile_shift, std::vector<char> code) {
+        if (tile_shift <= kMaxTileShift && !tried_[tile_shift]) preloaded_[tile_shift] = std::move(code);
+    }
     bool fits(uint32_t tile_shift) const {
         return tile_shift >= kMinTileShift && tile_shift <= kMaxTileShift &&
                raster_lds_bytes(tile_shift, fixed.samples) <= 160u * 1024u;
@@ -83,6 +98,7 @@ class PipelineVariants {
 
   private:
     PipelineKernels k_[kMaxTileShift + 1];
+    std::vector<char> preloaded_[kMaxTileShift + 1];
     bool tried_[kMaxTileShift + 1] = {};
     std::string err_[kMaxTileShift + 1];
 };

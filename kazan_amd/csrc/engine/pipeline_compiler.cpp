@@ -41,13 +41,7 @@ const char *const kCompileOptions[] = {
 };
 constexpr int kNumOptions = sizeof(kCompileOptions) / sizeof(kCompileOptions[0]);
 
-uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull) {
-    for (unsigned char c : s) {
-        h ^= c;
-        h *= 1099511628211ull;
-    }
-    return h;
-}
+uint64_t fnv1a(const std::string &s) { return fnv1a64(s.data(), s.size()); }
 
 std::string cache_dir() {
     if (const char *e = std::getenv("KAZAN_PIPELINE_CACHE")) return e;
@@ -100,6 +94,24 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
     return os.str();
 }
 
+uint64_t fnv1a64(const void *data, size_t n, uint64_t h) {
+    const unsigned char *p = (const unsigned char *)data;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+bool is_amdgpu_code_object(const char *data, size_t n) {
+    // e_ident: 0x7f 'E' 'L' 'F', EI_CLASS = 2 (64-bit), EI_DATA = 1 (LSB);
+    // e_machine (offset 18, u16) = EM_AMDGPU (224)
+    if (!data || n < 64) return false;
+    const unsigned char *p = (const unsigned char *)data;
+    return p[0] == 0x7f && p[1] == 'E' && p[2] == 'L' && p[3] == 'F' && p[4] == 2 && p[5] == 1 &&
+           (p[18] | (p[19] << 8)) == 224;
+}
+
 std::string pipeline_code_dir() { return cache_dir(); }
 
 std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source) {
@@ -114,16 +126,16 @@ std::string pipeline_code_name(const FixedState &fs, const std::string &shader_s
 }
 
 bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
-                           std::string &log, bool *cache_hit) {
+                           std::string &log, bool *cache_hit, bool read_cache) {
     const std::string src = pipeline_full_source(fs, shader_source);
     const std::string dir = cache_dir();
     const std::string path = dir + "/" + pipeline_code_name(fs, shader_source);
     if (cache_hit) *cache_hit = false;
-    {
+    if (read_cache) {
         std::ifstream in(path, std::ios::binary);
         if (in) {
             code.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
-            if (!code.empty()) {
+            if (is_amdgpu_code_object(code.data(), code.size())) {
                 if (cache_hit) *cache_hit = true;
                 return true;
             }
@@ -199,12 +211,26 @@ const PipelineKernels *PipelineVariants::get(uint32_t tile_shift, std::string *e
         tried_[tile_shift] = true;
         FixedState fs = fixed;
         fs.tile_shift = tile_shift;
-        std::vector<char> code;
-        std::string log;
-        if (!compile_pipeline_code(fs, shader_source, code, log, nullptr))
-            err_[tile_shift] = "kernel compilation failed:\n" + log;
-        else if (!load_pipeline_kernels(code, k_[tile_shift], err_[tile_shift]))
-            k_[tile_shift] = PipelineKernels{};
+        bool ok = false;
+        std::vector<char> &pre = preloaded_[tile_shift];
+        if (!pre.empty()) {  // from a VkPipelineCache
+            ok = load_pipeline_kernels(pre, k_[tile_shift], err_[tile_shift]);
+            if (!ok) k_[tile_shift] = PipelineKernels{};
+            std::vector<char>().swap(pre);
+        }
+        for (int attempt = 0; !ok && attempt < 2; ++attempt) {
+            // the second attempt replaces an on-disk copy that did not load
+            std::vector<char> code;
+            std::string log;
+            bool hit = false;
+            if (!compile_pipeline_code(fs, shader_source, code, log, &hit, attempt == 0)) {
+                err_[tile_shift] = "kernel compilation failed:\n" + log;
+                break;
+            }
+            ok = load_pipeline_kernels(code, k_[tile_shift], err_[tile_shift]);
+            if (!ok) k_[tile_shift] = PipelineKernels{};
+            if (!hit) break;
+        }
     }
     if (!k_[tile_shift].module) {
         if (err) *err = err_[tile_shift];

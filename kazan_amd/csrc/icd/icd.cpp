@@ -11,11 +11,13 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "kazan/kazan_icd.h"
@@ -23,7 +25,8 @@
 #include "objects.h"
 
 namespace kz {
-bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *ci, Pipeline **out, std::string &err);
+bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *ci, PipelineCache *cache,
+                              Pipeline **out, std::string &err);
 void destroy_pipeline(Pipeline *p);
 }  // namespace kz
 
@@ -652,20 +655,6 @@ bool read_file(const std::string &path, std::vector<char> &out) {
     return !out.empty();
 }
 
-void write_if_absent(const std::string &dir, const std::string &name, const std::vector<char> &code) {
-    const std::string path = dir + "/" + name;
-    if (FILE *f = std::fopen(path.c_str(), "rb")) {
-        std::fclose(f);
-        return;
-    }
-    mkdir(dir.c_str(), 0755);
-    const std::string tmp = path + ".tmp" + std::to_string(getpid());
-    FILE *f = std::fopen(tmp.c_str(), "wb");
-    if (!f) return;
-    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
-    std::fclose(f);
-    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
-}
 
 void cache_header(const PhysicalDevice &pd, uint8_t out[kCacheHeader]) {
     const uint32_t w[4] = {(uint32_t)kCacheHeader, 1u /* VK_PIPELINE_CACHE_HEADER_VERSION_ONE */, pd.props.vendorID,
@@ -675,7 +664,11 @@ void cache_header(const PhysicalDevice &pd, uint8_t out[kCacheHeader]) {
 }
 
 // Entries of data written by vkGetPipelineCacheData for this device; other
-// data (another device, driver version or format) is ignored, as Vulkan allows.
+// data (another device, driver version or format) is ignored, as Vulkan
+// allows, and so is an entry whose checksum does not match its bytes or
+// whose bytes are not a gfx950 code object. Entries stay in the cache
+// object: pipelines created with it load them (nothing is written to the
+// shared on-disk cache), and one that does not load is compiled instead.
 void cache_load(const PhysicalDevice &pd, PipelineCache *c, const uint8_t *data, size_t size) {
     uint8_t want[kCacheHeader];
     cache_header(pd, want);
@@ -683,21 +676,21 @@ void cache_load(const PhysicalDevice &pd, PipelineCache *c, const uint8_t *data,
     uint32_t count;
     std::memcpy(&count, data + kCacheHeader, 4);
     size_t at = kCacheHeader + 4;
-    const std::string dir = kz::pipeline_code_dir();
     for (uint32_t i = 0; i < count; ++i) {
-        if (size - at < kCacheName + 8) return;
+        if (size - at < kCacheName + 16) return;
         char name[kCacheName + 1] = {};
         std::memcpy(name, data + at, kCacheName);
-        uint64_t n;
+        uint64_t n, sum;
         std::memcpy(&n, data + at + kCacheName, 8);
-        at += kCacheName + 8;
+        std::memcpy(&sum, data + at + kCacheName + 8, 8);
+        at += kCacheName + 16;
         if (n == 0 || n > size - at) return;
         const std::string nm(name);
-        if (nm.empty() || nm.find('/') != std::string::npos || nm.find("..") != std::string::npos) return;
-        std::vector<char> code(data + at, data + at + n);
+        const char *bytes = (const char *)data + at;
         at += n;
-        write_if_absent(dir, nm, code);
-        cache_add(c, nm, std::move(code));
+        if (nm.empty() || nm.find('/') != std::string::npos || nm.find("..") != std::string::npos) continue;
+        if (kz::fnv1a64(bytes, n) != sum || !kz::is_amdgpu_code_object(bytes, n)) continue;
+        cache_add(c, nm, std::vector<char>(bytes, bytes + n));
     }
 }
 }  // namespace
@@ -721,12 +714,13 @@ VKAPI_ATTR void VKAPI_CALL vkDestroyPipelineCache(VkDevice, VkPipelineCache cach
 VKAPI_ATTR VkResult VKAPI_CALL vkGetPipelineCacheData(VkDevice device, VkPipelineCache cache, size_t *pDataSize,
                                                       void *pData) {
     PipelineCache *c = from<PipelineCache>(cache);
+    std::lock_guard<std::mutex> lock(c->mu);
     if (!c->pending.empty()) {  // variants compiled since their pipeline was created (tile-size tuning)
         const std::string dir = kz::pipeline_code_dir();
         std::vector<std::string> still;
         for (const std::string &name : c->pending) {
             std::vector<char> code;
-            if (read_file(dir + "/" + name, code))
+            if (read_file(dir + "/" + name, code) && kz::is_amdgpu_code_object(code.data(), code.size()))
                 cache_add(c, name, std::move(code));
             else
                 still.push_back(name);
@@ -734,7 +728,7 @@ VKAPI_ATTR VkResult VKAPI_CALL vkGetPipelineCacheData(VkDevice device, VkPipelin
         c->pending.swap(still);
     }
     size_t total = kCacheHeader + 4;
-    for (const auto &e : c->entries) total += kCacheName + 8 + e.second.size();
+    for (const auto &e : c->entries) total += kCacheName + 16 + e.second.size();
     if (!pData) {
         *pDataSize = total;
         return VK_SUCCESS;
@@ -748,13 +742,14 @@ VKAPI_ATTR VkResult VKAPI_CALL vkGetPipelineCacheData(VkDevice device, VkPipelin
     size_t at = kCacheHeader + 4;
     uint32_t count = 0;
     for (const auto &e : c->entries) {
-        const size_t need = kCacheName + 8 + e.second.size();
+        const size_t need = kCacheName + 16 + e.second.size();
         if (*pDataSize - at < need) break;
         std::memset(out + at, 0, kCacheName);
         std::memcpy(out + at, e.first.data(), std::min(e.first.size(), kCacheName));
-        const uint64_t n = e.second.size();
+        const uint64_t n = e.second.size(), sum = kz::fnv1a64(e.second.data(), n);
         std::memcpy(out + at + kCacheName, &n, 8);
-        std::memcpy(out + at + kCacheName + 8, e.second.data(), n);
+        std::memcpy(out + at + kCacheName + 8, &sum, 8);
+        std::memcpy(out + at + kCacheName + 16, e.second.data(), n);
         at += need;
         ++count;
     }
@@ -768,10 +763,20 @@ VKAPI_ATTR VkResult VKAPI_CALL vkMergePipelineCaches(VkDevice, VkPipelineCache d
                                                      const VkPipelineCache *srcs) {
     PipelineCache *d = from<PipelineCache>(dst);
     for (uint32_t i = 0; i < n; ++i) {
-        const PipelineCache *s = from<PipelineCache>(srcs[i]);
+        PipelineCache *s = from<PipelineCache>(srcs[i]);
         if (s == d) continue;
-        for (const auto &e : s->entries) cache_add(d, e.first, e.second);
-        d->pending.insert(d->pending.end(), s->pending.begin(), s->pending.end());
+        // a snapshot of the source under its lock, then the destination under its own
+        std::vector<std::pair<std::string, std::vector<char>>> entries;
+        std::vector<std::string> pending;
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            entries = s->entries;
+            pending = s->pending;
+        }
+        std::lock_guard<std::mutex> lock(d->mu);
+        for (auto &e : entries) cache_add(d, e.first, std::move(e.second));
+        for (auto &name : pending)
+            if (std::find(d->pending.begin(), d->pending.end(), name) == d->pending.end()) d->pending.push_back(name);
     }
     return VK_SUCCESS;
 }
@@ -949,7 +954,7 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipe
     for (uint32_t i = 0; i < count; ++i) {
         Pipeline *p = nullptr;
         std::string err;
-        if (!create_graphics_pipeline(d, &ci[i], &p, err)) {
+        if (!create_graphics_pipeline(d, &ci[i], cache ? from<PipelineCache>(cache) : nullptr, &p, err)) {
             std::fprintf(stderr, "kazan: vkCreateGraphicsPipelines: %s\n", err.c_str());
             pPipelines[i] = nullptr;
             result = VK_ERROR_INVALID_SHADER_NV;
@@ -959,14 +964,19 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipe
         if (cache) {  // the pipeline's code objects (each tile size's variant; those compiled later at data time)
             PipelineCache *c = from<PipelineCache>(cache);
             const std::string dir = kz::pipeline_code_dir();
+            std::lock_guard<std::mutex> lock(c->mu);
             for (uint32_t s = kz::kMinTileShift; s <= kz::kMaxTileShift; ++s) {
+                if (!p->variants.fits(s)) continue;  // a tile size this pipeline can never compile
                 FixedState fs = p->fixed;
                 fs.tile_shift = s;
                 const std::string name = kz::pipeline_code_name(fs, p->shader_source);
+                bool have = false;
+                for (const auto &e : c->entries) have = have || e.first == name;
+                if (have) continue;
                 std::vector<char> code;
-                if (read_file(dir + "/" + name, code))
+                if (read_file(dir + "/" + name, code) && kz::is_amdgpu_code_object(code.data(), code.size()))
                     cache_add(c, name, std::move(code));
-                else
+                else if (std::find(c->pending.begin(), c->pending.end(), name) == c->pending.end())
                     c->pending.push_back(name);
             }
         }
@@ -1460,16 +1470,51 @@ VKAPI_ATTR VkResult VKAPI_CALL vkWaitForFences(VkDevice device, uint32_t n, cons
                                                uint64_t timeout) {
     Device *d = from<Device>(device);
     set_device(d);
-    (void)waitAll;
-    (void)timeout;
-    for (uint32_t i = 0; i < n; ++i) {
-        Fence *f = from<Fence>(fences[i]);
-        if (f->signaled) continue;
-        if (!f->submitted) return VK_TIMEOUT;
-        if (hipEventSynchronize(f->event) != hipSuccess) return VK_ERROR_DEVICE_LOST;
-        f->signaled = true;
+    // 1 signaled, 0 not yet, -1 device error
+    auto poll = [](Fence *f) {
+        if (f->signaled) return 1;
+        if (!f->submitted) return 0;
+        const hipError_t e = hipEventQuery(f->event);
+        if (e == hipSuccess) {
+            f->signaled = true;
+            return 1;
+        }
+        return e == hipErrorNotReady ? 0 : -1;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    // timeouts beyond ~292 years are infinite (and would overflow the clock)
+    const bool infinite = timeout >= (uint64_t)INT64_MAX;
+    for (;;) {
+        uint32_t done = 0, unsubmitted = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            Fence *f = from<Fence>(fences[i]);
+            const int r = poll(f);
+            if (r < 0) return VK_ERROR_DEVICE_LOST;
+            done += (uint32_t)r;
+            unsubmitted += (r == 0 && !f->submitted);
+        }
+        if (waitAll ? done == n : (done > 0 || n == 0)) break;
+        if (timeout == 0) return VK_TIMEOUT;
+        const uint64_t elapsed = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                     std::chrono::steady_clock::now() - t0).count();
+        if (!infinite && elapsed >= timeout) return VK_TIMEOUT;
+        if (waitAll && unsubmitted == 0 && infinite) {
+            // every fence is on its way: block on each instead of polling
+            for (uint32_t i = 0; i < n; ++i) {
+                Fence *f = from<Fence>(fences[i]);
+                if (f->signaled) continue;
+                if (hipEventSynchronize(f->event) != hipSuccess) return VK_ERROR_DEVICE_LOST;
+                f->signaled = true;
+            }
+            continue;
+        }
+        // a fence another thread has yet to submit, a partial wait or a
+        // finite timeout: poll
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
-    d->engine->collect_stats();
+    // the statistics fold waits for the whole queue: only when that costs
+    // nothing more than the wait itself
+    if (waitAll || hipStreamQuery(d->engine->stream()) == hipSuccess) d->engine->collect_stats();
     return VK_SUCCESS;
 }
 

@@ -12,6 +12,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -154,6 +155,9 @@ struct Pipeline {
 // cache header (32 bytes) followed by a u32 entry count and, per entry, the
 // file name (32 bytes, zero-padded), a u64 size and the code object.
 struct PipelineCache {
+    // Vulkan lets several threads use one cache at once (unless it was
+    // created externally synchronized): every access takes the lock
+    std::mutex mu;
     std::vector<std::pair<std::string, std::vector<char>>> entries;
     std::vector<std::string> pending;  // variants not compiled yet when their pipeline was created
 };

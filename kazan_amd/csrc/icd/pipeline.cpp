@@ -57,7 +57,8 @@ void build_slot_maps(const PipelineLayout *layout, std::map<std::pair<uint32_t, 
     }
 }
 
-bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *ci, Pipeline **out, std::string &err) {
+bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *ci, PipelineCache *cache,
+                              Pipeline **out, std::string &err) {
     (void)dev;
     const VkPipelineShaderStageCreateInfo *vs = nullptr, *fs = nullptr;
     for (uint32_t i = 0; i < ci->stageCount; ++i) {
@@ -184,6 +185,18 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
     p->shader_source = co.source;
     p->variants.fixed = fx;
     p->variants.shader_source = co.source;
+    // code objects the application's pipeline cache holds for this pipeline
+    if (cache) {
+        std::lock_guard<std::mutex> lock(cache->mu);
+        for (uint32_t s = kMinTileShift; s <= kMaxTileShift; ++s) {
+            if (!p->variants.fits(s)) continue;
+            FixedState fs = fx;
+            fs.tile_shift = s;
+            const std::string name = pipeline_code_name(fs, co.source);
+            for (const auto &e : cache->entries)
+                if (e.first == name) p->variants.preload(s, e.second);
+        }
+    }
     // the default tile size now; others when the autotuner asks for them
     if (!p->variants.get(fx.tile_shift, &err)) return false;
     *out = p.release();

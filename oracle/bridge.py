@@ -58,6 +58,8 @@ def lib():
         _lib.kzo_encode_unorm8.restype = c_u32
         _lib.kzo_encode_unorm8.argtypes = [c_f32]
         _lib.kzo_bilinear.argtypes = [ctypes.c_void_p, c_u32, c_u32, c_u32, c_u32, c_f32, c_f32, ctypes.c_void_p]
+        _lib.kzo_vertex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_u32, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.kzo_fragment.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     return _lib
 
 
@@ -92,10 +94,30 @@ def make_state(scene):
     return st, keep
 
 
-def render(scene, threads=0):
-    """Returns (colour (H,W,S,4) u8, depth (H,W,S) f32, resolved (H,W,4) u8, stats dict)."""
-    s = scene
-    st, keep = make_state(s)
+def vertex(scene, draw_index, vi):
+    """The oracle's vertex shader on vertex `vi` of draw `draw_index`:
+    (clip position [4], varyings [8]) as float32 arrays."""
+    st, keep = make_state(scene)
+    draws = _draws(scene)
+    pos = np.zeros(4, np.float32)
+    var = np.zeros(8, np.float32)
+    lib().kzo_vertex(ctypes.byref(st), ctypes.byref(draws[draw_index]), vi, pos.ctypes.data, var.ctypes.data)
+    del keep
+    return pos, var
+
+
+def fragment(scene, var):
+    """The oracle's fragment shader on one set of varyings: unclamped RGBA."""
+    st, keep = make_state(scene)
+    v = np.zeros(8, np.float32)
+    v[:len(var)] = var
+    out = np.zeros(4, np.float32)
+    lib().kzo_fragment(ctypes.byref(st), v.ctypes.data, out.ctypes.data)
+    del keep
+    return out
+
+
+def _draws(s):
     draws = (KzoDraw * len(s.draws))()
     for i, dr in enumerate(s.draws):
         draws[i].vertices = s.vertices.ctypes.data
@@ -107,6 +129,14 @@ def render(scene, threads=0):
             m = np.ascontiguousarray(dr["mvp"], dtype=np.float32).ravel()
             for k in range(16):
                 draws[i].mvp[k] = m[k]
+    return draws
+
+
+def render(scene, threads=0):
+    """Returns (colour (H,W,S,4) u8, depth (H,W,S) f32, resolved (H,W,4) u8, stats dict)."""
+    s = scene
+    st, keep = make_state(s)
+    draws = _draws(s)
     W, H, S = s.width, s.height, s.samples
     color = np.zeros((H, W, S, 4), np.uint8)
     depth = np.zeros((H, W, S), np.float32)

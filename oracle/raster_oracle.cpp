@@ -425,6 +425,15 @@ void kzo_bilinear(const uint8_t *tex, uint32_t w, uint32_t h, uint32_t addr_u, u
     }
 }
 
+void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4], float var[8]) {
+    Vtx v{};
+    run_vs(*st, *d, vi, v);
+    for (int i = 0; i < 4; ++i) pos[i] = v.pos[i];
+    for (int i = 0; i < kMaxVar; ++i) var[i] = v.var[i];
+}
+
+void kzo_fragment(const KzoState *st, const float *var, float out[4]) { run_fs(*st, var, out); }
+
 int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8_t *color, float *depth,
                uint8_t *resolved, KzoStats *stats, int threads) {
     const KzoState &st = *stp;

@@ -132,6 +132,11 @@ float kzo_i64_to_f32(int64_t v);
 void kzo_bilinear(const uint8_t *tex, uint32_t w, uint32_t h, uint32_t addr_u,
                   uint32_t addr_v, float u, float v, float out[4]);
 uint32_t kzo_encode_unorm8(float v);
+/* The vertex shader of st->vs_kind on vertex `vi` of draw `d` (clip-space
+ * position and up to 8 varyings), and the fragment shader of st->fs_kind on
+ * interpolated varyings (unclamped RGBA). */
+void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4], float var[8]);
+void kzo_fragment(const KzoState *st, const float *var, float out[4]);
 
 #ifdef __cplusplus
 }
