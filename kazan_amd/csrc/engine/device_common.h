@@ -270,6 +270,43 @@ enum {
 __device__ __forceinline__ float kz_fs_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float kz_fs_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Shader integer and float helpers with SPIR-V's definitions where C++
+// leaves the result undefined: division by zero gives 0 (SPIR-V leaves it
+// undefined; this keeps the GPU from computing garbage slowly), INT_MIN / -1
+// wraps, shift counts use their low 5 bits, out-of-range float -> unsigned
+// conversions saturate.
+__device__ __forceinline__ int32_t kz_sdiv(int32_t a, int32_t b) {
+    return b == 0 ? 0 : (b == -1 ? (int32_t)(0u - (uint32_t)a) : a / b);
+}
+__device__ __forceinline__ int32_t kz_udiv(int32_t a, int32_t b) {
+    return b == 0 ? 0 : (int32_t)((uint32_t)a / (uint32_t)b);
+}
+__device__ __forceinline__ int32_t kz_srem(int32_t a, int32_t b) { return (b == 0 || b == -1) ? 0 : a % b; }
+__device__ __forceinline__ int32_t kz_smod(int32_t a, int32_t b) {
+    const int32_t r = kz_srem(a, b);
+    return (r != 0 && ((r < 0) != (b < 0))) ? r + b : r;
+}
+__device__ __forceinline__ int32_t kz_umod(int32_t a, int32_t b) {
+    return b == 0 ? 0 : (int32_t)((uint32_t)a % (uint32_t)b);
+}
+__device__ __forceinline__ int32_t kz_shl(int32_t a, int32_t b) { return (int32_t)((uint32_t)a << (b & 31)); }
+__device__ __forceinline__ int32_t kz_lshr(int32_t a, int32_t b) { return (int32_t)((uint32_t)a >> (b & 31)); }
+__device__ __forceinline__ int32_t kz_ashr(int32_t a, int32_t b) { return a >> (b & 31); }
+__device__ __forceinline__ int32_t kz_smin(int32_t a, int32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int32_t kz_smax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int32_t kz_umin(int32_t a, int32_t b) { return (uint32_t)a < (uint32_t)b ? a : b; }
+__device__ __forceinline__ int32_t kz_umax(int32_t a, int32_t b) { return (uint32_t)a > (uint32_t)b ? a : b; }
+__device__ __forceinline__ int32_t kz_sclamp(int32_t x, int32_t lo, int32_t hi) { return kz_smin(kz_smax(x, lo), hi); }
+__device__ __forceinline__ int32_t kz_uclamp(int32_t x, int32_t lo, int32_t hi) { return kz_umin(kz_umax(x, lo), hi); }
+__device__ __forceinline__ int32_t kz_sabs(int32_t a) { return a < 0 ? (int32_t)(0u - (uint32_t)a) : a; }
+__device__ __forceinline__ float kz_fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x); }
+__device__ __forceinline__ float kz_fmod_glsl(float x, float y) { return x - y * __builtin_floorf(x / y); }
+__device__ __forceinline__ uint32_t kz_f2u(float x) {
+    return !(x > 0.0f) ? 0u : (x >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)x);
+}
+// A shader array's run-time word index, clamped into its n words.
+__device__ __forceinline__ int32_t kz_word_index(int32_t i, int32_t n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
 // Tile t's list: its offset and count. A tile flagged by a cut (binning
 // dropped the pairs of triangles past the list capacity, some of them in
 // this tile) reads as offset list_capacity with at least one entry: the

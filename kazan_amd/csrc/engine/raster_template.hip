@@ -11,7 +11,7 @@
 //   KZ_DST_ALPHA, KZ_ALPHA_OP, KZ_WRITE_MASK
 //   void kz_vertex(const KzFrame&, const KzDraw&, uint32_t vertex_index,
 //                  float pos[4], float vary[KZ_NV]);
-//   void kz_fragment(const KzFrame&, const float vary[KZ_NV], float out[4]);
+//   bool kz_fragment(const KzFrame&, const float vary[KZ_NV], float out[4]);  (false: discarded)
 //
 // The rasterisation rules restated here are the Vulkan 1.1 polygon rules at
 // Kazan's advertised precision (vulkan-driver/src/api_impl.rs:1559-1568,
@@ -871,7 +871,9 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
                 for (int i = 0; i < KZ_NV; ++i) vary[i] = (a0[i] + b1 * da1[i]) + b2 * da2[i];
                 const float zc = fminf(fmaxf((z0 + l1 * dz1) + l2 * dz2, f.z_lo), f.z_hi);
                 const float frag_coord[4] = {(float)px + 0.5f, (float)py + 0.5f, zc, q};
-                kz_fragment(f, vary, frag_coord, col);
+                // a discarded fragment takes no part in the depth test,
+                // depth write or blend
+                if (!kz_fragment(f, vary, frag_coord, col)) maybe = 0;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) col[c] = kz_clamp01(col[c]);
             }

@@ -1,15 +1,25 @@
 // backend_hip.cpp — shader IR -> HIP device functions.
 //
-// Every IR value becomes one C++ local per component; every pointer is
-// resolved at compile time to (storage root, constant byte offset), since the
-// translator only produces constant access chains. Float operations are
-// emitted one rounding at a time in IR order and the kernel is compiled with
-// -ffp-contract=off and IEEE division / square root, so the generated code
-// computes exactly what the IR (and the CPU oracle's restatement) says.
+// Every IR value becomes one C++ variable per component, declared at the top
+// of the stage function; every pointer is resolved at compile time to a
+// storage root plus a constant byte offset, plus a run-time word index for
+// dynamically indexed access chains. Structured control flow lowers to
+// labels and gotos: a Block's breaks jump to a label right after it, a
+// Loop's continues to a label right before it, a Switch to the chosen
+// block's label. Returning from the stage function is a break from block 0;
+// the fragment stage returns false when it discards (OpKill), the planned
+// ABI's bool result (docs/Example Translation from SPIR-V to LLVM IR.md
+// :190-198). Float operations are emitted one rounding at a time in IR
+// order and the kernel is compiled with -ffp-contract=off and IEEE division /
+// square root, so the generated code computes exactly what the IR (and the
+// CPU oracle's restatement) says.
 #include "backend_hip.h"
+
+#include "../engine/device_common.h"
 
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <set>
 #include <sstream>
 
@@ -27,6 +37,7 @@ struct PtrRef {
     Root root = Root::None;
     uint32_t index = 0;   // ubo/texture slot or local number
     uint32_t offset = 0;  // bytes
+    std::string dyn;      // run-time word index added to offset / 4 ("" = none)
 };
 
 std::string flit(uint32_t bits) {
@@ -43,6 +54,11 @@ std::string flit(uint32_t bits) {
     return b;
 }
 
+uint32_t block_words_of(const ir::InterfaceBlock &b) {
+    uint32_t w = (b.size + 3) / 4;
+    return w < 4 ? 4 : w;
+}
+
 struct StageGen {
     const ir::Module &m;
     const PipelineCodegenInput &pin;
@@ -50,6 +66,8 @@ struct StageGen {
     std::ostringstream os;
     std::map<uint32_t, PtrRef> ptrs;
     std::map<uint32_t, uint32_t> local_index;  // pointer value -> local number
+    std::vector<uint32_t> local_words;         // words of each local
+    uint32_t in_words = 4, out_words = 4;
     bool is_fs;
 
     StageGen(const ir::Module &mm, const PipelineCodegenInput &p, std::string &e, bool fs)
@@ -88,28 +106,39 @@ struct StageGen {
         }
     }
 
+    // values are declared at the top of the function (structured code
+    // jumps between scopes) and assigned where the IR computes them
     std::string lhs(uint32_t id, int c) {
         const ir::Value &v = m.value(id);
-        return "const " + ctype(v.type.scalar) + " " +
-               (v.type.lanes > 1 ? "v" + std::to_string(id) + "_" + std::to_string(c) : "v" + std::to_string(id));
+        return v.type.lanes > 1 ? "v" + std::to_string(id) + "_" + std::to_string(c) : "v" + std::to_string(id);
+    }
+
+    // A word index: constant, or constant + run-time part clamped into an
+    // array of n words (an out-of-range index of a shader's own arrays
+    // stays inside them instead of touching other memory).
+    static std::string index_expr(uint32_t w, const std::string &dyn, uint32_t n) {
+        if (dyn.empty()) return std::to_string(w);
+        return "kz_word_index(" + std::to_string(w) + dyn + ", " + std::to_string(n) + ")";
     }
 
     // the storage expression of 32-bit word `w` of a root, as float bits
     bool word_expr(const PtrRef &p, uint32_t w, bool for_store, std::string &out) {
         const std::string ws = std::to_string(w);
         switch (p.root) {
-        case Root::In: out = "kz_in[" + ws + "]"; return true;
-        case Root::Out: out = "kz_out[" + ws + "]"; return true;
-        case Root::Position: out = "kz_pos[" + ws + "]"; return true;
-        case Root::Local: out = "kz_local" + std::to_string(p.index) + "[" + ws + "]"; return true;
+        case Root::In: out = "kz_in[" + index_expr(w, p.dyn, in_words) + "]"; return true;
+        case Root::Out: out = "kz_out[" + index_expr(w, p.dyn, out_words) + "]"; return true;
+        case Root::Position: out = "kz_pos[" + index_expr(w, p.dyn, 4) + "]"; return true;
+        case Root::Local:
+            out = "kz_local" + std::to_string(p.index) + "[" + index_expr(w, p.dyn, local_words.at(p.index)) + "]";
+            return true;
         case Root::Push:
             if (for_store) return fail("store to push constants");
             if (is_fs) return fail("push constants in a fragment shader are not supported");
-            out = "__uint_as_float(d.push[" + ws + "])";
+            out = "__uint_as_float(d.push[" + index_expr(w, p.dyn, KZ_PUSH_WORDS) + "])";
             return true;
         case Root::Ubo:
             if (for_store) return fail("store to a uniform buffer");
-            out = "((const float *)f.ubo[" + std::to_string(p.index) + "])[" + ws + "]";
+            out = "((const float *)f.ubo[" + std::to_string(p.index) + "])[" + ws + p.dyn + "]";
             return true;
         case Root::VertexIndex:
             out = "__int_as_float((int32_t)kz_vertex_index)";
@@ -125,6 +154,16 @@ struct StageGen {
         }
     }
 
+    void lanes_expr(uint32_t res, const std::function<std::string(int)> &e) {
+        for (int c = 0; c < m.value(res).type.lanes; ++c) os << "    " << lhs(res, c) << " = " << e(c) << ";\n";
+    }
+
+    bool emit_block(uint32_t b) {
+        for (const auto &ins : m.main.blocks.at(b).body)
+            if (!emit_ins(ins)) return false;
+        return true;
+    }
+
     bool emit_ins(const ir::Instruction &ins) {
         auto res = [&]() { return ins.results.at(0); };
         auto lanes = [&](uint32_t id) { return (int)m.value(id).type.lanes; };
@@ -137,6 +176,46 @@ struct StageGen {
             ptrs[res()] = p;
             return true;
         }
+        case Op::PtrIndex: {
+            auto it = ptrs.find(ins.args[0]);
+            if (it == ptrs.end()) return fail("pointer index from an unknown pointer");
+            if (ins.fields[0] % 4) return fail("pointer index stride is not a whole number of words");
+            PtrRef p = it->second;
+            if (p.root == Root::Texture) return fail("indexed texture");
+            p.dyn += " + " + comp(ins.args[1], 0) + " * " + std::to_string(ins.fields[0] / 4);
+            ptrs[res()] = p;
+            return true;
+        }
+        case Op::Block:
+            os << "    {\n";
+            if (!emit_block(ins.fields[0])) return false;
+            os << "    }\n    kzb" << ins.fields[0] << ":;\n";
+            return true;
+        case Op::Loop:
+            os << "    kzl" << ins.fields[0] << ":;\n    {\n";
+            if (!emit_block(ins.fields[0])) return false;
+            os << "    }\n    goto kzl" << ins.fields[0] << ";\n";
+            return true;
+        case Op::BreakBlock:
+            if (ins.fields.at(0) == 0) os << "    return true;\n";
+            else os << "    goto kzb" << ins.fields[0] << ";\n";
+            return true;
+        case Op::ContinueLoop: os << "    goto kzl" << ins.fields.at(0) << ";\n"; return true;
+        case Op::Switch: {
+            const Scalar st = m.value(ins.args[0]).type.scalar;
+            os << "    switch ((int32_t)" << comp(ins.args[0], 0) << ") {\n";
+            for (size_t i = 1; i + 1 < ins.fields.size(); i += 2) {
+                const uint32_t lit = st == Scalar::Bool ? (ins.fields[i] != 0) : ins.fields[i];
+                os << "    case (int32_t)0x" << std::hex << lit << std::dec << "u: goto kzb" << ins.fields[i + 1] << ";\n";
+            }
+            os << "    default: goto kzb" << ins.fields.at(0) << ";\n    }\n";
+            return true;
+        }
+        case Op::Kill:
+            if (!is_fs) return fail("discard outside a fragment shader");
+            os << "    return false;\n";
+            return true;
+        case Op::Unreachable: os << "    return true;\n"; return true;
         case Op::Load: {
             auto it = ptrs.find(ins.args[0]);
             if (it == ptrs.end()) return fail("load from an unknown pointer");
@@ -179,13 +258,145 @@ struct StageGen {
             [[fallthrough]];
         case Op::FAdd:
         case Op::FSub:
-        case Op::FMul:
-        case Op::Add: {
+        case Op::FMul: {
             const char *o = ins.op == Op::FSub ? " - " : ins.op == Op::FMul ? " * " : ins.op == Op::FDiv ? " / " : " + ";
             for (int c = 0; c < lanes(res()); ++c)
                 os << "    " << lhs(res(), c) << " = " << comp(ins.args[0], c) << o << comp(ins.args[1], c) << ";\n";
             return true;
         }
+        // integer arithmetic wraps (two's complement), as SPIR-V defines it
+        case Op::Add:
+        case Op::ISub:
+        case Op::IMul: {
+            const char *o = ins.op == Op::Add ? " + " : ins.op == Op::ISub ? " - " : " * ";
+            const auto a = ins.args[0], b = ins.args[1];
+            lanes_expr(res(), [&](int c) { return "(int32_t)((uint32_t)" + comp(a, c) + o + "(uint32_t)" + comp(b, c) + ")"; });
+            return true;
+        }
+        case Op::SDiv:
+        case Op::UDiv:
+        case Op::SRem:
+        case Op::SMod:
+        case Op::UMod:
+        case Op::Shl:
+        case Op::ShrL:
+        case Op::ShrA:
+        case Op::SMin:
+        case Op::SMax:
+        case Op::UMin:
+        case Op::UMax:
+        case Op::FMod:
+        case Op::FRem:
+        case Op::Pow: {
+            static const std::map<Op, const char *> fn = {
+                {Op::SDiv, "kz_sdiv"}, {Op::UDiv, "kz_udiv"}, {Op::SRem, "kz_srem"}, {Op::SMod, "kz_smod"},
+                {Op::UMod, "kz_umod"}, {Op::Shl, "kz_shl"}, {Op::ShrL, "kz_lshr"}, {Op::ShrA, "kz_ashr"},
+                {Op::SMin, "kz_smin"}, {Op::SMax, "kz_smax"}, {Op::UMin, "kz_umin"}, {Op::UMax, "kz_umax"},
+                {Op::FMod, "kz_fmod_glsl"}, {Op::FRem, "__builtin_fmodf"}, {Op::Pow, "__builtin_powf"}};
+            const auto a = ins.args[0], b = ins.args[1];
+            const std::string f = fn.at(ins.op);
+            lanes_expr(res(), [&](int c) { return f + "(" + comp(a, c) + ", " + comp(b, c) + ")"; });
+            return true;
+        }
+        case Op::SClamp:
+        case Op::UClamp: {
+            const std::string f = ins.op == Op::SClamp ? "kz_sclamp" : "kz_uclamp";
+            lanes_expr(res(), [&](int c) {
+                return f + "(" + comp(ins.args[0], c) + ", " + comp(ins.args[1], c) + ", " + comp(ins.args[2], c) + ")";
+            });
+            return true;
+        }
+        case Op::And:
+        case Op::Or:
+        case Op::Xor: {
+            const bool bl = m.value(ins.args[0]).type.scalar == Scalar::Bool;
+            const char *o = ins.op == Op::And ? (bl ? " && " : " & ") : ins.op == Op::Or ? (bl ? " || " : " | ")
+                                                                                          : (bl ? " != " : " ^ ");
+            lanes_expr(res(), [&](int c) { return "(" + comp(ins.args[0], c) + o + comp(ins.args[1], c) + ")"; });
+            return true;
+        }
+        case Op::Not: {
+            const bool bl = m.value(ins.args[0]).type.scalar == Scalar::Bool;
+            lanes_expr(res(), [&](int c) { return std::string(bl ? "!" : "~") + comp(ins.args[0], c); });
+            return true;
+        }
+        case Op::INeg:
+            lanes_expr(res(), [&](int c) { return "(int32_t)(0u - (uint32_t)" + comp(ins.args[0], c) + ")"; });
+            return true;
+        case Op::SAbs:
+            lanes_expr(res(), [&](int c) { return "kz_sabs(" + comp(ins.args[0], c) + ")"; });
+            return true;
+        case Op::FSign:
+        case Op::Trunc:
+        case Op::RoundEven:
+        case Op::Exp2:
+        case Op::Log2:
+        case Op::Exp:
+        case Op::Log:
+        case Op::Sin:
+        case Op::Cos: {
+            static const std::map<Op, const char *> fn = {
+                {Op::FSign, "kz_fsign"}, {Op::Trunc, "__builtin_truncf"}, {Op::RoundEven, "__builtin_rintf"},
+                {Op::Exp2, "__builtin_exp2f"}, {Op::Log2, "__builtin_log2f"}, {Op::Exp, "__builtin_expf"},
+                {Op::Log, "__builtin_logf"}, {Op::Sin, "__builtin_sinf"}, {Op::Cos, "__builtin_cosf"}};
+            const std::string f = fn.at(ins.op);
+            lanes_expr(res(), [&](int c) { return f + "(" + comp(ins.args[0], c) + ")"; });
+            return true;
+        }
+        case Op::FCmp: {
+            static const char *const form[] = {
+                "(%a == %b)", "(%a < %b || %a > %b)", "(%a < %b)", "(%a <= %b)", "(%a > %b)", "(%a >= %b)",
+                "!(%a < %b || %a > %b)", "(%a != %b)", "!(%a >= %b)", "!(%a > %b)", "!(%a <= %b)", "!(%a < %b)",
+                "(%a == %a && %b == %b)", "(%a != %a || %b != %b)"};
+            if (ins.fields.at(0) >= sizeof form / sizeof form[0]) return fail("float comparison predicate");
+            lanes_expr(res(), [&](int c) {
+                std::string e = form[ins.fields[0]];
+                for (size_t k; (k = e.find("%a")) != std::string::npos;) e.replace(k, 2, comp(ins.args[0], c));
+                for (size_t k; (k = e.find("%b")) != std::string::npos;) e.replace(k, 2, comp(ins.args[1], c));
+                return e;
+            });
+            return true;
+        }
+        case Op::ICmp: {
+            static const char *const op[] = {" == ", " != ", " < ", " <= ", " > ", " >= ", " < ", " <= ", " > ", " >= "};
+            const uint32_t pr = ins.fields.at(0);
+            if (pr >= sizeof op / sizeof op[0]) return fail("integer comparison predicate");
+            const bool u = pr >= ir::IULt;
+            lanes_expr(res(), [&](int c) {
+                const std::string a = comp(ins.args[0], c), b = comp(ins.args[1], c);
+                return u ? "((uint32_t)" + a + op[pr] + "(uint32_t)" + b + ")" : "(" + a + op[pr] + b + ")";
+            });
+            return true;
+        }
+        case Op::ConvUToF:
+            lanes_expr(res(), [&](int c) { return "(float)(uint32_t)" + comp(ins.args[0], c); });
+            return true;
+        case Op::ConvFToU:
+            lanes_expr(res(), [&](int c) { return "(int32_t)kz_f2u(" + comp(ins.args[0], c) + ")"; });
+            return true;
+        case Op::Bitcast: {
+            const Scalar from = m.value(ins.args[0]).type.scalar, to = m.value(res()).type.scalar;
+            lanes_expr(res(), [&](int c) {
+                const std::string a = comp(ins.args[0], c);
+                if (from == to) return a;
+                return to == Scalar::F32 ? "__int_as_float(" + a + ")" : "__float_as_int(" + a + ")";
+            });
+            return true;
+        }
+        case Op::Any:
+        case Op::All: {
+            std::string e;
+            for (int c = 0; c < lanes(ins.args[0]); ++c)
+                e += (c ? (ins.op == Op::Any ? " || " : " && ") : "") + comp(ins.args[0], c);
+            os << "    " << lhs(res(), 0) << " = (" << e << ");\n";
+            return true;
+        }
+        case Op::IsNan:
+            lanes_expr(res(), [&](int c) { return "(" + comp(ins.args[0], c) + " != " + comp(ins.args[0], c) + ")"; });
+            return true;
+        case Op::IsInf:
+            lanes_expr(res(), [&](int c) { return "(__builtin_fabsf(" + comp(ins.args[0], c) + ") == __builtin_inff())"; });
+            return true;
         case Op::FNeg:
             for (int c = 0; c < lanes(res()); ++c) os << "    " << lhs(res(), c) << " = -" << comp(ins.args[0], c) << ";\n";
             return true;
@@ -265,8 +476,6 @@ struct StageGen {
                 os << "    " << lhs(res(), c) << " = " << comp(ins.args[0], c) << (ins.op == Op::FOrdLt ? " < " : " > ")
                    << comp(ins.args[1], c) << ";\n";
             return true;
-        case Op::Kill: return fail("fragment discard (OpKill) is not supported yet");
-        case Op::BreakBlock: return true;
         }
         return fail("unknown IR instruction");
     }
@@ -304,24 +513,36 @@ struct StageGen {
             for (const auto &v : vars) {
                 const uint32_t words = (std::max(v.size, 4u) + 3) / 4;
                 decl << "    float kz_local" << nlocal << "[" << words << "] = {};\n";
-                ptrs[v.pointer] = {Root::Local, nlocal, 0};
+                ptrs[v.pointer] = {Root::Local, nlocal, 0, ""};
+                local_words.push_back(words);
                 ++nlocal;
             }
         };
         add_locals(m.invocation_globals);
         add_locals(m.main.locals);
+        // every value the function computes, declared up front
+        for (const auto &blk : m.main.blocks)
+            for (const auto &ins : blk.body)
+                for (uint32_t r : ins.results) {
+                    const ir::Value &v = m.value(r);
+                    if (v.type.scalar == Scalar::DataPtr || v.type.scalar == Scalar::Texture) continue;
+                    decl << "    " << ctype(v.type.scalar);
+                    for (int c = 0; c < v.type.lanes; ++c) decl << (c ? ", " : " ") << lhs(r, c);
+                    decl << ";\n";
+                }
+        in_words = block_words_of(m.user_inputs);
+        out_words = block_words_of(m.user_outputs);
         if (is_fs)
-            os << "__device__ __forceinline__ void " << fn_name
+            os << "__device__ __forceinline__ bool " << fn_name
                << "(const KzFrame &f, const float *kz_in, float *kz_out, const float *kz_frag_coord) {\n";
         else
-            os << "__device__ __forceinline__ void " << fn_name
+            os << "__device__ __forceinline__ bool " << fn_name
                << "(const KzFrame &f, const KzDraw &d, uint32_t kz_vertex_index, const float *kz_in, float *kz_out, "
                   "float *kz_pos) {\n";
         os << "    (void)f; (void)kz_in; (void)kz_out;\n";
         os << decl.str();
-        for (const auto &ins : m.main.body.body)
-            if (!emit_ins(ins)) return false;
-        os << "}\n\n";
+        if (!emit_block(0)) return false;
+        os << "    return true;\n}\n\n";
         return true;
     }
 };
@@ -426,7 +647,8 @@ bool generate_pipeline_source(const PipelineCodegenInput &in, PipelineCodegenOut
     }
     os << "    (void)vary;\n}\n\n";
 
-    os << "__device__ __forceinline__ void kz_fragment(const KzFrame &f, const float *vary, const float *frag_coord, "
+    // false: the fragment shader discarded the fragment
+    os << "__device__ __forceinline__ bool kz_fragment(const KzFrame &f, const float *vary, const float *frag_coord, "
           "float o[4]) {\n";
     if (in.fs) {
         const uint32_t fin_words = block_words(in.fs->user_inputs), fout_words = block_words(in.fs->user_outputs);
@@ -435,10 +657,12 @@ bool generate_pipeline_source(const PipelineCodegenInput &in, PipelineCodegenOut
         for (size_t j = 0; j < slots.size(); ++j) os << "    kz_in[" << slots[j] << "] = vary[" << j << "];\n";
         os << "    float kz_out[" << fout_words << "];\n";
         os << "    for (int i = 0; i < " << fout_words << "; ++i) kz_out[i] = 0.0f;\n";
-        os << "    kz_fs_main(f, kz_in, kz_out, frag_coord);\n";
+        os << "    const bool keep = kz_fs_main(f, kz_in, kz_out, frag_coord);\n";
         os << "    o[0] = kz_out[0]; o[1] = kz_out[1]; o[2] = kz_out[2]; o[3] = kz_out[3];\n";
+        os << "    return keep;\n";
     } else {
         os << "    (void)f; (void)vary; (void)frag_coord; o[0] = o[1] = o[2] = o[3] = 0.0f;\n";
+        os << "    return true;\n";
     }
     os << "}\n";
     out.source = os.str();

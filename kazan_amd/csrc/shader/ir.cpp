@@ -63,8 +63,68 @@ const char *op_text(Op op) {
     case Op::FOrdGt: return "ford_gt";
     case Op::Kill: return "kill";
     case Op::BreakBlock: return "break";
+    case Op::ISub: return "sub";
+    case Op::IMul: return "mul";
+    case Op::SDiv: return "sdiv";
+    case Op::UDiv: return "udiv";
+    case Op::SRem: return "srem";
+    case Op::SMod: return "smod";
+    case Op::UMod: return "umod";
+    case Op::Shl: return "shl";
+    case Op::ShrL: return "lshr";
+    case Op::ShrA: return "ashr";
+    case Op::And: return "and";
+    case Op::Or: return "or";
+    case Op::Xor: return "xor";
+    case Op::Not: return "not";
+    case Op::INeg: return "neg";
+    case Op::SMin: return "smin";
+    case Op::SMax: return "smax";
+    case Op::UMin: return "umin";
+    case Op::UMax: return "umax";
+    case Op::SAbs: return "sabs";
+    case Op::SClamp: return "sclamp";
+    case Op::UClamp: return "uclamp";
+    case Op::FMod: return "fmod";
+    case Op::FRem: return "frem";
+    case Op::FSign: return "fsign";
+    case Op::Trunc: return "trunc";
+    case Op::RoundEven: return "round_even";
+    case Op::Exp2: return "exp2";
+    case Op::Log2: return "log2";
+    case Op::Exp: return "exp";
+    case Op::Log: return "log";
+    case Op::Pow: return "pow";
+    case Op::Sin: return "sin";
+    case Op::Cos: return "cos";
+    case Op::FCmp: return "fcmp";
+    case Op::ICmp: return "icmp";
+    case Op::ConvUToF: return "convert_u_to_f";
+    case Op::ConvFToU: return "convert_f_to_u";
+    case Op::Bitcast: return "bitcast";
+    case Op::Any: return "any";
+    case Op::All: return "all";
+    case Op::IsNan: return "is_nan";
+    case Op::IsInf: return "is_inf";
+    case Op::PtrIndex: return "ptr_index";
+    case Op::Block: return "block";
+    case Op::Loop: return "loop";
+    case Op::ContinueLoop: return "continue";
+    case Op::Switch: return "branch";
+    case Op::Unreachable: return "unreachable";
     }
     return "?";
+}
+
+const char *fcmp_text(uint32_t p) {
+    static const char *const k[] = {"oeq", "one", "olt", "ole", "ogt", "oge", "ueq",
+                                     "une", "ult", "ule", "ugt", "uge", "ord", "uno"};
+    return p < sizeof k / sizeof k[0] ? k[p] : "?";
+}
+
+const char *icmp_text(uint32_t p) {
+    static const char *const k[] = {"eq", "ne", "slt", "sle", "sgt", "sge", "ult", "ule", "ugt", "uge"};
+    return p < sizeof k / sizeof k[0] ? k[p] : "?";
 }
 
 struct Printer {
@@ -157,9 +217,20 @@ uint32_t Module::new_const_i32(int32_t i) {
 
 std::string instruction_text(const Module &m, const Instruction &ins) {
     std::string s = op_text(ins.op);
-    if (ins.op == Op::BreakBlock) {
-        s += " " + m.main.body.name + "[]";
-    } else {
+    auto block_name = [&](uint32_t b) { return b < m.main.blocks.size() ? m.main.blocks[b].name : "?"; };
+    switch (ins.op) {
+    case Op::BreakBlock:
+    case Op::ContinueLoop: s += " " + block_name(ins.fields.at(0)) + "[]"; break;
+    case Op::Kill:
+    case Op::Unreachable: break;
+    case Op::Switch: {
+        s += " " + value_use(m, ins.args.at(0)) + " {";
+        for (size_t i = 1; i + 1 < ins.fields.size(); i += 2)
+            s += " " + hex(ins.fields[i]) + " -> " + block_name(ins.fields[i + 1]) + ",";
+        s += " default -> " + block_name(ins.fields.at(0)) + " }";
+        break;
+    }
+    default: {
         s += " [";
         for (size_t i = 0; i < ins.args.size(); ++i) {
             if (i) s += ", ";
@@ -170,7 +241,10 @@ std::string instruction_text(const Module &m, const Instruction &ins) {
         case Op::Load:
         case Op::Store: s += ", align: " + hex(ins.fields.empty() ? 4 : ins.fields[0]); break;
         case Op::PtrOffset: s += ", offset: " + hex(ins.fields[0]); break;
+        case Op::PtrIndex: s += ", stride: " + hex(ins.fields[0]); break;
         case Op::Extract: s += ", index: " + std::to_string(ins.fields[0]); break;
+        case Op::FCmp: s += ", " + std::string(fcmp_text(ins.fields[0])); break;
+        case Op::ICmp: s += ", " + std::string(icmp_text(ins.fields[0])); break;
         case Op::Shuffle: {
             s += ", lanes: [";
             for (size_t i = 0; i < ins.fields.size(); ++i) s += (i ? ", " : "") + std::to_string(ins.fields[i]);
@@ -187,10 +261,49 @@ std::string instruction_text(const Module &m, const Instruction &ins) {
         }
         s += "]";
     }
+    }
     if (ins.has_location)
         s += " @ \"" + ins.loc.file + "\":" + std::to_string(ins.loc.line) + ":" + std::to_string(ins.loc.column);
     return s;
 }
+
+namespace {
+
+// Block and Loop print their bodies nested, as the reference's text form
+// does (block.rs: `block NAME -> [] { ... }`, `loop NAME[] -> [] { -> [];
+// BODY { ... } }`).
+void print_body(Printer &p, const Module &m, uint32_t b);
+
+void print_instruction(Printer &p, const Module &m, const Instruction &ins) {
+    if (ins.op == Op::Block) {
+        const uint32_t b = ins.fields.at(0);
+        p.line("block " + m.main.blocks[b].name + " -> [] {");
+        ++p.indent;
+        print_body(p, m, b);
+        --p.indent;
+        p.line("};");
+    } else if (ins.op == Op::Loop) {
+        const uint32_t b = ins.fields.at(0);
+        p.line("loop " + m.main.blocks[b].name + "[] -> [] {");
+        ++p.indent;
+        p.line("-> [];");
+        p.line(m.main.blocks[b].name + " {");
+        ++p.indent;
+        print_body(p, m, b);
+        --p.indent;
+        p.line("}");
+        --p.indent;
+        p.line("};");
+    } else {
+        p.line(instruction_text(m, ins) + ";");
+    }
+}
+
+void print_body(Printer &p, const Module &m, uint32_t b) {
+    for (const auto &ins : m.main.blocks[b].body) print_instruction(p, m, ins);
+}
+
+}  // namespace
 
 std::string module_text(const Module &m) {
     Printer p;
@@ -249,9 +362,9 @@ std::string module_text(const Module &m) {
     for (const auto &v : m.main.locals) p.line(variable_text(m, v.type, v.alignment, v.pointer) + ";");
     --p.indent;
     p.line("}");
-    p.line(m.main.body.name + " {");
+    p.line(m.main.body().name + " {");
     ++p.indent;
-    for (const auto &ins : m.main.body.body) p.line(instruction_text(m, ins) + ";");
+    print_body(p, m, 0);
     --p.indent;
     p.line("}");
     --p.indent;

@@ -77,9 +77,63 @@ enum class Op : uint8_t {
     Select,   // [cond, a, b]
     FOrdLt,
     FOrdGt,
-    Kill,     // discard the fragment
-    BreakBlock,
+    Kill,     // discard the fragment (ends the invocation)
+    BreakBlock,  // field: target block; jumps to the first instruction after that block
+    // integer arithmetic (And, Or, Xor, Not also on bool: the logical forms)
+    ISub,
+    IMul,
+    SDiv,
+    UDiv,
+    SRem,     // sign of the dividend
+    SMod,     // sign of the divisor
+    UMod,
+    Shl,
+    ShrL,
+    ShrA,
+    And,
+    Or,
+    Xor,
+    Not,
+    INeg,
+    SMin,
+    SMax,
+    UMin,
+    UMax,
+    SAbs,
+    SClamp,
+    UClamp,
+    // float
+    FMod,     // x - y * floor(x / y)
+    FRem,     // sign of x
+    FSign,
+    Trunc,
+    RoundEven,
+    Exp2,
+    Log2,
+    Exp,
+    Log,
+    Pow,
+    Sin,
+    Cos,
+    FCmp,     // field: predicate (FCmpPred); result bool lanes
+    ICmp,     // field: predicate (ICmpPred); result bool lanes
+    ConvUToF,
+    ConvFToU,
+    Bitcast,
+    Any,
+    All,
+    IsNan,
+    IsInf,
+    PtrIndex,  // [ptr, index] + field: stride in bytes -> ptr + index * stride
+    Block,     // field: block; runs the block's body
+    Loop,      // field: the loop's body block; runs it again on each continue
+    ContinueLoop,  // field: the loop's body block
+    Switch,    // [selector] + fields: default block, then (literal, block) pairs: breaks from the chosen block
+    Unreachable,
 };
+
+enum FCmpPred : uint32_t { FOEq, FONe, FOLt, FOLe, FOGt, FOGe, FUEq, FUNe, FULt, FULe, FUGt, FUGe, FOrd, FUno };
+enum ICmpPred : uint32_t { IEq, INe, ISLt, ISLe, ISGt, ISGe, IULt, IULe, IUGt, IUGe };
 
 struct Location {
     std::string file;
@@ -131,15 +185,27 @@ struct Variable {
     uint32_t size = 0;  // bytes of storage
 };
 
+// A block of structured code. The reference IR nests blocks inside the
+// instructions that own them (shader-compiler-ir/src/block.rs:76-184); here
+// a function keeps them in one table and Block / Loop / BreakBlock /
+// ContinueLoop / Switch refer to them by index. Block 0 is the function's
+// body: breaking from it returns.
 struct Block {
     std::string name;
     std::vector<Instruction> body;
+    bool loop_body = false;  // the body of a Loop (its name is the loop's)
 };
 
 struct Function {
     std::string name;
     std::vector<Variable> locals;
-    Block body;
+    std::vector<Block> blocks{Block{}};
+    Block &body() { return blocks[0]; }
+    const Block &body() const { return blocks[0]; }
+    uint32_t new_block(const std::string &name, bool loop_body = false) {
+        blocks.push_back(Block{name, {}, loop_body});
+        return (uint32_t)blocks.size() - 1;
+    }
 };
 
 enum class Stage : uint8_t { Vertex, Fragment, Compute };

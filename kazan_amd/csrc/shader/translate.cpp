@@ -1,6 +1,7 @@
 // translate.cpp — SPIR-V -> shader IR for the draw path's shader subset.
 #include "translate.h"
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <set>
@@ -30,6 +31,18 @@ enum SOp : uint16_t {
     OpVectorTimesScalar = 142, OpMatrixTimesVector = 145, OpDot = 148, OpSelect = 169, OpFOrdLessThan = 184,
     OpFOrdGreaterThan = 186, OpNoLine = 317, OpModuleProcessed = 330, OpSelectionMerge = 247, OpLoopMerge = 246,
     OpLabel = 248, OpBranch = 249, OpBranchConditional = 250, OpKill = 252, OpReturn = 253,
+    OpUndef = 1, OpFunctionParameter = 55, OpFunctionCall = 57, OpCompositeInsert = 82, OpCopyObject = 83,
+    OpConvertFToU = 109, OpConvertUToF = 112, OpBitcast = 124, OpSNegate = 126, OpISub = 130, OpIMul = 132,
+    OpUDiv = 134, OpSDiv = 135, OpUMod = 137, OpSRem = 138, OpSMod = 139, OpFRem = 140, OpFMod = 141,
+    OpAny = 154, OpAll = 155, OpIsNan = 156, OpIsInf = 157, OpLogicalEqual = 164, OpLogicalNotEqual = 165,
+    OpLogicalOr = 166, OpLogicalAnd = 167, OpLogicalNot = 168, OpIEqual = 170, OpINotEqual = 171,
+    OpUGreaterThan = 172, OpSGreaterThan = 173, OpUGreaterThanEqual = 174, OpSGreaterThanEqual = 175,
+    OpULessThan = 176, OpSLessThan = 177, OpULessThanEqual = 178, OpSLessThanEqual = 179, OpFOrdEqual = 180,
+    OpFUnordEqual = 181, OpFOrdNotEqual = 182, OpFUnordNotEqual = 183, OpFUnordLessThan = 185,
+    OpFUnordGreaterThan = 187, OpFOrdLessThanEqual = 188, OpFUnordLessThanEqual = 189,
+    OpFOrdGreaterThanEqual = 190, OpFUnordGreaterThanEqual = 191, OpShiftRightLogical = 194,
+    OpShiftRightArithmetic = 195, OpShiftLeftLogical = 196, OpBitwiseOr = 197, OpBitwiseXor = 198,
+    OpBitwiseAnd = 199, OpNot = 200, OpPhi = 245, OpSwitch = 251, OpReturnValue = 254, OpUnreachable = 255,
 };
 
 enum Storage : uint32_t {
@@ -66,6 +79,40 @@ struct Val {
     uint32_t struct_var = 0;
 };
 
+// One basic block of a SPIR-V function: its instructions [begin, end) (the
+// OpLabel excluded, the merge instruction included as a no-op) and its
+// terminator at `end`.
+struct CfgBlock {
+    size_t begin = 0, end = 0;
+    uint32_t merge = 0, cont = 0;  // OpSelectionMerge / OpLoopMerge operands
+    bool loop = false, selection = false;
+    bool emitted = false;
+    // values stored into phi locals when control leaves this block:
+    // (phi result id, incoming value id)
+    std::vector<std::pair<uint32_t, uint32_t>> phi_out;
+};
+
+// The control-flow graph of one function being translated (the entry
+// point, or a callee being inlined).
+struct FnCfg {
+    uint32_t fn = 0, ret_type = 0;
+    std::vector<uint32_t> params;
+    std::map<uint32_t, CfgBlock> blocks;
+    std::vector<uint32_t> order;  // labels in declaration order
+    uint32_t ret_block = 0;       // IR block whose break is a return
+    uint32_t ret_local = 0;       // IR pointer receiving OpReturnValue (inlined calls)
+};
+
+// An enclosing structured construct: which branch targets are jumps out of
+// or within it, and the IR blocks those jumps break from.
+struct Construct {
+    enum Kind { Selection, Loop, Switch } kind = Selection;
+    uint32_t merge_label = 0, merge_block = 0;
+    uint32_t cont_label = 0, cont_block = 0;    // loop: continue target and the block whose end reaches it
+    uint32_t header_label = 0, loop_block = 0;  // loop: the back edge continues loop_block
+    std::vector<std::pair<uint32_t, uint32_t>> cases;  // switch: case label -> block whose end reaches it
+};
+
 struct Translator {
     const SpirvModule &sm;
     ir::Module &m;
@@ -84,6 +131,13 @@ struct Translator {
     std::map<std::pair<uint32_t, uint32_t>, uint32_t> block_builtin_ptr;
     ir::Location cur_loc;
     bool have_loc = false;
+    // structured emission state
+    uint32_t cur_block = 0;
+    FnCfg *fcfg = nullptr;
+    std::vector<Construct> constructs;
+    size_t construct_base = 0;  // constructs of the function being emitted start here
+    std::map<uint32_t, uint32_t> phi_local;  // phi result id -> IR local pointer
+    int inline_depth = 0;
 
     Translator(const SpirvModule &s, ir::Module &mm, std::string &e) : sm(s), m(mm), err(e) {}
 
@@ -200,9 +254,10 @@ struct Translator {
             ins.has_location = true;
             ins.loc = cur_loc;
         }
-        m.main.body.body.push_back(std::move(ins));
+        m.main.blocks[cur_block].body.push_back(std::move(ins));
         return r;
     }
+    void emit_jump(Op op, uint32_t block) { emit(op, {}, Type{}, {block}); }
 
     uint32_t ptr_offset(uint32_t base, uint32_t off) {
         if (off == 0) return base;
@@ -218,6 +273,17 @@ struct Translator {
     bool collect();
     bool build_interface(uint32_t entry_fn, const std::vector<uint32_t> &iface, ir::Stage stage);
     bool translate_function(uint32_t fn);
+    bool build_cfg(uint32_t fn, FnCfg &c);
+    bool emit_from(uint32_t label, bool header_entry);
+    bool emit_target(uint32_t label);
+    int try_jump(uint32_t label);
+    bool emit_if(uint32_t cond, uint32_t t, uint32_t f);
+    bool emit_switch(const CfgBlock &b, const std::vector<uint32_t> &o);
+    bool emit_phi_stores(const CfgBlock &b);
+    bool body_ins(const SpirvInstruction &ins);
+    bool call(const std::vector<uint32_t> &o);
+    uint32_t local_var(const Type &t, uint32_t bytes, const std::string &name);
+    const Val *binary_operands(const std::vector<uint32_t> &o, const Val *&b);
     bool load(uint32_t rtype, uint32_t rid, const Val &p);
     bool store(const Val &p, const Val &v);
     bool access_chain(uint32_t rid, uint32_t ptr_type, const Val &base, const std::vector<uint32_t> &idx);
@@ -572,9 +638,23 @@ bool Translator::access_chain(uint32_t rid, uint32_t ptr_type, const Val &base, 
     }
     uint32_t off = 0;
     for (size_t i = first; i < idx.size(); ++i) {
-        if (!consts.count(idx[i])) return fail("dynamic access-chain indices are not supported");
-        const uint32_t k = const_u32(idx[i]);
         const SType &t = types.at(cur);
+        if (!consts.count(idx[i])) {
+            // a run-time index: the constant offset so far, then index * stride
+            uint32_t stride = 0;
+            if (t.k == SType::Array) stride = array_stride(cur, r.layout);
+            else if (t.k == SType::Matrix) stride = r.mat_stride;
+            else if (t.k == SType::Vector) stride = 4;
+            else return fail("run-time index into a struct");
+            const Val *iv = get(idx[i]);
+            if (!iv) return false;
+            if (m.value(iv->ir[0]).type != Type::i32()) return fail("access-chain index is not a 32-bit integer");
+            ptr = emit(Op::PtrIndex, {ptr_offset(ptr, off), iv->ir[0]}, Type::ptr(), {stride});
+            off = 0;
+            cur = t.k == SType::Array || t.k == SType::Matrix || t.k == SType::Vector ? t.elem : cur;
+            continue;
+        }
+        const uint32_t k = const_u32(idx[i]);
         switch (t.k) {
         case SType::Struct:
             if (k >= t.members.size()) return fail("struct index out of range");
@@ -603,33 +683,603 @@ bool Translator::access_chain(uint32_t rid, uint32_t ptr_type, const Val &base, 
     return true;
 }
 
-bool Translator::translate_function(uint32_t fn) {
-    bool in_fn = false, seen_label = false, done = false;
-    for (const auto &ins : sm.instructions) {
-        const auto &o = ins.operands;
-        if (!in_fn) {
-            if (ins.opcode == OpFunction && o[1] == fn) {
-                in_fn = true;
-                m.main.name = "main";
+bool Translator::build_cfg(uint32_t fn, FnCfg &c) {
+    c.fn = fn;
+    const auto &ins = sm.instructions;
+    size_t i = 0;
+    while (i < ins.size() && !(ins[i].opcode == OpFunction && ins[i].operands.size() > 1 && ins[i].operands[1] == fn))
+        ++i;
+    if (i == ins.size()) return fail("function %" + std::to_string(fn) + " not found");
+    c.ret_type = ins[i].operands[0];
+    CfgBlock *cur = nullptr;
+    for (++i; i < ins.size() && ins[i].opcode != OpFunctionEnd; ++i) {
+        const auto &o = ins[i].operands;
+        switch (ins[i].opcode) {
+        case OpFunctionParameter:
+            if (!c.blocks.empty()) return fail("OpFunctionParameter after the first block");
+            c.params.push_back(o[1]);
+            break;
+        case OpLabel:
+            if (cur) return fail("block %" + std::to_string(o[0]) + " starts before the previous one ended");
+            if (c.blocks.count(o[0])) return fail("duplicate label");
+            cur = &c.blocks[o[0]];
+            cur->begin = i + 1;
+            c.order.push_back(o[0]);
+            break;
+        case OpSelectionMerge:
+            if (!cur) return fail("instruction outside a block");
+            cur->selection = true;
+            cur->merge = o[0];
+            break;
+        case OpLoopMerge:
+            if (!cur) return fail("instruction outside a block");
+            cur->loop = true;
+            cur->merge = o[0];
+            cur->cont = o[1];
+            break;
+        case OpBranch:
+        case OpBranchConditional:
+        case OpSwitch:
+        case OpReturn:
+        case OpReturnValue:
+        case OpKill:
+        case OpUnreachable:
+            if (!cur) return fail("terminator outside a block");
+            cur->end = i;
+            cur = nullptr;
+            break;
+        default:
+            if (!cur && ins[i].opcode != OpLine && ins[i].opcode != OpNoLine)
+                return fail("instruction outside a block");
+            break;
+        }
+    }
+    if (i == ins.size()) return fail("function without OpFunctionEnd");
+    if (cur) return fail("last block without a terminator");
+    if (c.order.empty()) return fail("function without blocks");
+    // phis become function locals: stored on the incoming edges, loaded
+    // where the phi stands
+    for (auto &kv : c.blocks) {
+        for (size_t k = kv.second.begin; k < kv.second.end; ++k) {
+            const auto &pi = ins[k];
+            if (pi.opcode != OpPhi) continue;
+            const uint32_t rt = pi.operands[0], rid = pi.operands[1];
+            const Type t = ir_type(rt);
+            if (t.scalar == ir::Scalar::Void || t.scalar == ir::Scalar::Texture || t.scalar == ir::Scalar::DataPtr)
+                return fail("OpPhi of an unsupported type");
+            phi_local[rid] = local_var(t, t.size_bytes(), value_name(rid) + "_phi");
+            for (size_t a = 2; a + 1 < pi.operands.size(); a += 2) {
+                auto pb = c.blocks.find(pi.operands[a + 1]);
+                if (pb == c.blocks.end()) return fail("OpPhi names an unknown predecessor");
+                pb->second.phi_out.push_back({rid, pi.operands[a]});
             }
+        }
+    }
+    return true;
+}
+
+uint32_t Translator::local_var(const Type &t, uint32_t bytes, const std::string &name) {
+    ir::Variable lv;
+    lv.type = t;
+    lv.size = bytes;
+    lv.alignment = 4;
+    lv.pointer = m.new_value(Type::ptr(), name);
+    m.main.locals.push_back(lv);
+    return lv.pointer;
+}
+
+bool Translator::emit_phi_stores(const CfgBlock &b) {
+    for (const auto &ps : b.phi_out) {
+        const Val *v = get(ps.second);
+        if (!v) return false;
+        emit(Op::Store, {phi_local.at(ps.first), v->ir[0]}, Type{}, {4});
+    }
+    return true;
+}
+
+// 1: `label` is a jump out of (or within) an enclosing construct and the
+// jump was emitted; 0: it is ordinary code to emit in place.
+int Translator::try_jump(uint32_t label) {
+    for (size_t i = constructs.size(); i-- > construct_base;) {
+        const Construct &c = constructs[i];
+        if (label == c.merge_label) {
+            emit_jump(Op::BreakBlock, c.merge_block);
+            return 1;
+        }
+        if (c.kind == Construct::Switch)
+            for (const auto &cs : c.cases)
+                if (cs.first == label) {
+                    emit_jump(Op::BreakBlock, cs.second);
+                    return 1;
+                }
+        if (c.kind == Construct::Loop) {
+            if (c.cont_block && label == c.cont_label) {
+                emit_jump(Op::BreakBlock, c.cont_block);
+                return 1;
+            }
+            if (label == c.header_label) {
+                emit_jump(Op::ContinueLoop, c.loop_block);
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+bool Translator::emit_target(uint32_t label) {
+    if (try_jump(label)) return true;
+    return emit_from(label, false);
+}
+
+// if (cond) t else f, as the reference IR writes it: nested blocks and a
+// branch that breaks from the chosen one
+//   block f_arm { block t_arm { branch cond {1 -> t_arm, default -> f_arm} } <t> } <f>
+bool Translator::emit_if(uint32_t cond, uint32_t t, uint32_t f) {
+    const uint32_t fb = m.main.new_block(value_name(f)), tb = m.main.new_block(value_name(t));
+    const uint32_t outer = cur_block;
+    emit_jump(Op::Block, fb);
+    cur_block = fb;
+    emit_jump(Op::Block, tb);
+    cur_block = tb;
+    emit(Op::Switch, {cond}, Type{}, {fb, 1u, tb});
+    cur_block = fb;
+    if (!emit_target(t)) return false;
+    cur_block = outer;
+    return emit_target(f);
+}
+
+// OpSwitch: the case targets in block order, each case's code after the end
+// of its own block, nested so that falling off one case's code reaches the
+// next one's (fallthrough), with the dispatch innermost.
+bool Translator::emit_switch(const CfgBlock &b, const std::vector<uint32_t> &o) {
+    const Val *sel = get(o[0]);
+    if (!sel) return false;
+    if (m.value(sel->ir[0]).type != Type::i32()) return fail("OpSwitch selector is not a 32-bit integer");
+    std::vector<uint32_t> targets;
+    auto add = [&](uint32_t l) {
+        if (l != b.merge && std::find(targets.begin(), targets.end(), l) == targets.end()) targets.push_back(l);
+    };
+    add(o[1]);
+    for (size_t i = 2; i + 1 < o.size(); i += 2) add(o[i + 1]);
+    auto pos = [&](uint32_t l) {
+        return std::find(fcfg->order.begin(), fcfg->order.end(), l) - fcfg->order.begin();
+    };
+    std::sort(targets.begin(), targets.end(), [&](uint32_t x, uint32_t y) { return pos(x) < pos(y); });
+    // A case that falls through to another must be laid out right before
+    // it. The case construct of target t is what t reaches without passing
+    // the merge or another target; the first other target it reaches is the
+    // one it falls through to.
+    std::map<uint32_t, uint32_t> falls_to;
+    for (uint32_t t : targets) {
+        std::vector<uint32_t> stack{t};
+        std::set<uint32_t> seen{t};
+        while (!stack.empty() && !falls_to.count(t)) {
+            const uint32_t l = stack.back();
+            stack.pop_back();
+            auto bi = fcfg->blocks.find(l);
+            if (bi == fcfg->blocks.end()) continue;
+            const SpirvInstruction &term = sm.instructions[bi->second.end];
+            std::vector<uint32_t> succ;
+            if (term.opcode == OpBranch) succ = {term.operands[0]};
+            else if (term.opcode == OpBranchConditional) succ = {term.operands[1], term.operands[2]};
+            else if (term.opcode == OpSwitch)
+                for (size_t i = 1; i < term.operands.size(); i += 2) succ.push_back(term.operands[i]);
+            if (bi->second.merge) succ.push_back(bi->second.merge);
+            for (uint32_t n : succ) {
+                if (n == b.merge || seen.count(n)) continue;
+                if (std::find(targets.begin(), targets.end(), n) != targets.end()) {
+                    falls_to[t] = n;
+                    break;
+                }
+                seen.insert(n);
+                stack.push_back(n);
+            }
+        }
+    }
+    {
+        std::vector<uint32_t> chained;
+        std::set<uint32_t> has_pred;
+        for (const auto &kv : falls_to) has_pred.insert(kv.second);
+        for (uint32_t t : targets) {
+            if (has_pred.count(t)) continue;
+            for (uint32_t u = t; u && std::find(chained.begin(), chained.end(), u) == chained.end();) {
+                chained.push_back(u);
+                auto it = falls_to.find(u);
+                u = it == falls_to.end() ? 0 : it->second;
+            }
+        }
+        if (chained.size() != targets.size()) return fail("switch cases fall through in a cycle");
+        targets.swap(chained);
+    }
+    const uint32_t mb = m.main.new_block(value_name(b.merge));
+    const uint32_t outer = cur_block;
+    emit_jump(Op::Block, mb);
+    cur_block = mb;
+    std::vector<uint32_t> blk(targets.size());
+    for (size_t k = 0; k < targets.size(); ++k) blk[k] = m.main.new_block(value_name(targets[k]));
+    for (size_t k = targets.size(); k-- > 0;) {
+        emit_jump(Op::Block, blk[k]);
+        cur_block = blk[k];
+    }
+    auto block_of = [&](uint32_t l) {
+        for (size_t k = 0; k < targets.size(); ++k)
+            if (targets[k] == l) return blk[k];
+        return mb;
+    };
+    std::vector<uint32_t> fields{block_of(o[1])};
+    for (size_t i = 2; i + 1 < o.size(); i += 2) {
+        fields.push_back(o[i]);
+        fields.push_back(block_of(o[i + 1]));
+    }
+    emit(Op::Switch, {sel->ir[0]}, Type{}, fields);
+    Construct c;
+    c.kind = Construct::Switch;
+    c.merge_label = b.merge;
+    c.merge_block = mb;
+    for (size_t k = 0; k < targets.size(); ++k) c.cases.push_back({targets[k], blk[k]});
+    constructs.push_back(c);
+    for (size_t k = 0; k < targets.size(); ++k) {
+        cur_block = k + 1 < targets.size() ? blk[k + 1] : mb;
+        if (!emit_from(targets[k], false)) return false;
+    }
+    constructs.pop_back();
+    cur_block = outer;
+    return true;
+}
+
+bool Translator::emit_from(uint32_t label, bool header_entry) {
+    for (;;) {
+        auto it = fcfg->blocks.find(label);
+        if (it == fcfg->blocks.end()) return fail("branch to an unknown block %" + std::to_string(label));
+        CfgBlock &b = it->second;
+        if (b.loop && !header_entry) {
+            // block merge { loop header { block cont { <header .. body> } <continue construct> } }
+            if (b.emitted) return fail("unstructured control flow: loop header reached twice");
+            const uint32_t mb = m.main.new_block(value_name(b.merge));
+            const uint32_t lb = m.main.new_block(value_name(label), true);
+            const uint32_t cb = b.cont != label ? m.main.new_block(value_name(b.cont)) : 0;
+            const uint32_t outer = cur_block;
+            emit_jump(Op::Block, mb);
+            cur_block = mb;
+            emit_jump(Op::Loop, lb);
+            cur_block = lb;
+            if (cb) {
+                emit_jump(Op::Block, cb);
+                cur_block = cb;
+            }
+            Construct c;
+            c.kind = Construct::Loop;
+            c.merge_label = b.merge;
+            c.merge_block = mb;
+            c.cont_label = b.cont;
+            c.cont_block = cb;
+            c.header_label = label;
+            c.loop_block = lb;
+            constructs.push_back(c);
+            if (!emit_from(label, true)) return false;
+            if (cb) {
+                // the continue construct: runs after the body's continues,
+                // ends with the back edge (or a break)
+                cur_block = lb;
+                constructs.back().cont_block = 0;
+                if (!emit_from(b.cont, false)) return false;
+            }
+            constructs.pop_back();
+            cur_block = outer;
+            if (try_jump(b.merge)) return true;
+            label = b.merge;
             continue;
         }
-        if (ins.opcode == OpFunctionEnd) break;
-        if (done) continue;
-        auto V = [&](uint32_t id) -> const Val * { return get(id); };
+        header_entry = false;
+        if (b.emitted) return fail("unstructured control flow: block %" + std::to_string(label) + " is reached twice");
+        b.emitted = true;
+        for (size_t i = b.begin; i < b.end; ++i)
+            if (!body_ins(sm.instructions[i])) return false;
+        const SpirvInstruction &t = sm.instructions[b.end];
+        const auto &o = t.operands;
+        switch (t.opcode) {
+        case OpReturn:
+            emit_jump(Op::BreakBlock, fcfg->ret_block);
+            return true;
+        case OpReturnValue: {
+            const Val *v = get(o[0]);
+            if (!v) return false;
+            if (!fcfg->ret_local) return fail("OpReturnValue in a function without a result");
+            emit(Op::Store, {fcfg->ret_local, v->ir[0]}, Type{}, {4});
+            emit_jump(Op::BreakBlock, fcfg->ret_block);
+            return true;
+        }
+        case OpKill:
+            if (m.stage != ir::Stage::Fragment) return fail("OpKill outside a fragment shader");
+            emit(Op::Kill, {}, Type{});
+            return true;
+        case OpUnreachable:
+            emit(Op::Unreachable, {}, Type{});
+            return true;
+        case OpBranch:
+            if (!emit_phi_stores(b)) return false;
+            if (try_jump(o[0])) return true;
+            label = o[0];
+            continue;
+        case OpBranchConditional: {
+            if (!emit_phi_stores(b)) return false;
+            const Val *c = get(o[0]);
+            if (!c) return false;
+            if (m.value(c->ir[0]).type != Type::boolean()) return fail("branch condition is not a bool");
+            if (!b.selection) return emit_if(c->ir[0], o[1], o[2]);  // a loop's exit test, a conditional break
+            const uint32_t mb = m.main.new_block(value_name(b.merge));
+            const uint32_t outer = cur_block;
+            emit_jump(Op::Block, mb);
+            cur_block = mb;
+            Construct sc;
+            sc.kind = Construct::Selection;
+            sc.merge_label = b.merge;
+            sc.merge_block = mb;
+            constructs.push_back(sc);
+            if (!emit_if(c->ir[0], o[1], o[2])) return false;
+            constructs.pop_back();
+            cur_block = outer;
+            if (try_jump(b.merge)) return true;
+            label = b.merge;
+            continue;
+        }
+        case OpSwitch:
+            if (!emit_phi_stores(b)) return false;
+            if (!b.selection) return fail("OpSwitch without OpSelectionMerge");
+            if (!emit_switch(b, o)) return false;
+            if (try_jump(b.merge)) return true;
+            label = b.merge;
+            continue;
+        default: return fail("unsupported block terminator " + std::to_string(t.opcode));
+        }
+    }
+}
+
+// OpFunctionCall: the callee inlined into a block that its returns break from
+bool Translator::call(const std::vector<uint32_t> &o) {
+    if (++inline_depth > 32) return fail("function calls nested too deeply (recursion?)");
+    FnCfg cc;
+    if (!build_cfg(o[2], cc)) return false;
+    if (cc.params.size() != o.size() - 3) return fail("OpFunctionCall argument count differs from the callee's");
+    std::set<uint32_t> before;
+    for (const auto &kv : vals) before.insert(kv.first);
+    for (size_t i = 0; i < cc.params.size(); ++i) {
+        const Val *a = get(o[3 + i]);
+        if (!a) return false;
+        vals[cc.params[i]] = *a;
+    }
+    const SType &rt = types.at(o[0]);
+    Type rty;
+    if (rt.k != SType::Void) {
+        rty = ir_type(o[0]);
+        if (rty.scalar == ir::Scalar::Void) return fail("function result of an unsupported type");
+        cc.ret_local = local_var(rty, rty.size_bytes(), value_name(o[1]) + "_ret");
+    }
+    cc.ret_block = m.main.new_block("inline_" + value_name(o[2]));
+    const uint32_t outer = cur_block;
+    emit_jump(Op::Block, cc.ret_block);
+    cur_block = cc.ret_block;
+    FnCfg *saved = fcfg;
+    const size_t saved_base = construct_base;
+    fcfg = &cc;
+    construct_base = constructs.size();
+    const bool ok = emit_from(cc.order[0], false);
+    fcfg = saved;
+    construct_base = saved_base;
+    cur_block = outer;
+    if (!ok) return false;
+    // the callee's ids may be defined again by another call
+    for (auto it = vals.begin(); it != vals.end();) it = before.count(it->first) ? std::next(it) : vals.erase(it);
+    if (rt.k != SType::Void) {
+        Val r;
+        r.type = o[0];
+        r.ir.push_back(emit(Op::Load, {cc.ret_local}, rty, {4}, value_name(o[1])));
+        vals[o[1]] = r;
+    }
+    --inline_depth;
+    return true;
+}
+
+bool Translator::translate_function(uint32_t fn) {
+    m.main.name = "main";
+    FnCfg c;
+    if (!build_cfg(fn, c)) return false;
+    m.main.body().name = value_name(c.order[0]);
+    c.ret_block = 0;
+    fcfg = &c;
+    cur_block = 0;
+    const bool ok = emit_from(c.order[0], false);
+    fcfg = nullptr;
+    return ok;
+}
+
+const Val *Translator::binary_operands(const std::vector<uint32_t> &o, const Val *&b) {
+    const Val *a = get(o[2]);
+    b = o.size() > 3 ? get(o[3]) : nullptr;
+    return a;
+}
+
+bool Translator::body_ins(const SpirvInstruction &ins) {
+    const auto &o = ins.operands;
+    auto V = [&](uint32_t id) -> const Val * { return get(id); };
+    auto define = [&](uint32_t rtype, uint32_t rid, uint32_t ir_id) {
+        Val r;
+        r.type = rtype;
+        r.ir.push_back(ir_id);
+        vals[rid] = r;
+        return true;
+    };
+    switch (ins.opcode) {
+    case OpLine:
+        cur_loc.file = strings.count(o[0]) ? strings[o[0]] : value_name(o[0]);
+        cur_loc.line = o[1];
+        cur_loc.column = o[2];
+        have_loc = true;
+        return true;
+    case OpNoLine: have_loc = false; return true;
+    case OpNop:
+    case OpSelectionMerge:
+    case OpLoopMerge:
+        return true;
+    case OpPhi: {
+        const Type t = ir_type(o[0]);
+        return define(o[0], o[1], emit(Op::Load, {phi_local.at(o[1])}, t, {4}, value_name(o[1])));
+    }
+    case OpFunctionCall: return call(o);
+    case OpCopyObject: {
+        const Val *a = V(o[2]);
+        if (!a) return false;
+        vals[o[1]] = *a;
+        return true;
+    }
+    case OpUndef: {
+        const Type t = ir_type(o[0]);
+        if (t.scalar == ir::Scalar::Void || t.scalar == ir::Scalar::Texture) return fail("OpUndef of an unsupported type");
+        const uint32_t vid = m.new_value(t, "");
+        m.value(vid).is_const = true;  // an undefined value may be any value: zero
+        return define(o[0], o[1], vid);
+    }
+    case OpCompositeInsert: {
+        const Val *obj = V(o[2]);
+        const Val *comp = V(o[3]);
+        if (!obj || !comp) return false;
+        const SType &ct = types.at(o[0]);
+        if (ct.k != SType::Vector || o.size() != 5) return fail("composite insert into an unsupported type");
+        const Type vt = ir_type(o[0]);
+        std::vector<uint32_t> parts;
+        for (uint32_t l = 0; l < vt.lanes; ++l)
+            parts.push_back(l == o[4] ? comp->ir[0] : emit(Op::Extract, {obj->ir[0]}, Type{vt.scalar, 1}, {l}));
+        return define(o[0], o[1], emit(Op::MakeVector, parts, vt, {}, value_name(o[1])));
+    }
+    case OpSNegate:
+    case OpNot:
+    case OpLogicalNot:
+    case OpConvertUToF:
+    case OpConvertFToU:
+    case OpBitcast:
+    case OpAny:
+    case OpAll:
+    case OpIsNan:
+    case OpIsInf: {
+        const Val *a = V(o[2]);
+        if (!a) return false;
+        Op op;
         switch (ins.opcode) {
-        case OpLine:
-            cur_loc.file = strings.count(o[0]) ? strings[o[0]] : value_name(o[0]);
-            cur_loc.line = o[1];
-            cur_loc.column = o[2];
-            have_loc = true;
-            break;
-        case OpNoLine: have_loc = false; break;
-        case OpLabel:
-            if (seen_label) return fail("structured control flow is not supported yet (more than one block)");
-            seen_label = true;
-            m.main.body.name = value_name(o[0]);
-            break;
+        case OpSNegate: op = Op::INeg; break;
+        case OpNot:
+        case OpLogicalNot: op = Op::Not; break;
+        case OpConvertUToF: op = Op::ConvUToF; break;
+        case OpConvertFToU: op = Op::ConvFToU; break;
+        case OpBitcast: op = Op::Bitcast; break;
+        case OpAny: op = Op::Any; break;
+        case OpAll: op = Op::All; break;
+        case OpIsNan: op = Op::IsNan; break;
+        default: op = Op::IsInf; break;
+        }
+        const Type rt = ir_type(o[0]);
+        if (rt.scalar == ir::Scalar::Void) return fail("unsupported result type");
+        if (op == Op::Bitcast && rt.lanes != m.value(a->ir[0]).type.lanes) return fail("bitcast between vector sizes");
+        return define(o[0], o[1], emit(op, {a->ir[0]}, rt, {}, value_name(o[1])));
+    }
+    case OpISub:
+    case OpIMul:
+    case OpUDiv:
+    case OpSDiv:
+    case OpUMod:
+    case OpSRem:
+    case OpSMod:
+    case OpFRem:
+    case OpFMod:
+    case OpShiftLeftLogical:
+    case OpShiftRightLogical:
+    case OpShiftRightArithmetic:
+    case OpBitwiseAnd:
+    case OpBitwiseOr:
+    case OpBitwiseXor:
+    case OpLogicalAnd:
+    case OpLogicalOr: {
+        const Val *b = nullptr;
+        const Val *a = binary_operands(o, b);
+        if (!a || !b) return false;
+        Op op;
+        switch (ins.opcode) {
+        case OpISub: op = Op::ISub; break;
+        case OpIMul: op = Op::IMul; break;
+        case OpUDiv: op = Op::UDiv; break;
+        case OpSDiv: op = Op::SDiv; break;
+        case OpUMod: op = Op::UMod; break;
+        case OpSRem: op = Op::SRem; break;
+        case OpSMod: op = Op::SMod; break;
+        case OpFRem: op = Op::FRem; break;
+        case OpFMod: op = Op::FMod; break;
+        case OpShiftLeftLogical: op = Op::Shl; break;
+        case OpShiftRightLogical: op = Op::ShrL; break;
+        case OpShiftRightArithmetic: op = Op::ShrA; break;
+        case OpBitwiseAnd:
+        case OpLogicalAnd: op = Op::And; break;
+        case OpBitwiseOr:
+        case OpLogicalOr: op = Op::Or; break;
+        default: op = Op::Xor; break;
+        }
+        return define(o[0], o[1], emit(op, {a->ir[0], b->ir[0]}, ir_type(o[0]), {}, value_name(o[1])));
+    }
+    case OpIEqual:
+    case OpINotEqual:
+    case OpLogicalEqual:
+    case OpLogicalNotEqual:
+    case OpUGreaterThan:
+    case OpSGreaterThan:
+    case OpUGreaterThanEqual:
+    case OpSGreaterThanEqual:
+    case OpULessThan:
+    case OpSLessThan:
+    case OpULessThanEqual:
+    case OpSLessThanEqual:
+    case OpFOrdEqual:
+    case OpFUnordEqual:
+    case OpFOrdNotEqual:
+    case OpFUnordNotEqual:
+    case OpFUnordLessThan:
+    case OpFUnordGreaterThan:
+    case OpFOrdLessThanEqual:
+    case OpFUnordLessThanEqual:
+    case OpFOrdGreaterThanEqual:
+    case OpFUnordGreaterThanEqual: {
+        const Val *b = nullptr;
+        const Val *a = binary_operands(o, b);
+        if (!a || !b) return false;
+        Op op = Op::ICmp;
+        uint32_t pred = 0;
+        switch (ins.opcode) {
+        case OpIEqual:
+        case OpLogicalEqual: pred = ir::IEq; break;
+        case OpINotEqual:
+        case OpLogicalNotEqual: pred = ir::INe; break;
+        case OpUGreaterThan: pred = ir::IUGt; break;
+        case OpSGreaterThan: pred = ir::ISGt; break;
+        case OpUGreaterThanEqual: pred = ir::IUGe; break;
+        case OpSGreaterThanEqual: pred = ir::ISGe; break;
+        case OpULessThan: pred = ir::IULt; break;
+        case OpSLessThan: pred = ir::ISLt; break;
+        case OpULessThanEqual: pred = ir::IULe; break;
+        case OpSLessThanEqual: pred = ir::ISLe; break;
+        default:
+            op = Op::FCmp;
+            switch (ins.opcode) {
+            case OpFOrdEqual: pred = ir::FOEq; break;
+            case OpFUnordEqual: pred = ir::FUEq; break;
+            case OpFOrdNotEqual: pred = ir::FONe; break;
+            case OpFUnordNotEqual: pred = ir::FUNe; break;
+            case OpFUnordLessThan: pred = ir::FULt; break;
+            case OpFUnordGreaterThan: pred = ir::FUGt; break;
+            case OpFOrdLessThanEqual: pred = ir::FOLe; break;
+            case OpFUnordLessThanEqual: pred = ir::FULe; break;
+            case OpFOrdGreaterThanEqual: pred = ir::FOGe; break;
+            default: pred = ir::FUGe; break;
+            }
+        }
+        return define(o[0], o[1], emit(op, {a->ir[0], b->ir[0]}, ir_type(o[0]), {pred}, value_name(o[1])));
+    }
         case OpVariable: {
             if (o[2] != SC_Function) return fail("non-function variable inside a function");
             const uint32_t pointee = types.at(o[0]).pointee;
@@ -826,7 +1476,61 @@ bool Translator::translate_function(uint32_t fn) {
             const Type rt = ir_type(o[0]);
             uint32_t res = 0;
             auto one = [&]() { return m.new_const_f32(1.0f); };
+            // operand count of each GLSL.std.450 instruction this translator takes
+            static const std::map<uint32_t, uint32_t> arity = {
+                {1, 1}, {2, 1}, {3, 1}, {4, 1}, {5, 1}, {6, 1}, {8, 1}, {9, 1}, {10, 1}, {13, 1}, {14, 1},
+                {26, 2}, {27, 1}, {28, 1}, {29, 1}, {30, 1}, {31, 1}, {32, 1}, {37, 2}, {38, 2}, {39, 2},
+                {40, 2}, {41, 2}, {42, 2}, {43, 3}, {44, 3}, {45, 3}, {46, 3}, {48, 2}, {49, 3}, {66, 1},
+                {67, 2}, {68, 2}, {69, 1}, {71, 2}};
+            auto ar = arity.find(inst);
+            if (ar == arity.end()) return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
+            if (a.size() != ar->second)
+                return fail("GLSL.std.450 instruction " + std::to_string(inst) + " takes " +
+                            std::to_string(ar->second) + " operands");
+            if (rt.scalar == ir::Scalar::Void) return fail("GLSL.std.450 result of an unsupported type");
+            auto splat = [&](uint32_t scalar) { return rt.lanes > 1 ? emit(Op::Splat, {scalar}, rt) : scalar; };
+            auto unary = [&](Op op) { return emit(op, {a[0]->ir[0]}, rt); };
+            auto binary = [&](Op op) { return emit(op, {a[0]->ir[0], a[1]->ir[0]}, rt); };
             switch (inst) {
+            case 1: {                                                                                   // Round: trunc(x + copysign(0.5, x))
+                const uint32_t half = splat(m.new_const_f32(0.5f));
+                const uint32_t neg = emit(Op::FCmp, {a[0]->ir[0], splat(m.new_const_f32(0.0f))},
+                                          Type::boolean(rt.lanes), {ir::FOLt});
+                const uint32_t h = emit(Op::Select, {neg, emit(Op::FNeg, {half}, rt), half}, rt);
+                res = emit(Op::Trunc, {emit(Op::FAdd, {a[0]->ir[0], h}, rt)}, rt);
+                break;
+            }
+            case 2: res = unary(Op::RoundEven); break;
+            case 3: res = unary(Op::Trunc); break;
+            case 5: res = unary(Op::SAbs); break;
+            case 6: res = unary(Op::FSign); break;
+            case 13: res = unary(Op::Sin); break;
+            case 14: res = unary(Op::Cos); break;
+            case 26: res = binary(Op::Pow); break;
+            case 27: res = unary(Op::Exp); break;
+            case 28: res = unary(Op::Log); break;
+            case 29: res = unary(Op::Exp2); break;
+            case 30: res = unary(Op::Log2); break;
+            case 38: res = binary(Op::UMin); break;
+            case 39: res = binary(Op::SMin); break;
+            case 41: res = binary(Op::UMax); break;
+            case 42: res = binary(Op::SMax); break;
+            case 44: res = emit(Op::UClamp, {a[0]->ir[0], a[1]->ir[0], a[2]->ir[0]}, rt); break;
+            case 45: res = emit(Op::SClamp, {a[0]->ir[0], a[1]->ir[0], a[2]->ir[0]}, rt); break;
+            case 48: {                                                                                  // Step: x < edge ? 0 : 1
+                const uint32_t lt = emit(Op::FCmp, {a[1]->ir[0], a[0]->ir[0]}, Type::boolean(rt.lanes), {ir::FOLt});
+                res = emit(Op::Select, {lt, splat(m.new_const_f32(0.0f)), splat(one())}, rt);
+                break;
+            }
+            case 49: {                                                                                  // SmoothStep: t*t*(3 - 2t), t = clamp((x-e0)/(e1-e0), 0, 1)
+                const uint32_t d = emit(Op::FDiv, {emit(Op::FSub, {a[2]->ir[0], a[0]->ir[0]}, rt),
+                                                   emit(Op::FSub, {a[1]->ir[0], a[0]->ir[0]}, rt)}, rt);
+                const uint32_t t = emit(Op::FClamp, {d, splat(m.new_const_f32(0.0f)), splat(one())}, rt);
+                const uint32_t k = emit(Op::FSub, {splat(m.new_const_f32(3.0f)),
+                                                   emit(Op::FMul, {splat(m.new_const_f32(2.0f)), t}, rt)}, rt);
+                res = emit(Op::FMul, {emit(Op::FMul, {t, t}, rt), k}, rt);
+                break;
+            }
             case 31: res = emit(Op::Sqrt, {a[0]->ir[0]}, rt); break;                                    // Sqrt
             case 32: res = emit(Op::FDiv, {one(), emit(Op::Sqrt, {a[0]->ir[0]}, rt)}, rt); break;      // InverseSqrt
             case 37: res = emit(Op::FMin, {a[0]->ir[0], a[1]->ir[0]}, rt); break;                      // FMin
@@ -849,9 +1553,11 @@ bool Translator::translate_function(uint32_t fn) {
                 res = emit(Op::FSub, {a[0]->ir[0], emit(Op::Floor, {a[0]->ir[0]}, rt)}, rt);
                 break;
             case 4: res = emit(Op::FMax, {a[0]->ir[0], emit(Op::FNeg, {a[0]->ir[0]}, rt)}, rt); break;    // FAbs
-            case 46: {                                                                                  // FMix: x + (y - x) * a
-                const uint32_t d = emit(Op::FSub, {a[1]->ir[0], a[0]->ir[0]}, rt);
-                res = emit(Op::FAdd, {a[0]->ir[0], emit(Op::FMul, {d, a[2]->ir[0]}, rt)}, rt);
+            case 46: {                                                                                  // FMix: x * (1 - a) + y * a
+                // the form whose precision Vulkan states; exact at a = 0 and a = 1
+                const uint32_t om = emit(Op::FSub, {splat(one()), a[2]->ir[0]}, rt);
+                res = emit(Op::FAdd, {emit(Op::FMul, {a[0]->ir[0], om}, rt), emit(Op::FMul, {a[1]->ir[0], a[2]->ir[0]}, rt)},
+                           rt);
                 break;
             }
             case 67: {                                                                                  // Distance
@@ -897,26 +1603,9 @@ bool Translator::translate_function(uint32_t fn) {
             vals[o[1]] = r;
             break;
         }
-        case OpKill:
-            emit(Op::Kill, {}, Type{});
-            emit(Op::BreakBlock, {}, Type{});
-            done = true;
-            break;
-        case OpReturn:
-            emit(Op::BreakBlock, {}, Type{});
-            done = true;
-            break;
-        case OpSelectionMerge:
-        case OpLoopMerge:
-        case OpBranch:
-        case OpBranchConditional:
-            return fail("structured control flow is not supported yet");
-        default:
-            return fail("unsupported SPIR-V opcode " + std::to_string(ins.opcode) + " in function body");
-        }
+    default:
+        return fail("unsupported SPIR-V opcode " + std::to_string(ins.opcode) + " in function body");
     }
-    if (!in_fn) return fail("entry point function not found");
-    if (!done) return fail("entry function does not end in OpReturn or OpKill");
     return true;
 }
 

@@ -24,6 +24,7 @@ VS_MVP_POS3_NORMAL3 = 3
 FS_COLOR = 0
 FS_TEXTURE = 1
 FS_PHONG = 2
+FS_ALPHA_TEST = 3  # interpolated colour, discarded where alpha < 0.5
 
 COMPARE_LESS = 1
 BLEND_ONE, BLEND_ZERO = 1, 0

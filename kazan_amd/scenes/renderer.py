@@ -23,7 +23,8 @@ _VS = {
     g.VS_MVP_POS3_NORMAL3: (shaders.vs_mvp_pos3_normal3,
                             [(0, vk.FORMAT_R32G32B32_SFLOAT, 0), (1, vk.FORMAT_R32G32B32_SFLOAT, 12)]),
 }
-_FS = {g.FS_COLOR: shaders.fs_color, g.FS_TEXTURE: shaders.fs_texture, g.FS_PHONG: shaders.fs_phong}
+_FS = {g.FS_COLOR: shaders.fs_color, g.FS_TEXTURE: shaders.fs_texture, g.FS_PHONG: shaders.fs_phong,
+       g.FS_ALPHA_TEST: shaders.fs_alpha_test}
 
 BUFFER_USAGE_ALL = 0x1FF
 IMAGE_USAGE_ALL = 0xFF

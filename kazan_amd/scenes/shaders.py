@@ -209,7 +209,7 @@ def fs_phong(num_lights=8):
     void main() {
       vec3 n = v_normal * (1.0 / sqrt(dot(v_normal, v_normal)));
       vec3 c = L.ambient.xyz;
-      for (int i = 0; i < N; ++i) {            // unrolled in the module
+      for (int i = 0; i < N; ++i) {
         vec3 Lv = L.pos[i].xyz - v_pos;
         float d2 = dot(Lv, Lv);
         vec3 l = Lv * (1.0 / sqrt(d2));
@@ -222,10 +222,14 @@ def fs_phong(num_lights=8):
         c = c + L.color[i].xyz * k;
       }
       o = vec4(c, 1.0);
-    }"""
+    }
+    The loop is a structured SPIR-V loop as an optimising compiler writes it:
+    OpLoopMerge in the header, the counter and the colour carried by OpPhi,
+    the exit test in its own block, the increment in the continue target, and
+    L.pos[i] / L.color[i] as run-time access-chain indices."""
 
     def body(m, var):
-        f32, v3, v4 = m.t_float(), m.t_vec(3), m.t_vec(4)
+        f32, v3, v4, i32, b = m.t_float(), m.t_vec(3), m.t_vec(4), m.t_int(), m.t_bool()
         arr = m.t_array(v4, m.c_int(num_lights, signed=False))
         m.decorate(arr, sp.DecorationArrayStride, 16)
         blk = m.t_struct(arr, arr, v4)
@@ -242,30 +246,72 @@ def fs_phong(num_lights=8):
         one, zero = m.c_float(1.0), m.c_float(0.0)
         pv4 = m.t_ptr(sp.StorageUniform, v4)
         f = sp.Function(m)
+        entry = None
         vp = f.load(v3, vpos_in)
         vn = f.load(v3, vnrm_in)
         rn = f.fdiv(f32, one, f.ext(f32, sp.GLSL_Sqrt, f.dot(vn, vn)))
         n = f.vec_times_scalar(v3, vn, rn)
         amb = f.load(v4, f.access_chain(pv4, ubo, m.c_int(2)))
-        c = f.shuffle(v3, amb, amb, 0, 1, 2)
-        for i in range(num_lights):
-            lp = f.load(v4, f.access_chain(pv4, ubo, m.c_int(0), m.c_int(i)))
-            lc = f.load(v4, f.access_chain(pv4, ubo, m.c_int(1), m.c_int(i)))
-            Lv = f.fsub(v3, f.shuffle(v3, lp, lp, 0, 1, 2), vp)
-            d2 = f.dot(Lv, Lv)
-            l = f.vec_times_scalar(v3, Lv, f.fdiv(f32, one, f.ext(f32, sp.GLSL_Sqrt, d2)))
-            ndl = f.ext(f32, sp.GLSL_FMax, f.dot(n, l), zero)
-            lz = f.extract(f32, l, 2)
-            hv = f.construct(v3, f.extract(f32, l, 0), f.extract(f32, l, 1), f.fadd(f32, lz, one))
-            h = f.vec_times_scalar(v3, hv, f.fdiv(f32, one, f.ext(f32, sp.GLSL_Sqrt, f.dot(hv, hv))))
-            s = f.ext(f32, sp.GLSL_FMax, f.dot(n, h), zero)
-            for _ in range(4):
-                s = f.fmul(f32, s, s)
-            att = f.fdiv(f32, one, f.fadd(f32, one, d2))
-            k = f.fmul(f32, f.fadd(f32, ndl, s), att)
-            c = f.fadd(v3, c, f.vec_times_scalar(v3, f.shuffle(v3, lc, lc, 0, 1, 2), k))
+        c0 = f.shuffle(v3, amb, amb, 0, 1, 2)
+        header, cond, loop_body, cont, merge = (f.new_label() for _ in range(5))
+        entry = f.entry_label
+        f.branch(header)
+        # header: the carried counter and colour
+        f.label(header)
+        i_next, c_next = m.id(), m.id()  # defined in the continue target and the body, used by the phis
+        i = f.phi(i32, (m.c_int(0), entry), (i_next, cont))
+        c = f.phi(v3, (c0, entry), (c_next, cont))
+        f.loop_merge(merge, cont)
+        f.branch(cond)
+        f.label(cond)
+        f.branch_cond(f.binop(sp.OpSLessThan, b, i, m.c_int(num_lights)), loop_body, merge)
+        f.label(loop_body)
+        lp = f.load(v4, f.access_chain(pv4, ubo, m.c_int(0), i))
+        lc = f.load(v4, f.access_chain(pv4, ubo, m.c_int(1), i))
+        Lv = f.fsub(v3, f.shuffle(v3, lp, lp, 0, 1, 2), vp)
+        d2 = f.dot(Lv, Lv)
+        l = f.vec_times_scalar(v3, Lv, f.fdiv(f32, one, f.ext(f32, sp.GLSL_Sqrt, d2)))
+        ndl = f.ext(f32, sp.GLSL_FMax, f.dot(n, l), zero)
+        lz = f.extract(f32, l, 2)
+        hv = f.construct(v3, f.extract(f32, l, 0), f.extract(f32, l, 1), f.fadd(f32, lz, one))
+        h = f.vec_times_scalar(v3, hv, f.fdiv(f32, one, f.ext(f32, sp.GLSL_Sqrt, f.dot(hv, hv))))
+        sv = f.ext(f32, sp.GLSL_FMax, f.dot(n, h), zero)
+        for _ in range(4):
+            sv = f.fmul(f32, sv, sv)
+        att = f.fdiv(f32, one, f.fadd(f32, one, d2))
+        k = f.fmul(f32, f.fadd(f32, ndl, sv), att)
+        f.result_into(c_next, sp.OpFAdd, v3, c, f.vec_times_scalar(v3, f.shuffle(v3, lc, lc, 0, 1, 2), k))
+        f.branch(cont)
+        f.label(cont)
+        f.result_into(i_next, sp.OpIAdd, i32, i, m.c_int(1))
+        f.branch(header)
+        f.label(merge)
         cx, cy, cz = (f.extract(f32, c, j) for j in range(3))
         f.store(out, f.construct(v4, cx, cy, cz, one))
+        f.ret()
+        return f
+
+    return _fragment_module(body).to_bytes()
+
+
+def fs_alpha_test():
+    """layout(location=0) in vec4 v_color; layout(location=0) out vec4 o;
+    void main() { if (v_color.a < 0.5) discard; o = v_color; }"""
+
+    def body(m, var):
+        f32, v4, b = m.t_float(), m.t_vec(4), m.t_bool()
+        col_in = var(sp.StorageInput, v4, "v_color", location=0)
+        out = var(sp.StorageOutput, v4, "o", location=0)
+        f = sp.Function(m)
+        c = f.load(v4, col_in)
+        lt = f.binop(sp.OpFOrdLessThan, b, f.extract(f32, c, 3), m.c_float(0.5))
+        kill, merge = f.new_label(), f.new_label()
+        f.selection_merge(merge)
+        f.branch_cond(lt, kill, merge)
+        f.label(kill)
+        f.kill()
+        f.label(merge)
+        f.store(out, c)
         f.ret()
         return f
 

@@ -69,8 +69,26 @@ OpSelectionMerge = 247
 OpLabel = 248
 OpBranch = 249
 OpBranchConditional = 250
+OpSwitch = 251
 OpKill = 252
 OpReturn = 253
+OpReturnValue = 254
+OpIAdd = 128
+OpISub = 130
+OpIMul = 132
+OpSDiv = 135
+OpSMod = 139
+OpFMod = 141
+OpLogicalAnd = 167
+OpLogicalNot = 168
+OpIEqual = 170
+OpSGreaterThan = 173
+OpSLessThan = 177
+OpFOrdLessThanEqual = 188
+OpFOrdGreaterThanEqual = 190
+OpBitwiseAnd = 199
+OpShiftRightLogical = 194
+OpConvertFToU = 109
 
 # enumerants
 CapabilityShader = 1
@@ -281,23 +299,82 @@ class Module:
 
 
 class Function:
-    """Emits one function body into the module, in order."""
+    """Emits one function body into the module, in order. `ret` and `params`
+    are type ids (default: void main()); self.params holds the parameter
+    ids."""
 
-    def __init__(self, module, name="main"):
+    def __init__(self, module, name="main", ret=None, params=()):
         self.m = module
         self.id = module.id()
-        void = module.t_void()
-        module.functions += module._inst(OpFunction, void, self.id, FunctionControlNone, module.t_fn(void))
+        ret = module.t_void() if ret is None else ret
+        module.functions += module._inst(OpFunction, ret, self.id, FunctionControlNone, module.t_fn(ret, *params))
         module.name(self.id, name)
-        self.label()
+        self.params = []
+        for p in params:
+            pid = module.id()
+            module.functions += module._inst(OpFunctionParameter, p, pid)
+            self.params.append(pid)
+        self.entry_label = self.label()
 
     def _emit(self, op, *operands):
         self.m.functions += self.m._inst(op, *operands)
 
-    def label(self):
-        i = self.m.id()
+    def label(self, i=None):
+        """Starts a block: a new label, or the id reserved by new_label()."""
+        i = self.m.id() if i is None else i
         self._emit(OpLabel, i)
         return i
+
+    def new_label(self):
+        return self.m.id()
+
+    def result_into(self, rid, op, type_id, *operands):
+        """An instruction whose result id was reserved earlier (a value a
+        phi names before it is defined)."""
+        self._emit(op, type_id, rid, *operands)
+        return rid
+
+    # structured control flow ---------------------------------------------
+    def var(self, pointee):
+        """A Function-storage variable (emit these in the first block)."""
+        return self._result(OpVariable, self.m.t_ptr(StorageFunction, pointee), StorageFunction)
+
+    def branch(self, target):
+        self._emit(OpBranch, target)
+
+    def branch_cond(self, cond, t, f):
+        self._emit(OpBranchConditional, cond, t, f)
+
+    def selection_merge(self, merge):
+        self._emit(OpSelectionMerge, merge, 0)
+
+    def loop_merge(self, merge, cont):
+        self._emit(OpLoopMerge, merge, cont, 0)
+
+    def switch(self, selector, default, cases):
+        """cases: [(literal, label)]"""
+        self._emit(OpSwitch, selector, default, [w for c in cases for w in c])
+
+    def phi(self, type_id, *pairs):
+        """pairs: (value, predecessor label)"""
+        return self._result(OpPhi, type_id, [w for p in pairs for w in p])
+
+    def kill(self):
+        self._emit(OpKill)
+
+    def call(self, ret_type, fn, *args):
+        return self._result(OpFunctionCall, ret_type, fn, list(args))
+
+    def ret_only(self):
+        """OpReturn without ending the function (a return inside the body)."""
+        self._emit(OpReturn)
+
+    def end(self):
+        self._emit(OpFunctionEnd)
+
+    def ret_value(self, v):
+        self._emit(OpReturnValue, v)
+        self._emit(OpFunctionEnd)
 
     def _result(self, op, type_id, *operands):
         r = self.m.id()

@@ -60,6 +60,7 @@ def lib():
         _lib.kzo_bilinear.argtypes = [ctypes.c_void_p, c_u32, c_u32, c_u32, c_u32, c_f32, c_f32, ctypes.c_void_p]
         _lib.kzo_vertex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_u32, ctypes.c_void_p, ctypes.c_void_p]
         _lib.kzo_fragment.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.kzo_fragment.restype = ctypes.c_int
     return _lib
 
 
@@ -107,14 +108,15 @@ def vertex(scene, draw_index, vi):
 
 
 def fragment(scene, var):
-    """The oracle's fragment shader on one set of varyings: unclamped RGBA."""
+    """The oracle's fragment shader on one set of varyings: unclamped RGBA,
+    or None when the shader discards the fragment."""
     st, keep = make_state(scene)
     v = np.zeros(8, np.float32)
     v[:len(var)] = var
     out = np.zeros(4, np.float32)
-    lib().kzo_fragment(ctypes.byref(st), v.ctypes.data, out.ctypes.data)
+    kept = lib().kzo_fragment(ctypes.byref(st), v.ctypes.data, out.ctypes.data)
     del keep
-    return out
+    return out if kept else None
 
 
 def _draws(s):

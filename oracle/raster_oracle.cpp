@@ -280,11 +280,15 @@ void fs_phong(const KzoState &st, const float *var, float out[4]) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = 1.0f;
 }
 
-void run_fs(const KzoState &st, const float *var, float out[4]) {
+// Returns false when the fragment shader discards the fragment.
+bool run_fs(const KzoState &st, const float *var, float out[4]) {
     switch (st.fs_kind) {
     case KZO_FS_COLOR:
         for (int i = 0; i < 4; ++i) out[i] = var[i];
         break;
+    case KZO_FS_ALPHA_TEST:
+        for (int i = 0; i < 4; ++i) out[i] = var[i];
+        return !(var[3] < 0.5f);
     case KZO_FS_TEXTURE:
         if (st.tex_filter == 0) {
             // NEAREST: texel floor(u * W), floor(v * H), same addressing
@@ -307,7 +311,10 @@ void run_fs(const KzoState &st, const float *var, float out[4]) {
     default:
         out[0] = out[1] = out[2] = out[3] = 0.0f;
     }
+    return true;
 }
+
+bool fs_can_discard(uint32_t fs) { return fs == KZO_FS_ALPHA_TEST; }
 
 struct BandStats {
     uint64_t fragments = 0, depth_passed = 0;
@@ -340,6 +347,28 @@ void raster_band(const KzoState &st, const std::vector<Tri> &tris, int row0, int
                 if (!mask) continue;
                 bs.fragments += __builtin_popcount(mask);
                 const size_t pix = (size_t)py * st.width + px;
+                // varyings and the fragment shader at the pixel centre: before
+                // the depth test when the shader can discard (R-KILL), else
+                // only for fragments that pass it (the same result)
+                float src[4];
+                auto shade = [&]() {
+                    Edges E;
+                    eval_edges(t, (int64_t)px * 16 + 8, (int64_t)py * 16 + 8, E);
+                    const float l0 = kzo_i64_to_f32(E.e[0]) * t.invS;
+                    const float l1 = kzo_i64_to_f32(E.e[1]) * t.invS;
+                    const float l2 = kzo_i64_to_f32(E.e[2]) * t.invS;
+                    const float q = ((l0 * t.iw[0]) + (l1 * t.iw[1])) + (l2 * t.iw[2]);
+                    const float rq = 1.0f / q;
+                    const float b1 = (l1 * t.iw[1]) * rq;
+                    const float b2 = (l2 * t.iw[2]) * rq;
+                    float var[kMaxVar];
+                    for (int i = 0; i < nv; ++i) var[i] = (t.a0[i] + b1 * t.da1[i]) + b2 * t.da2[i];
+                    const bool keep = run_fs(st, var, src);
+                    for (int c = 0; c < 4; ++c) src[c] = clamp01(src[c]);
+                    return keep;
+                };
+                const bool late = fs_can_discard(st.fs_kind);
+                if (late && !shade()) continue;
                 uint32_t pass = 0;
                 for (uint32_t s = 0; s < ns; ++s) {
                     if (!(mask >> s & 1u)) continue;
@@ -350,21 +379,7 @@ void raster_band(const KzoState &st, const std::vector<Tri> &tris, int row0, int
                 }
                 if (!pass) continue;
                 bs.depth_passed += __builtin_popcount(pass);
-                // Varyings and the fragment shader at the pixel centre.
-                Edges E;
-                eval_edges(t, (int64_t)px * 16 + 8, (int64_t)py * 16 + 8, E);
-                const float l0 = kzo_i64_to_f32(E.e[0]) * t.invS;
-                const float l1 = kzo_i64_to_f32(E.e[1]) * t.invS;
-                const float l2 = kzo_i64_to_f32(E.e[2]) * t.invS;
-                const float q = ((l0 * t.iw[0]) + (l1 * t.iw[1])) + (l2 * t.iw[2]);
-                const float rq = 1.0f / q;
-                const float b1 = (l1 * t.iw[1]) * rq;
-                const float b2 = (l2 * t.iw[2]) * rq;
-                float var[kMaxVar];
-                for (int i = 0; i < nv; ++i) var[i] = (t.a0[i] + b1 * t.da1[i]) + b2 * t.da2[i];
-                float src[4];
-                run_fs(st, var, src);
-                for (int c = 0; c < 4; ++c) src[c] = clamp01(src[c]);
+                if (!late) shade();
                 for (uint32_t s = 0; s < ns; ++s) {
                     if (!(pass >> s & 1u)) continue;
                     uint8_t *cp = color + (pix * ns + s) * 4;
@@ -432,7 +447,7 @@ void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4],
     for (int i = 0; i < kMaxVar; ++i) var[i] = v.var[i];
 }
 
-void kzo_fragment(const KzoState *st, const float *var, float out[4]) { run_fs(*st, var, out); }
+int kzo_fragment(const KzoState *st, const float *var, float out[4]) { return run_fs(*st, var, out) ? 1 : 0; }
 
 int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8_t *color, float *depth,
                uint8_t *resolved, KzoStats *stats, int threads) {

@@ -47,6 +47,9 @@
  *  R-ORD  fragments are applied in primitive order (draw order, then
  *         triangle order within a draw).
  *  R-RES  4x resolve: (s0 + s1 + s2 + s3 + 2) >> 2 per channel.
+ *  R-KILL a fragment shader that discards (OpKill) runs before the depth
+ *         test: a discarded fragment is counted as a fragment but takes no
+ *         part in the depth test, depth write or blend.
  *
  * A fragment (for the F counter) is a covered (triangle, sample) pair after
  * the scissor and before the depth test.
@@ -72,7 +75,8 @@ enum KzoVertexShader {
 enum KzoFragmentShader {
     KZO_FS_COLOR = 0,    /* out = v_color */
     KZO_FS_TEXTURE = 1,  /* out = texture(tex, v_uv) */
-    KZO_FS_PHONG = 2     /* Phong, lights from the uniform block */
+    KZO_FS_PHONG = 2,    /* Phong, lights from the uniform block */
+    KZO_FS_ALPHA_TEST = 3 /* out = v_color; discard when v_color.a < 0.5 */
 };
 
 typedef struct KzoState {
@@ -136,7 +140,7 @@ uint32_t kzo_encode_unorm8(float v);
  * position and up to 8 varyings), and the fragment shader of st->fs_kind on
  * interpolated varyings (unclamped RGBA). */
 void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4], float var[8]);
-void kzo_fragment(const KzoState *st, const float *var, float out[4]);
+int kzo_fragment(const KzoState *st, const float *var, float out[4]); /* 0: discarded */
 
 #ifdef __cplusplus
 }

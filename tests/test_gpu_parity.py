@@ -338,3 +338,22 @@ def test_phong_big_triangles(mctx, oracle):
     s.num_lights = 8
     st, ost, _ = check_parity(mctx, oracle, s)
     assert ost["fragments"] > 2000
+
+
+# --- fragment discard (OpKill): late depth ----------------------------------
+
+@pytest.mark.parametrize("blend", [0, 1])
+def test_alpha_test_discard(mctx, oracle, blend):
+    """if (v_color.a < 0.5) discard: a discarded fragment writes no depth and
+    no colour, so the triangles behind it show through (the oracle runs such
+    a shader before its depth test)."""
+    r = np.random.default_rng(15 + blend)
+    centres = r.uniform(4, 60, (1500, 1, 2))
+    tris = np.round((centres + r.uniform(-8, 8, (1500, 3, 2))) * 16) / 16
+    s = scene_from_pixels(tris, colors=r.uniform(0, 1, (1500, 3, 4)), z=r.uniform(0, 1, (1500, 3)))
+    s.fs = g.FS_ALPHA_TEST
+    if blend:
+        s.blend = (1, g.BLEND_SRC_ALPHA, g.BLEND_ONE_MINUS_SRC_ALPHA, 0, g.BLEND_ONE, g.BLEND_ONE_MINUS_SRC_ALPHA, 0)
+    st, ost, _ = check_parity(mctx, oracle, s)
+    assert 0 < ost["depth_passed"] < ost["fragments"]
+    assert st["depth_passed"] == ost["depth_passed"]
