@@ -85,6 +85,22 @@ KAZAN_EXPORT int kazan_cost_model_eval(const void *blob, size_t bytes, const flo
 KAZAN_EXPORT int kazan_cost_model_train(const float *x, const float *y, uint32_t n, float *state, uint32_t steps,
                                         uint32_t step0, uint32_t batch, float lr, uint64_t seed, float *loss_out);
 
+/* kazan_cost_model_eval_noalloc: kazan_cost_model_eval with the same
+ * arguments and results, for callers that cannot allocate device memory
+ * (tight serving loops, code that forbids hipMalloc). It has no
+ * counterpart in the reference. It calls no hipMalloc/hipFree: the model,
+ * the features and the predictions are staged through one static device
+ * array of KAZAN_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, so n is limited to KAZAN_NOALLOC_MAX_ROWS (larger n returns -5 and
+ * touches nothing). Calls on one GPU share that array: within a process
+ * they are serialised by a lock; concurrent calls from several processes
+ * each use their own copy. Returns the codes of kazan_cost_model_eval, or
+ * -5 for n > KAZAN_NOALLOC_MAX_ROWS. */
+#define KAZAN_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define KAZAN_NOALLOC_MAX_ROWS 245760u
+KAZAN_EXPORT int kazan_cost_model_eval_noalloc(const void *blob, size_t bytes, const float *features, uint32_t n,
+                                               float *out);
+
 /* what: 0 = generated shader source, 1 = full kernel source, 2 = compile to a
  * code object and report its size and whether the cache held it */
 KAZAN_EXPORT int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,

@@ -31,8 +31,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
+#include <mutex>
 
 #include "device_common.h"
+#include "kazan/kazan_tools.h"
 #include "kernels.h"
 
 namespace {
@@ -2024,4 +2027,60 @@ hipError_t kz_launch_resolve4(hipStream_t st, const uint32_t *src, uint32_t *dst
     if ((uint64_t)w * h == 0) return hipSuccess;
     kz_resolve4<<<grid_for((uint64_t)w * h, 256), 256, 0, st>>>(src, dst, width, x0, y0, w, h);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// kazan_cost_model_eval_noalloc: kazan_cost_model_eval (tools.cpp) without
+// device allocations. The model and the feature and prediction staging live
+// in `scratch`, a file-scope device array each GPU holds for the life of
+// the process. Its device address is looked up once per device with
+// hipGetSymbolAddress (in host code the array's name is only a stand-in);
+// every use goes through that address. Calls on one device are serialised
+// by a mutex; the array keeps the last call's contents, and nothing in it
+// needs to start at zero (the kernel writes every prediction it is asked
+// for).
+
+namespace {
+
+constexpr size_t kNoAllocModelBytes = (sizeof(KzCostModel) + 255) / 256 * 256;
+constexpr size_t kNoAllocFeatBytes = (size_t)KAZAN_NOALLOC_MAX_ROWS * KZ_FEATURES * sizeof(float);
+constexpr size_t kNoAllocPredBytes = (size_t)KAZAN_NOALLOC_MAX_ROWS * sizeof(float);
+static_assert(kNoAllocFeatBytes % 256 == 0, "prediction staging stays 256-byte aligned");
+static_assert(kNoAllocModelBytes + kNoAllocFeatBytes + kNoAllocPredBytes <= KAZAN_NOALLOC_SCRATCH_BYTES,
+              "scratch layout exceeds its stated size");
+
+__device__ __align__(256) static unsigned char scratch[KAZAN_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu[kMaxDevices];
+unsigned char *g_noalloc_ptr[kMaxDevices];  // device address of `scratch`, per device (under its mutex)
+
+}  // namespace
+
+extern "C" int kazan_cost_model_eval_noalloc(const void *blob, size_t bytes, const float *features, uint32_t n,
+                                             float *out) {
+    KzCostModel m;
+    if (!blob || bytes != sizeof m || (!features && n) || (!out && n)) return -1;
+    std::memcpy(&m, blob, sizeof m);
+    if (m.magic != KZ_COST_MODEL_MAGIC || m.version != 1 || m.bytes != sizeof m) return -2;
+    if (n == 0) return 0;
+    if (n > KAZAN_NOALLOC_MAX_ROWS) return -5;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -4;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu[dev]);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -4;
+        g_noalloc_ptr[dev] = (unsigned char *)p;
+    }
+    unsigned char *base = g_noalloc_ptr[dev];
+    KzCostModel *dm = (KzCostModel *)base;
+    float *df = (float *)(base + kNoAllocModelBytes);
+    float *dp = (float *)(base + kNoAllocModelBytes + kNoAllocFeatBytes);
+    if (hipMemcpy(dm, &m, sizeof m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(df, features, (size_t)n * KZ_FEATURES * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        kz_launch_cost_mlp(nullptr, dm, df, n, dp) != hipSuccess ||
+        hipMemcpy(out, dp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return -4;
+    return 0;
 }
