@@ -69,8 +69,8 @@ KAZAN_EXPORT VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetPhysicalDevicePro
 /* Counters of vkGetDrawStatsKAZAN, in order. */
 enum KazanDrawStat {
     KAZAN_STAT_TRIANGLES_IN = 0,          /* primitives assembled */
-    KAZAN_STAT_TRIANGLES_SETUP = 1,       /* survived clip-reject, cull, zero area */
-    KAZAN_STAT_TRIANGLES_UNSUPPORTED = 2, /* needed z or guard-band clipping: dropped */
+    KAZAN_STAT_TRIANGLES_SETUP = 1,       /* survived clip-reject, cull, zero area (clipped: a piece did) */
+    KAZAN_STAT_TRIANGLES_CLIPPED = 2,     /* clipped against the near/far planes or the guard band */
     KAZAN_STAT_FRAGMENTS = 3,             /* covered (triangle, sample) pairs */
     KAZAN_STAT_DEPTH_PASSED = 4,          /* samples that passed the depth test */
     KAZAN_STAT_LIST_ENTRIES = 5,          /* binned (triangle, tile) pairs */

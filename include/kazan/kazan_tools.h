@@ -10,9 +10,19 @@
  *                                lib.rs:191-215) followed by the IR's text form
  *                                (assert_ir_matches_file!, shader-compiler-ir/
  *                                src/macros.rs:39-48)
+ *   kazan_ir_round_trip          counterpart of the reference's IR text parser
+ *                                (shader-compiler-ir/src/text.rs, used by
+ *                                assert_ir_matches_file!, macros.rs:39-80, and
+ *                                the instruction text tests,
+ *                                instructions_impl.rs:560-622): parses IR text
+ *                                and prints it back
  *   kazan_compile_pipeline_spirv runs the whole pipeline compile (SPIR-V -> IR
  *                                -> HIP source -> gfx950 code object) without
  *                                a device, as vkCreateGraphicsPipelines does
+ *   kazan_compile_compute_spirv  the same for a compute shader, as
+ *                                vkCreateComputePipelines does (reference
+ *                                ComputePipeline::create, vulkan-driver/src/
+ *                                pipeline.rs:122-154)
  *
  * All return 0 on success and a negative value on failure, with a message in
  * `out`. Output longer than `cap` is truncated; *out_len receives the full
@@ -101,10 +111,21 @@ KAZAN_EXPORT int kazan_cost_model_train(const float *x, const float *y, uint32_t
 KAZAN_EXPORT int kazan_cost_model_eval_noalloc(const void *blob, size_t bytes, const float *features, uint32_t n,
                                                float *out);
 
+/* what: 0 = a module's text; 1 = value definitions "[name: type, ...]"
+ * followed by one instruction, which is printed back alone */
+KAZAN_EXPORT int kazan_ir_round_trip(const char *text, uint32_t what, char *out, size_t cap, size_t *out_len);
+
 /* what: 0 = generated shader source, 1 = full kernel source, 2 = compile to a
  * code object and report its size and whether the cache held it */
 KAZAN_EXPORT int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,
                                               size_t *out_len);
+
+/* A GLCompute entry point; storage and uniform buffers take kernel slots in
+ * (set, binding) order. what: 0 = generated kernel body, 1 = full kernel
+ * source, 2 = compile to a code object and report its size, whether the
+ * cache held it and the workgroup size */
+KAZAN_EXPORT int kazan_compile_compute_spirv(const uint32_t *words, size_t nwords, const char *entry, uint32_t what,
+                                             char *out, size_t cap, size_t *out_len);
 
 #ifdef __cplusplus
 }

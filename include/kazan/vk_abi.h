@@ -1195,6 +1195,16 @@ typedef struct VkGraphicsPipelineCreateInfo {
     int32_t basePipelineIndex;
 } VkGraphicsPipelineCreateInfo;
 
+typedef struct VkComputePipelineCreateInfo {
+    VkStructureType sType;
+    const void *pNext;
+    VkPipelineCreateFlags flags;
+    VkPipelineShaderStageCreateInfo stage;
+    VkPipelineLayout layout;
+    VkPipeline basePipelineHandle;
+    int32_t basePipelineIndex;
+} VkComputePipelineCreateInfo;
+
 typedef struct VkAttachmentDescription {
     VkAttachmentDescriptionFlags flags;
     VkFormat format;

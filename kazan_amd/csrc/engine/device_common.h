@@ -21,6 +21,7 @@ typedef __hip_internal::int64_t int64_t;
 #define KZ_MAX_UBOS 4
 #define KZ_PUSH_WORDS 32
 #define KZ_MAX_VARYINGS 16
+#define KZ_MAX_SSBOS 8
 
 // Per-draw record. One record per (vkCmdDraw*, instance).
 struct KzDraw {
@@ -43,6 +44,18 @@ struct KzTexture {
     uint32_t w, h;
     uint32_t filter;  // VkFilter
     uint32_t addr_u, addr_v;
+    uint32_t pad;
+};
+
+// The argument of a compute pipeline's kernel (vkCmdDispatch): the bound
+// descriptors and push constants at dispatch time, and the dispatch's base
+// workgroup (vkCmdDispatchBase).
+struct KzComputeFrame {
+    KzTexture tex[KZ_MAX_TEXTURES];
+    const uint8_t *ubo[KZ_MAX_UBOS];
+    uint8_t *ssbo[KZ_MAX_SSBOS];
+    uint32_t push[KZ_PUSH_WORDS];
+    uint32_t base[3];
     uint32_t pad;
 };
 
@@ -203,7 +216,7 @@ struct KzTrainParams {
 enum {
     KZ_STAT_TRIANGLES_IN = 0,
     KZ_STAT_TRIANGLES_SETUP = 1,
-    KZ_STAT_TRIANGLES_UNSUPPORTED = 2,
+    KZ_STAT_TRIANGLES_CLIPPED = 2,
     KZ_STAT_FRAGMENTS = 3,
     KZ_STAT_DEPTH_PASSED = 4,
     KZ_STAT_LIST_ENTRIES = 5,

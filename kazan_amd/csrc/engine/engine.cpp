@@ -735,6 +735,15 @@ hipError_t Engine::launch_ops_one(const Op &op) {
             KZ_TRY(kz_launch_resolve4(stream_, op.rsrc, op.rdst, op.rwidth, op.x0, op.y0, op.w, op.h));
             ++pending_launches_;
             break;
+        case Op::Dispatch: {
+            size_t size = sizeof op.cframe;
+            void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, const_cast<KzComputeFrame *>(&op.cframe),
+                              HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+            KZ_TRY(hipModuleLaunchKernel(op.cs_kernel, op.groups[0], op.groups[1], op.groups[2], op.block[0],
+                                         op.block[1], op.block[2], 0, stream_, nullptr, config));
+            ++pending_launches_;
+            break;
+        }
         }
     }
     return hipSuccess;

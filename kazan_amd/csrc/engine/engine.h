@@ -29,6 +29,7 @@ struct FixedState {
     uint32_t blend = 0;
     uint32_t src_color = 1, dst_color = 0, color_op = 0, src_alpha = 1, dst_alpha = 0, alpha_op = 0;
     uint32_t write_mask = 15;
+    bool optimize = true;  // -O3 (false: -O0, the backend API's NoOptimizations)
 };
 
 // A pipeline's device code: kz_bin_count and kz_raster for one FixedState.
@@ -60,6 +61,11 @@ uint64_t fnv1a64(const void *data, size_t n, uint64_t h = 1469598103934665603ull
 // hiprtc writes; anything else in a cache is damaged or foreign.
 bool is_amdgpu_code_object(const char *data, size_t n);
 std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source);
+// A compute pipeline's kernel (generate_compute_source's output with the
+// shared device structs), compiled through the same code-object cache.
+std::string compute_full_source(const std::string &compute_source);
+bool compile_compute_code(const std::string &compute_source, bool optimize, std::vector<char> &code, std::string &log,
+                          bool *cache_hit);
 // Loads the code object on the current device.
 bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, std::string &err);
 
@@ -137,7 +143,12 @@ struct BatchTuning {
 };
 
 struct Op {
-    enum Kind { Fill, Batch, Copy, Copy2D, Resolve } kind;
+    enum Kind { Fill, Batch, Copy, Copy2D, Resolve, Dispatch } kind;
+    // Dispatch: a compute pipeline's kernel over groups[3] workgroups of
+    // block[3] invocations, with its descriptors and push constants
+    hipFunction_t cs_kernel = nullptr;
+    uint32_t groups[3] = {0, 0, 0}, block[3] = {1, 1, 1};
+    KzComputeFrame cframe{};
     // Fill: ptr = attachment base, rectangle in pixels, spp words per pixel
     uint32_t *fill_ptr = nullptr;
     uint32_t fill_width = 0, fill_spp = 1, fill_value = 0;

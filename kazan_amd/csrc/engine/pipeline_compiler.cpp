@@ -27,9 +27,9 @@ namespace kz {
 
 namespace {
 
+// (the optimisation level, -O3 or -O0 per FixedState::optimize, comes first)
 const char *const kCompileOptions[] = {
     "--offload-arch=gfx950",
-    "-O3",
     "-std=c++17",
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
@@ -42,6 +42,8 @@ const char *const kCompileOptions[] = {
 constexpr int kNumOptions = sizeof(kCompileOptions) / sizeof(kCompileOptions[0]);
 
 uint64_t fnv1a(const std::string &s) { return fnv1a64(s.data(), s.size()); }
+
+const char *opt_level(const FixedState &fs) { return fs.optimize ? "-O3" : "-O0"; }
 
 std::string cache_dir() {
     if (const char *e = std::getenv("KAZAN_PIPELINE_CACHE")) return e;
@@ -114,8 +116,12 @@ bool is_amdgpu_code_object(const char *data, size_t n) {
 
 std::string pipeline_code_dir() { return cache_dir(); }
 
-std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source) {
-    std::string key_text = pipeline_full_source(fs, shader_source);
+namespace {
+
+// the cache file name of a complete source compiled at optimisation `opt`
+std::string code_name(const std::string &full_source, const char *opt) {
+    std::string key_text = full_source;
+    key_text += std::string("\n") + opt;
     for (int i = 0; i < kNumOptions; ++i) key_text += std::string("\n") + kCompileOptions[i];
     int rtc_major = 0, rtc_minor = 0;
     hiprtcVersion(&rtc_major, &rtc_minor);
@@ -125,11 +131,11 @@ std::string pipeline_code_name(const FixedState &fs, const std::string &shader_s
     return name;
 }
 
-bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
-                           std::string &log, bool *cache_hit, bool read_cache) {
-    const std::string src = pipeline_full_source(fs, shader_source);
+// hiprtc through the on-disk code-object cache
+bool compile_cached(const std::string &src, const char *opt, std::vector<char> &code, std::string &log,
+                    bool *cache_hit, bool read_cache) {
     const std::string dir = cache_dir();
-    const std::string path = dir + "/" + pipeline_code_name(fs, shader_source);
+    const std::string path = dir + "/" + code_name(src, opt);
     if (cache_hit) *cache_hit = false;
     if (read_cache) {
         std::ifstream in(path, std::ios::binary);
@@ -147,7 +153,10 @@ bool compile_pipeline_code(const FixedState &fs, const std::string &shader_sourc
         log = "hiprtcCreateProgram failed";
         return false;
     }
-    const hiprtcResult rc = hiprtcCompileProgram(prog, kNumOptions, kCompileOptions);
+    const char *options[kNumOptions + 1];
+    options[0] = opt;
+    for (int i = 0; i < kNumOptions; ++i) options[i + 1] = kCompileOptions[i];
+    const hiprtcResult rc = hiprtcCompileProgram(prog, kNumOptions + 1, options);
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
     if (n > 1) {
@@ -172,6 +181,26 @@ bool compile_pipeline_code(const FixedState &fs, const std::string &shader_sourc
     }
     if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
     return true;
+}
+
+}  // namespace
+
+std::string pipeline_code_name(const FixedState &fs, const std::string &shader_source) {
+    return code_name(pipeline_full_source(fs, shader_source), opt_level(fs));
+}
+
+bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
+                           std::string &log, bool *cache_hit, bool read_cache) {
+    return compile_cached(pipeline_full_source(fs, shader_source), opt_level(fs), code, log, cache_hit, read_cache);
+}
+
+std::string compute_full_source(const std::string &compute_source) {
+    return "// generated compute kernel source\n" + std::string(kz_src_device_common) + "\n" + compute_source;
+}
+
+bool compile_compute_code(const std::string &compute_source, bool optimize, std::vector<char> &code, std::string &log,
+                          bool *cache_hit) {
+    return compile_cached(compute_full_source(compute_source), optimize ? "-O3" : "-O0", code, log, cache_hit, true);
 }
 
 bool load_pipeline_kernels(const std::vector<char> &code, PipelineKernels &out, std::string &err) {

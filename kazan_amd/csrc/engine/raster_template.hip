@@ -138,11 +138,13 @@ struct KzTri {
 };
 
 // 0 = dropped (clip-rejected, degenerate or culled), 1 = set up,
-// 2 = unsupported (needs z or guard-band clipping; dropped and counted).
+// 2 = needs clipping (kz_clip_polygon; the pieces are set up with `piece`).
+// `piece`: p is a piece of a clipped triangle: no clip checks, and a
+// vertex with w <= 0 or outside the guard band drops the piece.
 // Straight-line: every field of t is written whatever the outcome (the
 // checks pick the status at the end), so that a wave whose lanes end in
 // different cases takes no branches.
-__device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], KzTri &t) {
+__device__ __forceinline__ int kz_setup_rt(const KzFrame &f, const float p[3][4], KzTri &t, bool piece) {
     // trivially rejected: all three vertices outside one clip plane
     bool reject = false;
 #pragma unroll
@@ -164,12 +166,14 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
         reject |= out == 3;
     }
     // needs clipping: w <= 0, z outside [0, w], or outside the guard band
+    // (a piece: w <= 0 or outside the band drops it)
     bool unsup = false;
     int32_t sx[3], sy[3];
     float zf[3], iw[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        unsup |= !(p[k][3] > 0.0f) || !(p[k][2] >= 0.0f) || !(p[k][2] <= p[k][3]);
+        unsup |= !(p[k][3] > 0.0f);
+        unsup |= !piece && (!(p[k][2] >= 0.0f) || !(p[k][2] <= p[k][3]));
         // x / 1 == x exactly: skip the IEEE divisions when w is exactly one
         const bool w1 = p[k][3] == 1.0f;
         const float xd = w1 ? p[k][0] : p[k][0] / p[k][3];
@@ -208,7 +212,126 @@ __device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], K
     t.px1 = min(xmax >> 4, f.sc_x1);
     t.py0 = max(ymin >> 4, f.sc_y0);
     t.py1 = min(ymax >> 4, f.sc_y1);
+    if (piece) return (unsup || S == 0 || culled) ? 0 : 1;
     return reject ? 0 : unsup ? 2 : (S == 0 || culled) ? 0 : 1;
+}
+
+__device__ __forceinline__ int kz_setup(const KzFrame &f, const float p[3][4], KzTri &t) {
+    return kz_setup_rt(f, p, t, false);
+}
+
+// ---------------------------------------------------------------------------
+// clipping (a triangle kz_setup gives status 2), one wave per triangle
+//
+// Homogeneous Sutherland-Hodgman against z >= 0, z <= w and the guard
+// band's planes x = gx * w, y = gy * w (screen x, y = -8000 and 8000 px,
+// inside viewportBoundsRange [-8192, 8191], vulkan-driver/src/
+// api_impl.rs:1568), in that order. A polygon vertex is a record of NC
+// floats: its clip position, then its varyings. Per plane, vertex i is
+// kept when its distance d >= 0, and the edge (i, i + 1) adds its crossing
+// point when its ends lie on different sides, interpolated from the inside
+// end, every component alike: t = d_in / (d_in - d_out),
+// v = v_in + t * (v_out - v_in) (so the shared edge of two triangles gets
+// the same point from both). The emitted vertices, in that order, are the
+// next polygon (at most KZ_CLIP_MAX; fewer than 3 drops the triangle).
+// Lane i holds vertex i: a plane costs a ballot pair for the output
+// positions and a gather of each output from the lane that emitted it.
+// The pieces are the fan (0, j + 1, j + 2), j = 0 .. n - 3, drawn in that
+// order in the triangle's place.
+#define KZ_CLIP_MAX 12
+
+struct KzClipBand {
+    float xl, xh, yl, yh;
+};
+
+__device__ __forceinline__ KzClipBand kz_clip_band(const KzFrame &f) {
+    const float ax = (-8000.0f - f.vp_ox) / f.vp_hw, bx = (8000.0f - f.vp_ox) / f.vp_hw;
+    const float ay = (-8000.0f - f.vp_oy) / f.vp_hh, by = (8000.0f - f.vp_oy) / f.vp_hh;
+    return {fminf(ax, bx), fmaxf(ax, bx), fminf(ay, by), fmaxf(ay, by)};
+}
+
+__device__ __forceinline__ float kz_clip_distance(int plane, const KzClipBand &g, const float *q) {
+    switch (plane) {
+    case 0: return q[2];
+    case 1: return q[3] - q[2];
+    case 2: return q[0] - g.xl * q[3];
+    case 3: return g.xh * q[3] - q[0];
+    case 4: return q[1] - g.yl * q[3];
+    default: return g.yh * q[3] - q[1];
+    }
+}
+
+// The whole wave calls this with lane `src`'s triangle (p, vary: that
+// lane's values count). Returns the polygon's vertex count (0: clipped
+// away); lane i < count holds vertex i in v (NC = 4 + varyings carried).
+template <int NC>
+__device__ __forceinline__ int kz_clip_polygon(const KzFrame &f, const float p[3][4], const float vary[3][KZ_NVA],
+                                               uint32_t src, uint32_t lane, float v[NC]) {
+    const KzClipBand g = kz_clip_band(f);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float ra = kz_readlane_f(p[0][c], src), rb = kz_readlane_f(p[1][c], src),
+                    rd = kz_readlane_f(p[2][c], src);
+        v[c] = lane == 0 ? ra : lane == 1 ? rb : rd;
+    }
+#pragma unroll
+    for (int i = 0; i < NC - 4; ++i) {
+        const float ra = kz_readlane_f(vary[0][i], src), rb = kz_readlane_f(vary[1][i], src),
+                    rd = kz_readlane_f(vary[2][i], src);
+        v[4 + i] = lane == 0 ? ra : lane == 1 ? rb : rd;
+    }
+    int n = 3;
+#pragma unroll 1
+    for (int plane = 0; plane < 6; ++plane) {
+        const bool act = lane < (uint32_t)n;
+        const uint32_t nxt = lane + 1 >= (uint32_t)n ? 0u : lane + 1;
+        float u[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) u[c] = kz_bperm_f(v[c], nxt);
+        const float di = kz_clip_distance(plane, g, v), dj = kz_clip_distance(plane, g, u);
+        const bool ini = di >= 0.0f, inj = dj >= 0.0f;
+        const bool keep = act && ini, cross = act && ini != inj;
+        const uint64_t mk = kz_ballot(keep), mc = kz_ballot(cross);
+        const int total = __popcll(mk) + __popcll(mc);
+        if (total < 3) return 0;
+        const uint32_t off = kz_rank(mk) + kz_rank(mc);
+        const uint32_t end = off + (keep ? 1u : 0u) + (cross ? 1u : 0u);
+        // output vertex `lane` comes from the first lane whose emissions end after it
+        uint32_t from = 0;
+        for (int i = 0; i < n; ++i) from += (uint32_t)kz_readlane_i((int32_t)end, (uint32_t)i) <= lane ? 1u : 0u;
+        from = min(from, (uint32_t)(n - 1));
+        // (both reads in full EXEC: a bpermute under a branch returns
+        // nothing from a source lane the branch switched off, so no
+        // short-circuit here)
+        const int32_t keep_from = kz_bperm_i(keep ? 1 : 0, from), off_from = kz_bperm_i((int32_t)off, from);
+        const bool kept = (keep_from != 0) & (lane == (uint32_t)off_from);
+        // the edge's crossing point, from its inside end
+        const float din = ini ? di : dj, dout = ini ? dj : di;
+        const float t = din / (din - dout);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float vin = ini ? v[c] : u[c], vout = ini ? u[c] : v[c];
+            const float x = vin + t * (vout - vin);
+            const float kv = kz_bperm_f(v[c], from), xv = kz_bperm_f(x, from);
+            v[c] = kept ? kv : xv;
+        }
+        n = min(total, KZ_CLIP_MAX);
+    }
+    return n;
+}
+
+// Piece j's vertices (polygon vertices 0, j + 1, j + 2) from the lanes of
+// kz_clip_polygon: clip positions into p, the NC - 4 varyings into vary.
+template <int NC>
+__device__ __forceinline__ void kz_clip_piece(const float v[NC], uint32_t j, float p[3][4], float vary[3][KZ_NVA]) {
+    const uint32_t src[3] = {0u, (j + 1) & 63u, (j + 2) & 63u};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[q][c] = kz_bperm_f(v[c], src[q]);
+#pragma unroll
+        for (int i = 0; i < NC - 4; ++i) vary[q][i] = kz_bperm_f(v[4 + i], src[q]);
+    }
 }
 
 // Coverage constants: the interior-positive orientation and the top-left
@@ -462,9 +585,44 @@ __device__ __forceinline__ void kz_store_rec(const KzFrame &f, uint32_t g, const
     for (int q = 0; q < KZ_REC_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
 }
 
+// The record of a triangle that needs clipping: word 15 = 1, its vertex
+// shader outputs instead of a setup (clip position P[v][c] at word 4v + c,
+// varying A[v][i] at 16 + 3i + v), from which the raster clips it.
+__device__ __forceinline__ void kz_store_clip_rec(const KzFrame &f, uint32_t g, const float p[3][4],
+                                                  const float vary[3][KZ_NVA]) {
+    uint32_t w[KZ_REC_WORDS];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) w[4 * v + c] = __float_as_uint(p[v][c]);
+    w[12] = w[13] = w[14] = 0u;
+    w[15] = 1u;
+#pragma unroll
+    for (int i = 0; i < KZ_NV; ++i)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) w[16 + 3 * i + v] = __float_as_uint(vary[v][i]);
+#pragma unroll
+    for (int i = 16 + 3 * KZ_NV; i < KZ_REC_WORDS; ++i) w[i] = 0u;
+    uint4 *dst = (uint4 *)(f.tri_rec + (size_t)g * KZ_REC_WORDS);
+#pragma unroll
+    for (int q = 0; q < KZ_REC_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+__device__ __forceinline__ void kz_load_clip_rec(const KzFrame &f, uint32_t g, float p[3][4], float vary[3][KZ_NVA]) {
+    const uint32_t *src = f.tri_rec + (size_t)g * KZ_REC_WORDS;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[v][c] = __uint_as_float(src[4 * v + c]);
+#pragma unroll
+        for (int i = 0; i < KZ_NVA; ++i) vary[v][i] = i < KZ_NV ? __uint_as_float(src[16 + 3 * i + v]) : 0.0f;
+    }
+}
+
 // The record of triangle g and its pixel box: the values kz_setup and the
-// varying differences gave when the record was written.
-__device__ __forceinline__ void kz_load_rec(const KzFrame &f, uint32_t g, KzTri &t, float *a0, float *da1,
+// varying differences gave when the record was written. Returns whether the
+// triangle needs clipping (its other values are then meaningless).
+__device__ __forceinline__ bool kz_load_rec(const KzFrame &f, uint32_t g, KzTri &t, float *a0, float *da1,
                                             float *da2) {
     const uint4 *src = (const uint4 *)(f.tri_rec + (size_t)g * KZ_REC_WORDS);
     uint32_t w[KZ_REC_WORDS];
@@ -501,20 +659,22 @@ __device__ __forceinline__ void kz_load_rec(const KzFrame &f, uint32_t g, KzTri 
         da1[i] = __uint_as_float(w[17 + 3 * i]);
         da2[i] = __uint_as_float(w[18 + 3 * i]);
     }
+    return w[15] != 0u;
 }
 
 extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame f) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = kz_lane();
-    uint32_t n_setup = 0, n_unsup = 0, n_pairs = 0;
+    uint32_t n_setup = 0, n_clip = 0, n_pairs = 0;
+    float p[3][4], vary[3][KZ_NVA];
+    KzTri t;
+    int st = 0;
+    uint64_t rect = KZ_RECT_EMPTY;
     if (g < f.total_tris) {
-        float p[3][4], vary[3][KZ_NVA];
         const uint32_t draw = kz_find_draw(f, g);
         if (f.tri_draw) f.tri_draw[g] = draw;  // the raster's lookup, one load instead of a search
         kz_fetch_triangle(f, g, draw, p, vary);
-        KzTri t;
-        const int st = kz_setup(f, p, t);
-        uint64_t rect = KZ_RECT_EMPTY;
+        st = kz_setup(f, p, t);
         if (st == 1) {
             n_setup = 1;
             // A small triangle's box shrinks to the pixels it covers, and a
@@ -566,8 +726,44 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
                 const float hi = fminf(fmaxf(fmaxf(t.z0, fmaxf(z1, z2)), 0.0f), 1.0f);
                 f.tri_z[g] = (uint32_t)(lo * 65535.0f + 0.5f) | ((uint32_t)(hi * 65535.0f + 0.5f) << 16);
             }
-        } else if (st == 2) {
-            n_unsup = 1;
+        }
+    }
+    // Triangles that need clipping: the whole wave clips each in turn (lane
+    // j sets up piece j); the triangle is binned by the union of its
+    // pieces' pixel boxes, and counts as set up when one of them is.
+    uint64_t cm = kz_ballot(st == 2);
+    while (cm) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(cm);
+        cm &= cm - 1;
+        float v[4], q[3][4], qv[3][KZ_NVA];
+        const int np = kz_clip_polygon<4>(f, p, vary, b, lane, v);  // (the box needs positions only)
+        kz_clip_piece<4>(v, lane, q, qv);
+        KzTri pt;
+        const bool piece = (int)lane + 2 < np && kz_setup_rt(f, q, pt, true) == 1;
+        const bool boxed = piece && pt.px0 <= pt.px1 && pt.py0 <= pt.py1;
+        const int32_t none = -0x40000000;  // below any pixel coordinate (an empty box when no piece has one)
+        const int32_t bx0 = -kz_wave_max(boxed ? -pt.px0 : none, lane);
+        const int32_t by0 = -kz_wave_max(boxed ? -pt.py0 : none, lane);
+        const int32_t bx1 = kz_wave_max(boxed ? pt.px1 : none, lane);
+        const int32_t by1 = kz_wave_max(boxed ? pt.py1 : none, lane);
+        const bool any = kz_ballot(piece) != 0;
+        if (lane == b) {
+            n_clip = 1;
+            n_setup = any ? 1 : 0;
+            if (bx0 <= bx1 && by0 <= by1) {
+                t.px0 = bx0;
+                t.py0 = by0;
+                t.px1 = bx1;
+                t.py1 = by1;
+                rect = (uint64_t)(uint32_t)bx0 | ((uint64_t)(uint32_t)by0 << 16) | ((uint64_t)(uint32_t)bx1 << 32) |
+                       ((uint64_t)(uint32_t)by1 << 48);
+            }
+        }
+    }
+    if (g < f.total_tris) {
+        if (st == 2) {
+            if (KZ_SETUP_RECORDS && f.tri_rec && rect != KZ_RECT_EMPTY) kz_store_clip_rec(f, g, p, vary);
+            if (f.tri_z) f.tri_z[g] = 0xFFFF0000u;  // depth range [0, 1]: the pieces are not kept
         }
         f.tri_rect[g] = rect;
         // (triangle, tile) pairs: the tiles of the pixel box
@@ -575,12 +771,12 @@ extern "C" __global__ __launch_bounds__(KZ_BIN_BLOCK) void kz_bin_count(KzFrame 
             n_pairs = (uint32_t)(((t.px1 >> KZ_TILE_SHIFT) - (t.px0 >> KZ_TILE_SHIFT) + 1) *
                                  ((t.py1 >> KZ_TILE_SHIFT) - (t.py0 >> KZ_TILE_SHIFT) + 1));
     }
-    const uint64_t ms = kz_ballot(n_setup != 0), mu = kz_ballot(n_unsup != 0), mi = kz_ballot(g < f.total_tris);
+    const uint64_t ms = kz_ballot(n_setup != 0), mc = kz_ballot(n_clip != 0), mi = kz_ballot(g < f.total_tris);
     if (lane == 0) {
-        unsigned long long *st = kz_stats_stripe(f);
-        if (mi) atomicAdd(&st[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
-        if (ms) atomicAdd(&st[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
-        if (mu) atomicAdd(&st[KZ_STAT_TRIANGLES_UNSUPPORTED], (unsigned long long)__popcll(mu));
+        unsigned long long *sts = kz_stats_stripe(f);
+        if (mi) atomicAdd(&sts[KZ_STAT_TRIANGLES_IN], (unsigned long long)__popcll(mi));
+        if (ms) atomicAdd(&sts[KZ_STAT_TRIANGLES_SETUP], (unsigned long long)__popcll(ms));
+        if (mc) atomicAdd(&sts[KZ_STAT_TRIANGLES_CLIPPED], (unsigned long long)__popcll(mc));
     }
     if (f.block_pairs) {  // the block's pair total, for the pair offsets of radix binning
         __shared__ uint32_t s_pairs[KZ_BIN_BLOCK / 64];
@@ -945,6 +1141,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 
     unsigned long long nfrag = 0, npass = 0;
     uint32_t k = 0;
+    uint32_t ks = 0;  // first piece still to draw of a clipped triangle at entry k
     if (!filter) kz_ahead_indices(nx, f);
 
     while (k < n) {
@@ -979,7 +1176,8 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         // (without records) the vertex shader and setup once more. Without
         // records every lane runs them, a lane without a triangle on vertex
         // indices 0 of draw 0 (discarded), so that the code is straight-line.
-        bool ok;
+        bool ok, clip = false;
+        float p[3][4], vary[3][KZ_NVA];  // (without records) the lane's vertex shader outputs
         if (KZ_SETUP_RECORDS && f.tri_rec) {
             L.t.x0 = L.t.y0 = L.t.d1x = L.t.d1y = L.t.d2x = L.t.d2y = 0;
             L.t.S = 0;
@@ -989,13 +1187,43 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             L.t.px1 = L.t.py1 = -1;
 #pragma unroll
             for (int i = 0; i < KZ_NVA; ++i) L.a0[i] = L.da1[i] = L.da2[i] = 0.0f;
-            if (valid) kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
-            ok = valid;
+            if (valid) clip = kz_load_rec(f, g, L.t, L.a0, L.da1, L.da2);
+            ok = valid && !clip;
         } else {
-            float p[3][4], vary[3][KZ_NVA];
             kz_fetch_vertices(f, draw, idx, p, vary);
-            const int st = kz_setup(f, p, L.t);
-            ok = valid && st == 1;
+        }
+        // A triangle that needs clipping is drawn as its own batch of
+        // pieces (lane j: piece ks + j), in its place: a batch whose first
+        // triangle needs it draws that triangle's pieces, any other batch
+        // ends before its first such triangle. One setup serves the lane's
+        // triangle and, in a second pass, its piece.
+        bool clip_batch = false;
+        uint32_t first_clip = 64u, npieces = 0;
+        for (int pass = 0;; ++pass) {
+            if (!(KZ_SETUP_RECORDS && f.tri_rec) || pass) {
+                const int st = kz_setup_rt(f, p, L.t, pass != 0);
+                if (pass) {
+                    ok = ks + lane < npieces && st == 1;
+                    break;
+                }
+                ok = valid && st == 1;
+                clip = valid && st == 2;
+            }
+            const uint64_t clipm = kz_ballot(clip);
+            clip_batch = (clipm & 1ull) != 0;
+            first_clip = clipm ? (uint32_t)__builtin_ctzll(clipm) : 64u;
+            if (!clip_batch) {
+                ok = ok && lane < first_clip;
+                break;
+            }
+            if (KZ_SETUP_RECORDS && f.tri_rec)  // lane 0's triangle: its record holds the shader outputs
+                kz_load_clip_rec(f, (uint32_t)kz_readlane_i((int32_t)g, 0), p, vary);
+            float v[4 + KZ_NV];
+            const int np = kz_clip_polygon<4 + KZ_NV>(f, p, vary, 0, lane, v);
+            npieces = np >= 3 ? (uint32_t)(np - 2) : 0u;
+            kz_clip_piece<4 + KZ_NV>(v, ks + lane, p, vary);
+        }
+        if (!(KZ_SETUP_RECORDS && f.tri_rec) || clip_batch) {
 #pragma unroll
             for (int i = 0; i < KZ_NVA; ++i) {
                 L.a0[i] = i < KZ_NV ? vary[0][i] : 0.0f;
@@ -1074,11 +1302,21 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
 
         // --- batch cut: keep the longest lane prefix whose fragments fit ---
         const uint32_t incl = kz_wave_incl_scan(cnt, lane);
-        const uint32_t take = (uint32_t)__popcll(kz_ballot(incl <= KZ_FRAG_CAP));
+        // (a batch that is not a clip batch also ends before its first
+        // triangle that needs clipping)
+        const uint32_t take = min((uint32_t)__popcll(kz_ballot(incl <= KZ_FRAG_CAP)), clip_batch ? 64u : first_clip);
         const bool taken = lane < take;
         const uint32_t total = (uint32_t)kz_readlane_i((int32_t)incl, take - 1);
         const uint32_t excl = incl - cnt;
-        if (!filter && nx.k != k + take) kz_ahead_ids(nx, ids, k + take, n, lane);
+        // where the next batch starts: past the pieces or the lanes taken
+        uint32_t next_k, next_ks = 0;
+        if (clip_batch) {
+            next_ks = ks + take < npieces ? ks + take : 0u;
+            next_k = next_ks ? k : k + 1;
+        } else {
+            next_k = k + take;
+        }
+        if (!filter && nx.k != next_k) kz_ahead_ids(nx, ids, next_k, n, lane);
 
         // --- phase 2: write the fragment queue in primitive order ---------
         if (taken && small) {  // the covered candidates, in row-major order
@@ -1142,7 +1380,8 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             kz_resolve<true>(f, L, X0, Y0, total, lane, s_color, s_depth, s_claim, s_frag, nfrag, npass);
         else
             kz_resolve<false>(f, L, X0, Y0, total, lane, s_color, s_depth, s_claim, s_frag, nfrag, npass);
-        k += take;
+        k = next_k;
+        ks = next_ks;
         __syncthreads();
     }
 

@@ -27,6 +27,7 @@
 namespace kz {
 bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *ci, PipelineCache *cache,
                               Pipeline **out, std::string &err);
+bool create_compute_pipeline(Device *dev, const VkComputePipelineCreateInfo *ci, Pipeline **out, std::string &err);
 void destroy_pipeline(Pipeline *p);
 }  // namespace kz
 
@@ -38,6 +39,7 @@ static_assert(sizeof(VkPhysicalDeviceLimits) == 504, "VkPhysicalDeviceLimits lay
 static_assert(sizeof(VkPhysicalDeviceProperties) == 824, "VkPhysicalDeviceProperties layout");
 static_assert(sizeof(VkPhysicalDeviceFeatures) == 220, "VkPhysicalDeviceFeatures layout");
 static_assert(sizeof(VkGraphicsPipelineCreateInfo) == 144, "VkGraphicsPipelineCreateInfo layout");
+static_assert(sizeof(VkComputePipelineCreateInfo) == 96, "VkComputePipelineCreateInfo layout");
 static_assert(sizeof(VkPhysicalDeviceMemoryProperties) == 520, "VkPhysicalDeviceMemoryProperties layout");
 
 constexpr VkDeviceSize kBufferAlignment = 256;
@@ -984,6 +986,28 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice device, VkPipe
     return result;
 }
 
+// Reference: vkCreateComputePipelines (api_impl.rs) -> ComputePipeline::create
+// (pipeline.rs:122-154).
+VKAPI_ATTR VkResult VKAPI_CALL vkCreateComputePipelines(VkDevice device, VkPipelineCache, uint32_t count,
+                                                        const VkComputePipelineCreateInfo *ci,
+                                                        const VkAllocationCallbacks *, VkPipeline *pPipelines) {
+    Device *d = from<Device>(device);
+    set_device(d);
+    VkResult result = VK_SUCCESS;
+    for (uint32_t i = 0; i < count; ++i) {
+        Pipeline *p = nullptr;
+        std::string err;
+        if (!create_compute_pipeline(d, &ci[i], &p, err)) {
+            std::fprintf(stderr, "kazan: vkCreateComputePipelines: %s\n", err.c_str());
+            pPipelines[i] = nullptr;
+            result = VK_ERROR_INVALID_SHADER_NV;
+            continue;
+        }
+        pPipelines[i] = to<VkPipeline>(p);
+    }
+    return result;
+}
+
 VKAPI_ATTR void VKAPI_CALL vkDestroyPipeline(VkDevice device, VkPipeline pipeline, const VkAllocationCallbacks *) {
     if (!pipeline) return;
     set_device(from<Device>(device));
@@ -1105,9 +1129,14 @@ VKAPI_ATTR VkResult VKAPI_CALL vkResetCommandBuffer(VkCommandBuffer commandBuffe
     return VK_SUCCESS;
 }
 
-VKAPI_ATTR void VKAPI_CALL vkCmdBindPipeline(VkCommandBuffer commandBuffer, VkPipelineBindPoint, VkPipeline pipeline) {
+VKAPI_ATTR void VKAPI_CALL vkCmdBindPipeline(VkCommandBuffer commandBuffer, VkPipelineBindPoint point,
+                                             VkPipeline pipeline) {
     CommandBuffer *cb = from<CommandBuffer>(commandBuffer);
     Pipeline *p = from<Pipeline>(pipeline);
+    if (point == VK_PIPELINE_BIND_POINT_COMPUTE) {
+        cb->st.compute_pipeline = p;
+        return;
+    }
     cb->st.pipeline = p;
     if (!p->dynamic_viewport) cb->st.viewport = p->viewport;
     if (!p->dynamic_scissor) cb->st.scissor = p->scissor;
@@ -1145,12 +1174,55 @@ VKAPI_ATTR void VKAPI_CALL vkCmdBindIndexBuffer(VkCommandBuffer commandBuffer, V
     cb->st.ib_type = type;
 }
 
-VKAPI_ATTR void VKAPI_CALL vkCmdBindDescriptorSets(VkCommandBuffer commandBuffer, VkPipelineBindPoint,
+VKAPI_ATTR void VKAPI_CALL vkCmdBindDescriptorSets(VkCommandBuffer commandBuffer, VkPipelineBindPoint point,
                                                    VkPipelineLayout, uint32_t firstSet, uint32_t n,
                                                    const VkDescriptorSet *sets, uint32_t, const uint32_t *) {
     CommandBuffer *cb = from<CommandBuffer>(commandBuffer);
-    for (uint32_t i = 0; i < n && firstSet + i < 8; ++i) cb->st.sets[firstSet + i] = from<DescriptorSet>(sets[i]);
-    cb->st.batch_open = false;
+    DescriptorSet **slots = point == VK_PIPELINE_BIND_POINT_COMPUTE ? cb->st.compute_sets : cb->st.sets;
+    for (uint32_t i = 0; i < n && firstSet + i < 8; ++i) slots[firstSet + i] = from<DescriptorSet>(sets[i]);
+    if (point != VK_PIPELINE_BIND_POINT_COMPUTE) cb->st.batch_open = false;
+}
+
+// vkCmdDispatch / vkCmdDispatchBase (reference: api_impl.rs:4677, a stub):
+// the bound compute pipeline's kernel over x * y * z workgroups, with the
+// descriptors and push constants bound now, in order with the other
+// commands of the buffer (one stream executes them).
+static void record_dispatch(CommandBuffer *cb, uint32_t bx, uint32_t by, uint32_t bz, uint32_t x, uint32_t y,
+                            uint32_t z) {
+    RecordState &st = cb->st;
+    Pipeline *p = st.compute_pipeline;
+    if (!p || !p->compute || !p->cs_kernel || x == 0 || y == 0 || z == 0) return;
+    Op op;
+    op.kind = Op::Dispatch;
+    op.cs_kernel = p->cs_kernel;
+    op.groups[0] = x;
+    op.groups[1] = y;
+    op.groups[2] = z;
+    for (int k = 0; k < 3; ++k) op.block[k] = p->local_size[k];
+    KzComputeFrame &f = op.cframe;
+    f.base[0] = bx;
+    f.base[1] = by;
+    f.base[2] = bz;
+    std::memcpy(f.push, st.push, sizeof f.push);
+    auto buffer_at = [&](uint32_t set, uint32_t binding) -> uint8_t * {
+        if (set >= 8 || !st.compute_sets[set]) return nullptr;
+        auto it = st.compute_sets[set]->entries.find(binding);
+        if (it == st.compute_sets[set]->entries.end() || !it->second.buffer) return nullptr;
+        return it->second.buffer->address() + it->second.offset;
+    };
+    for (const auto &kv : p->ubo_slot) f.ubo[kv.second] = buffer_at(kv.first.first, kv.first.second);
+    for (const auto &kv : p->ssbo_slot) f.ssbo[kv.second] = buffer_at(kv.first.first, kv.first.second);
+    cb->ops.push_back(op);
+    st.batch_open = false;
+}
+
+VKAPI_ATTR void VKAPI_CALL vkCmdDispatch(VkCommandBuffer commandBuffer, uint32_t x, uint32_t y, uint32_t z) {
+    record_dispatch(from<CommandBuffer>(commandBuffer), 0, 0, 0, x, y, z);
+}
+
+VKAPI_ATTR void VKAPI_CALL vkCmdDispatchBase(VkCommandBuffer commandBuffer, uint32_t bx, uint32_t by, uint32_t bz,
+                                             uint32_t x, uint32_t y, uint32_t z) {
+    record_dispatch(from<CommandBuffer>(commandBuffer), bx, by, bz, x, y, z);
 }
 
 VKAPI_ATTR void VKAPI_CALL vkCmdPushConstants(VkCommandBuffer commandBuffer, VkPipelineLayout, VkShaderStageFlags,
@@ -1650,6 +1722,7 @@ std::unordered_map<std::string, Entry> &entries() {
         KZ_D(vkCreateFramebuffer);
         KZ_D(vkDestroyFramebuffer);
         KZ_D(vkCreateGraphicsPipelines);
+        KZ_D(vkCreateComputePipelines);
         KZ_D(vkDestroyPipeline);
         KZ_D(vkCreateCommandPool);
         KZ_D(vkDestroyCommandPool);
@@ -1670,6 +1743,8 @@ std::unordered_map<std::string, Entry> &entries() {
         KZ_D(vkCmdEndRenderPass);
         KZ_D(vkCmdDraw);
         KZ_D(vkCmdDrawIndexed);
+        KZ_D(vkCmdDispatch);
+        KZ_D(vkCmdDispatchBase);
         KZ_D(vkCmdCopyBuffer);
         KZ_D(vkCmdCopyBufferToImage);
         KZ_D(vkCmdCopyImageToBuffer);

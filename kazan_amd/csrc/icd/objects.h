@@ -139,6 +139,17 @@ struct Framebuffer {
 };
 
 struct Pipeline {
+    // a compute pipeline (vkCreateComputePipelines): its kernel and its
+    // workgroup size; the graphics fields below are unused
+    bool compute = false;
+    hipModule_t cs_module = nullptr;
+    hipFunction_t cs_kernel = nullptr;
+    uint32_t local_size[3] = {1, 1, 1};
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> ssbo_slot;
+    ~Pipeline() {
+        if (cs_module) hipModuleUnload(cs_module);
+    }
+
     FixedState fixed;
     PipelineVariants variants;  // compiled kernels per tile size
     bool dynamic_viewport = false, dynamic_scissor = false;
@@ -168,6 +179,10 @@ struct CommandPool {
 
 struct RecordState {
     Pipeline *pipeline = nullptr;
+    // the compute bind point (vkCmdBindPipeline / vkCmdBindDescriptorSets
+    // with VK_PIPELINE_BIND_POINT_COMPUTE); push constants are shared
+    Pipeline *compute_pipeline = nullptr;
+    DescriptorSet *compute_sets[8] = {};
     VkViewport viewport{};
     VkRect2D scissor{};
     Buffer *vb[KZ_MAX_BINDINGS] = {};

@@ -1,5 +1,5 @@
-// pipeline.cpp — vkCreateGraphicsPipelines: state capture, SPIR-V -> IR ->
-// HIP, kernel compilation and loading.
+// pipeline.cpp — vkCreateGraphicsPipelines / vkCreateComputePipelines:
+// state capture, SPIR-V -> IR -> HIP, kernel compilation and loading.
 //
 // The reference collects the shader stages with get_shader_stages!
 // (vulkan-driver/src/pipeline.rs:69-108: vertex required, fragment optional,
@@ -44,15 +44,17 @@ std::vector<SpecConstant> specs(const VkSpecializationInfo *si) {
 }  // namespace
 
 void build_slot_maps(const PipelineLayout *layout, std::map<std::pair<uint32_t, uint32_t>, uint32_t> &ubo,
-                     std::map<std::pair<uint32_t, uint32_t>, uint32_t> &tex) {
+                     std::map<std::pair<uint32_t, uint32_t>, uint32_t> &tex,
+                     std::map<std::pair<uint32_t, uint32_t>, uint32_t> *ssbo) {
     if (!layout) return;
-    uint32_t nu = 0, nt = 0;
+    uint32_t nu = 0, nt = 0, ns = 0;
     for (uint32_t s = 0; s < layout->set_layouts.size(); ++s) {
         const DescriptorSetLayout *l = layout->set_layouts[s];
         if (!l) continue;
         for (const auto &b : l->bindings) {
             if (b.type == VK_DESCRIPTOR_TYPE_UNIFORM_BUFFER && nu < KZ_MAX_UBOS) ubo[{s, b.binding}] = nu++;
             if (b.type == VK_DESCRIPTOR_TYPE_COMBINED_IMAGE_SAMPLER && nt < KZ_MAX_TEXTURES) tex[{s, b.binding}] = nt++;
+            if (ssbo && b.type == VK_DESCRIPTOR_TYPE_STORAGE_BUFFER && ns < KZ_MAX_SSBOS) (*ssbo)[{s, b.binding}] = ns++;
         }
     }
 }
@@ -93,6 +95,9 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
     auto p = std::make_unique<Pipeline>();
     FixedState &fx = p->fixed;
     fx.tile_shift = env_tile_shift();
+    // VK_PIPELINE_CREATE_DISABLE_OPTIMIZATION_BIT: the backend's
+    // NoOptimizations mode (reference pipeline.rs:110-120)
+    fx.optimize = (ci->flags & 0x1u) == 0;
     fx.cull_mode = rs->cullMode;
     fx.front_face = rs->frontFace;
     fx.samples = ci->pMultisampleState ? (uint32_t)ci->pMultisampleState->rasterizationSamples : 1u;
@@ -139,7 +144,7 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
     }
     p->layout = reinterpret_cast<PipelineLayout *>(ci->layout);
     p->subpass = ci->subpass;
-    build_slot_maps(p->layout, p->ubo_slot, p->tex_slot);
+    build_slot_maps(p->layout, p->ubo_slot, p->tex_slot, nullptr);
 
     ir::Module vmod, fmod;
     const ShaderModule *vsm = reinterpret_cast<const ShaderModule *>(vs->module);
@@ -199,6 +204,50 @@ bool create_graphics_pipeline(Device *dev, const VkGraphicsPipelineCreateInfo *c
     }
     // the default tile size now; others when the autotuner asks for them
     if (!p->variants.get(fx.tile_shift, &err)) return false;
+    *out = p.release();
+    return true;
+}
+
+// vkCreateComputePipelines: reference ComputePipeline::create
+// (vulkan-driver/src/pipeline.rs:122-154: the one compute stage, the
+// layout, the optimisation mode), continued here to a HIP kernel.
+bool create_compute_pipeline(Device *dev, const VkComputePipelineCreateInfo *ci, Pipeline **out, std::string &err) {
+    (void)dev;
+    const VkPipelineShaderStageCreateInfo &st = ci->stage;
+    if (st.stage != VK_SHADER_STAGE_COMPUTE_BIT) {
+        err = "a compute pipeline's stage must be VK_SHADER_STAGE_COMPUTE_BIT";
+        return false;
+    }
+    auto p = std::make_unique<Pipeline>();
+    p->compute = true;
+    p->layout = reinterpret_cast<PipelineLayout *>(ci->layout);
+    build_slot_maps(p->layout, p->ubo_slot, p->tex_slot, &p->ssbo_slot);
+    ir::Module cmod;
+    const ShaderModule *sm = reinterpret_cast<const ShaderModule *>(st.module);
+    if (!translate_spirv(sm->code.data(), sm->code.size(), st.pName ? st.pName : "main", ir::Stage::Compute,
+                         specs(st.pSpecializationInfo), cmod, err)) {
+        err = "compute shader: " + err;
+        return false;
+    }
+    PipelineCodegenInput cg;
+    cg.ubo_slot = p->ubo_slot;
+    cg.ssbo_slot = p->ssbo_slot;
+    std::string source;
+    if (!generate_compute_source(cmod, cg, source, err)) return false;
+    std::vector<char> code;
+    std::string log;
+    if (!compile_compute_code(source, (ci->flags & 0x1u) == 0, code, log, nullptr)) {
+        err = "kernel compilation failed:\n" + log;
+        return false;
+    }
+    hipError_t e = hipModuleLoadData(&p->cs_module, code.data());
+    if (e == hipSuccess) e = hipModuleGetFunction(&p->cs_kernel, p->cs_module, "kz_compute");
+    if (e != hipSuccess) {
+        err = std::string("loading the compute kernel: ") + hipGetErrorString(e);
+        return false;
+    }
+    for (int k = 0; k < 3; ++k) p->local_size[k] = cmod.local_size[k];
+    p->shader_source = source;
     *out = p.release();
     return true;
 }

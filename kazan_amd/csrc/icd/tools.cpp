@@ -11,6 +11,7 @@
 #include "../engine/engine.h"
 #include "../engine/kernels.h"
 #include "../shader/backend_hip.h"
+#include "../shader/ir.h"
 #include "../shader/spirv_reader.h"
 #include "../shader/translate.h"
 
@@ -52,15 +53,35 @@ extern "C" int kazan_translate_to_ir_text(const uint32_t *words, size_t nwords, 
     return put(kz::ir::module_text(m), out, cap, out_len, 0);
 }
 
-extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t what, char *out, size_t cap,
-                                            size_t *out_len) {
-    kz::ir::Module vm, fm;
+extern "C" int kazan_ir_round_trip(const char *text, uint32_t what, char *out, size_t cap, size_t *out_len) {
+    if (!text) return put("no text", out, cap, out_len, -1);
+    kz::ir::Module m;
     std::string err;
-    if (!kz::translate_spirv(d->vs_words, d->vs_nwords, "main", kz::ir::Stage::Vertex, {}, vm, err))
-        return put("vertex shader: " + err, out, cap, out_len, -1);
-    if (d->fs_words && !kz::translate_spirv(d->fs_words, d->fs_nwords, "main", kz::ir::Stage::Fragment, {}, fm, err))
-        return put("fragment shader: " + err, out, cap, out_len, -1);
-    kz::PipelineCodegenInput in;
+    if (what == 0) {
+        if (!kz::ir::parse_module_text(text, m, err)) return put(err, out, cap, out_len, -1);
+        return put(kz::ir::module_text(m), out, cap, out_len, 0);
+    }
+    kz::ir::Instruction ins;
+    if (!kz::ir::parse_instruction_text(text, m, ins, err)) return put(err, out, cap, out_len, -1);
+    return put(kz::ir::instruction_text(m, ins), out, cap, out_len, 0);
+}
+
+namespace kz {
+
+// A KazanPipelineDesc's shaders translated and lowered to HIP source, and
+// its fixed-function state (shared by the offline compile tool and the
+// backend entry points of kazan_backend.h).
+bool pipeline_from_desc(const KazanPipelineDesc *d, FixedState &fx, std::string &source, std::string &err) {
+    ir::Module vm, fm;
+    if (!translate_spirv(d->vs_words, d->vs_nwords, "main", ir::Stage::Vertex, {}, vm, err)) {
+        err = "vertex shader: " + err;
+        return false;
+    }
+    if (d->fs_words && !translate_spirv(d->fs_words, d->fs_nwords, "main", ir::Stage::Fragment, {}, fm, err)) {
+        err = "fragment shader: " + err;
+        return false;
+    }
+    PipelineCodegenInput in;
     in.vs = &vm;
     in.fs = d->fs_words ? &fm : nullptr;
     for (uint32_t i = 0; i < d->num_bindings && i < 8; ++i) in.bindings.push_back({i, d->binding_stride[i], d->binding_rate[i]});
@@ -69,10 +90,9 @@ extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t
     for (uint32_t i = 0; i < d->num_ubos && i < 4; ++i) in.ubo_slot[{d->ubo_key[i] >> 16, d->ubo_key[i] & 0xFFFF}] = i;
     for (uint32_t i = 0; i < d->num_textures && i < 4; ++i)
         in.tex_slot[{d->texture_key[i] >> 16, d->texture_key[i] & 0xFFFF}] = i;
-    kz::PipelineCodegenOutput co;
-    if (!kz::generate_pipeline_source(in, co, err)) return put(err, out, cap, out_len, -1);
-    if (what == 0) return put(co.source, out, cap, out_len, 0);
-    kz::FixedState fx;
+    PipelineCodegenOutput co;
+    if (!generate_pipeline_source(in, co, err)) return false;
+    source = co.source;
     fx.tile_shift = d->tile_shift ? d->tile_shift : 4;
     fx.samples = d->samples ? d->samples : 1;
     fx.num_varyings = co.num_varyings;
@@ -89,13 +109,57 @@ extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t
     fx.dst_alpha = d->dst_alpha;
     fx.alpha_op = d->alpha_op;
     fx.write_mask = d->write_mask;
-    if (what == 1) return put(kz::pipeline_full_source(fx, co.source), out, cap, out_len, 0);
+    return true;
+}
+
+}  // namespace kz
+
+extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t what, char *out, size_t cap,
+                                            size_t *out_len) {
+    kz::FixedState fx;
+    std::string source, err;
+    if (!kz::pipeline_from_desc(d, fx, source, err)) return put(err, out, cap, out_len, -1);
+    if (what == 0) return put(source, out, cap, out_len, 0);
+    if (what == 1) return put(kz::pipeline_full_source(fx, source), out, cap, out_len, 0);
     std::vector<char> code;
     std::string log;
     bool hit = false;
-    if (!kz::compile_pipeline_code(fx, co.source, code, log, &hit)) return put(log, out, cap, out_len, -1);
+    if (!kz::compile_pipeline_code(fx, source, code, log, &hit)) return put(log, out, cap, out_len, -1);
     return put("code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
-                   " varyings=" + std::to_string(co.num_varyings),
+                   " varyings=" + std::to_string(fx.num_varyings),
+               out, cap, out_len, 0);
+}
+
+extern "C" int kazan_compile_compute_spirv(const uint32_t *words, size_t nwords, const char *entry, uint32_t what,
+                                           char *out, size_t cap, size_t *out_len) {
+    kz::ir::Module m;
+    std::string err;
+    if (!words) return put("null module", out, cap, out_len, -1);
+    if (!kz::translate_spirv(words, nwords, entry ? entry : "main", kz::ir::Stage::Compute, {}, m, err))
+        return put(err, out, cap, out_len, -1);
+    // storage buffers and uniform buffers take slots in (set, binding) order,
+    // as a pipeline layout listing them in that order gives them
+    kz::PipelineCodegenInput cg;
+    std::vector<std::pair<uint32_t, uint32_t>> ssbos, ubos;
+    for (const auto &r : m.resources) {
+        if (r.kind == kz::ir::ResourceKind::StorageBuffer) ssbos.push_back({r.set, r.binding});
+        if (r.kind == kz::ir::ResourceKind::UniformBuffer) ubos.push_back({r.set, r.binding});
+    }
+    std::sort(ssbos.begin(), ssbos.end());
+    std::sort(ubos.begin(), ubos.end());
+    for (size_t i = 0; i < ssbos.size() && i < KZ_MAX_SSBOS; ++i) cg.ssbo_slot[ssbos[i]] = (uint32_t)i;
+    for (size_t i = 0; i < ubos.size() && i < KZ_MAX_UBOS; ++i) cg.ubo_slot[ubos[i]] = (uint32_t)i;
+    std::string source;
+    if (!kz::generate_compute_source(m, cg, source, err)) return put(err, out, cap, out_len, -1);
+    if (what == 0) return put(source, out, cap, out_len, 0);
+    if (what == 1) return put(kz::compute_full_source(source), out, cap, out_len, 0);
+    std::vector<char> code;
+    std::string log;
+    bool hit = false;
+    if (!kz::compile_compute_code(source, true, code, log, &hit)) return put(log, out, cap, out_len, -1);
+    return put("code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
+                   " local_size=" + std::to_string(m.local_size[0]) + "x" + std::to_string(m.local_size[1]) + "x" +
+                   std::to_string(m.local_size[2]),
                out, cap, out_len, 0);
 }
 

@@ -31,7 +31,10 @@ using ir::Op;
 using ir::Scalar;
 using ir::Type;
 
-enum class Root { None, In, Out, Position, VertexIndex, InstanceIndex, FragCoord, Push, Ubo, Local, Texture };
+enum class Root {
+    None, In, Out, Position, VertexIndex, InstanceIndex, FragCoord, Push, Ubo, Local, Texture,
+    Storage, Workgroup, ComputeId,  // compute: storage buffers, workgroup memory, the invocation's ids
+};
 
 struct PtrRef {
     Root root = Root::None;
@@ -67,8 +70,10 @@ struct StageGen {
     std::map<uint32_t, PtrRef> ptrs;
     std::map<uint32_t, uint32_t> local_index;  // pointer value -> local number
     std::vector<uint32_t> local_words;         // words of each local
+    std::vector<uint32_t> wg_words;            // words of each workgroup variable
     uint32_t in_words = 4, out_words = 4;
     bool is_fs;
+    bool is_cs = false;
 
     StageGen(const ir::Module &mm, const PipelineCodegenInput &p, std::string &e, bool fs)
         : m(mm), pin(p), err(e), is_fs(fs) {}
@@ -134,8 +139,19 @@ struct StageGen {
         case Root::Push:
             if (for_store) return fail("store to push constants");
             if (is_fs) return fail("push constants in a fragment shader are not supported");
-            out = "__uint_as_float(d.push[" + index_expr(w, p.dyn, KZ_PUSH_WORDS) + "])";
+            out = std::string("__uint_as_float(") + (is_cs ? "f" : "d") + ".push[" +
+                  index_expr(w, p.dyn, KZ_PUSH_WORDS) + "])";
             return true;
+        case Root::Storage:  // (a storage buffer's size is the application's: indices are not clamped)
+            out = "((float *)f.ssbo[" + std::to_string(p.index) + "])[" + ws + p.dyn + "]";
+            return true;
+        case Root::Workgroup:
+            out = "kz_wg" + std::to_string(p.index) + "[" + index_expr(w, p.dyn, wg_words.at(p.index)) + "]";
+            return true;
+        case Root::ComputeId:  // index: which id; the word selects x, y or z
+            if (w > 2) return fail("compute built-in index");
+            out = "__int_as_float((int32_t)kz_ids[" + std::to_string(p.index * 3 + w) + "])";
+            return !for_store || fail("store to a built-in input");
         case Root::Ubo:
             if (for_store) return fail("store to a uniform buffer");
             out = "((const float *)f.ubo[" + std::to_string(p.index) + "])[" + ws + p.dyn + "]";
@@ -438,6 +454,29 @@ struct StageGen {
         case Op::Extract:
             os << "    " << lhs(res(), 0) << " = " << comp(ins.args[0], (int)ins.fields[0]) << ";\n";
             return true;
+        case Op::Barrier:
+            if (!is_cs) return fail("barrier outside a compute shader");
+            os << "    __syncthreads();\n";
+            return true;
+        case Op::Atomic: {
+            auto it = ptrs.find(ins.args[0]);
+            if (it == ptrs.end()) return fail("atomic on an unknown pointer");
+            const PtrRef &p = it->second;
+            if (p.root != Root::Storage && p.root != Root::Workgroup && p.root != Root::Local)
+                return fail("atomic on memory other than a storage buffer, workgroup or function variable");
+            if (p.offset % 4) return fail("unaligned atomic");
+            std::string e;
+            if (!word_expr(p, p.offset / 4, true, e)) return false;
+            const std::string v = comp(ins.args[1], 0);
+            static const char *const fn[] = {"atomicAdd", "atomicSub", "atomicMin", "atomicMax", "atomicMin",
+                                             "atomicMax", "atomicAnd", "atomicOr",  "atomicXor", "atomicExch"};
+            const uint32_t k = ins.fields.at(0);
+            if (k > ir::AExchange) return fail("unknown atomic");
+            const bool u = k == ir::AUMin || k == ir::AUMax;
+            os << "    " << lhs(res(), 0) << " = (int32_t)" << fn[k] << "((" << (u ? "unsigned" : "int") << " *)&("
+               << e << "), (" << (u ? "unsigned" : "int") << ")" << v << ");\n";
+            return true;
+        }
         case Op::Shuffle: {
             const int na = lanes(ins.args[0]);
             for (int c = 0; c < lanes(res()); ++c) {
@@ -489,10 +528,18 @@ struct StageGen {
             ptrs[b.pointer] = {Root::Position, 0, 0};
         }
         for (const auto &b : m.built_in_inputs) {
-            Root r = b.kind == ir::BuiltIn::VertexIndex     ? Root::VertexIndex
-                     : b.kind == ir::BuiltIn::InstanceIndex ? Root::InstanceIndex
-                                                            : Root::FragCoord;
-            ptrs[b.pointer] = {r, 0, 0};
+            switch (b.kind) {
+            case ir::BuiltIn::VertexIndex: ptrs[b.pointer] = {Root::VertexIndex, 0, 0}; break;
+            case ir::BuiltIn::InstanceIndex: ptrs[b.pointer] = {Root::InstanceIndex, 0, 0}; break;
+            case ir::BuiltIn::FragCoord: ptrs[b.pointer] = {Root::FragCoord, 0, 0}; break;
+            // kz_ids: global id, local id, workgroup id, workgroup count (x, y, z each), local index
+            case ir::BuiltIn::GlobalInvocationId: ptrs[b.pointer] = {Root::ComputeId, 0, 0}; break;
+            case ir::BuiltIn::LocalInvocationId: ptrs[b.pointer] = {Root::ComputeId, 1, 0}; break;
+            case ir::BuiltIn::WorkgroupId: ptrs[b.pointer] = {Root::ComputeId, 2, 0}; break;
+            case ir::BuiltIn::NumWorkgroups: ptrs[b.pointer] = {Root::ComputeId, 3, 0}; break;
+            case ir::BuiltIn::LocalInvocationIndex: ptrs[b.pointer] = {Root::ComputeId, 4, 0}; break;
+            default: return fail("unsupported built-in input");
+            }
         }
         for (const auto &r : m.resources) {
             if (r.kind == ir::ResourceKind::PushConstants) {
@@ -501,7 +548,13 @@ struct StageGen {
                 auto it = pin.ubo_slot.find({r.set, r.binding});
                 if (it == pin.ubo_slot.end()) return fail("uniform buffer without a descriptor binding in the layout");
                 ptrs[r.pointer] = {Root::Ubo, it->second, 0};
+            } else if (r.kind == ir::ResourceKind::StorageBuffer) {
+                if (!is_cs) return fail("storage buffers are supported in compute shaders only");
+                auto it = pin.ssbo_slot.find({r.set, r.binding});
+                if (it == pin.ssbo_slot.end()) return fail("storage buffer without a descriptor binding in the layout");
+                ptrs[r.pointer] = {Root::Storage, it->second, 0};
             } else {
+                if (is_cs) return fail("combined image samplers in compute shaders are not supported");
                 auto it = pin.tex_slot.find({r.set, r.binding});
                 if (it == pin.tex_slot.end()) return fail("sampler without a descriptor binding in the layout");
                 ptrs[r.pointer] = {Root::Texture, it->second, 0};
@@ -520,6 +573,11 @@ struct StageGen {
         };
         add_locals(m.invocation_globals);
         add_locals(m.main.locals);
+        for (const auto &v : m.workgroup_variables) {  // (declared at file scope by the caller)
+            const uint32_t words = (std::max(v.size, 4u) + 3) / 4;
+            ptrs[v.pointer] = {Root::Workgroup, (uint32_t)wg_words.size(), 0, ""};
+            wg_words.push_back(words);
+        }
         // every value the function computes, declared up front
         for (const auto &blk : m.main.blocks)
             for (const auto &ins : blk.body)
@@ -532,7 +590,10 @@ struct StageGen {
                 }
         in_words = block_words_of(m.user_inputs);
         out_words = block_words_of(m.user_outputs);
-        if (is_fs)
+        if (is_cs)
+            os << "__device__ __forceinline__ bool " << fn_name
+               << "(const KzComputeFrame &f, const uint32_t *kz_ids, const float *kz_in, float *kz_out) {\n";
+        else if (is_fs)
             os << "__device__ __forceinline__ bool " << fn_name
                << "(const KzFrame &f, const float *kz_in, float *kz_out, const float *kz_frag_coord) {\n";
         else
@@ -666,6 +727,44 @@ bool generate_pipeline_source(const PipelineCodegenInput &in, PipelineCodegenOut
     }
     os << "}\n";
     out.source = os.str();
+    return true;
+}
+
+bool generate_compute_source(const ir::Module &cs, const PipelineCodegenInput &in, std::string &out,
+                             std::string &err) {
+    const uint32_t lx = cs.local_size[0], ly = cs.local_size[1], lz = cs.local_size[2];
+    const uint64_t n = (uint64_t)lx * ly * lz;
+    if (cs.stage != ir::Stage::Compute || n == 0 || n > 1024) {
+        err = "compute shader without a workgroup size of 1 .. 1024 invocations";
+        return false;
+    }
+    std::ostringstream os;
+    // workgroup memory: one array per variable, shared by the workgroup
+    for (size_t i = 0; i < cs.workgroup_variables.size(); ++i) {
+        const uint32_t words = (std::max(cs.workgroup_variables[i].size, 4u) + 3) / 4;
+        os << "__shared__ float kz_wg" << i << "[" << words << "];\n";
+    }
+    StageGen g(cs, in, err, false);
+    g.is_cs = true;
+    if (!g.generate("kz_cs_main")) return false;
+    os << g.os.str();
+    // the kernel: one workgroup per dispatched group, one thread per invocation
+    os << "extern \"C\" __global__ __launch_bounds__(" << n << ") void kz_compute(KzComputeFrame f) {\n";
+    os << "    const uint32_t L[3] = {" << lx << "u, " << ly << "u, " << lz << "u};\n";
+    os << "    const uint32_t lid[3] = {threadIdx.x, threadIdx.y, threadIdx.z};\n";
+    os << "    const uint32_t wid[3] = {blockIdx.x + f.base[0], blockIdx.y + f.base[1], blockIdx.z + f.base[2]};\n";
+    os << "    uint32_t kz_ids[13];\n";
+    os << "    for (int k = 0; k < 3; ++k) {\n";
+    os << "        kz_ids[k] = wid[k] * L[k] + lid[k];\n";
+    os << "        kz_ids[3 + k] = lid[k];\n";
+    os << "        kz_ids[6 + k] = wid[k];\n";
+    os << "    }\n";
+    os << "    kz_ids[9] = gridDim.x; kz_ids[10] = gridDim.y; kz_ids[11] = gridDim.z;\n";
+    os << "    kz_ids[12] = lid[0] + L[0] * (lid[1] + L[1] * lid[2]);\n";
+    os << "    float kz_in[4] = {}, kz_out[4] = {};\n";
+    os << "    kz_cs_main(f, kz_ids, kz_in, kz_out);\n";
+    os << "}\n";
+    out = os.str();
     return true;
 }
 

@@ -40,6 +40,7 @@ struct PipelineCodegenInput {
     std::vector<VertexAttributeDesc> attributes;
     // (set, binding) -> slot in KzFrame::ubo / KzFrame::tex
     std::map<std::pair<uint32_t, uint32_t>, uint32_t> ubo_slot, tex_slot;
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> ssbo_slot;  // compute: storage buffers
 };
 
 struct PipelineCodegenOutput {
@@ -49,5 +50,12 @@ struct PipelineCodegenOutput {
 };
 
 bool generate_pipeline_source(const PipelineCodegenInput &in, PipelineCodegenOutput &out, std::string &err);
+
+// A compute pipeline's source: the shader as device function kz_cs_main,
+// its workgroup memory, and the kernel kz_compute(KzComputeFrame) that runs
+// one workgroup per dispatched group (the module's local size as its block).
+// Uses in.ubo_slot and in.ssbo_slot for the descriptor bindings.
+bool generate_compute_source(const ir::Module &cs, const PipelineCodegenInput &in, std::string &out,
+                             std::string &err);
 
 }  // namespace kz

@@ -9,6 +9,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <string>
 
 namespace kz {
 namespace ir {
@@ -33,7 +34,7 @@ std::string hex(uint32_t v) {
     return b;
 }
 
-const char *op_text(Op op) {
+const char *op_text(Op op) {  // (op_name below)
     switch (op) {
     case Op::Add: return "add";
     case Op::FAdd: return "fadd";
@@ -112,6 +113,8 @@ const char *op_text(Op op) {
     case Op::ContinueLoop: return "continue";
     case Op::Switch: return "branch";
     case Op::Unreachable: return "unreachable";
+    case Op::Barrier: return "barrier";
+    case Op::Atomic: return "atomic";
     }
     return "?";
 }
@@ -119,6 +122,11 @@ const char *op_text(Op op) {
 const char *fcmp_text(uint32_t p) {
     static const char *const k[] = {"oeq", "one", "olt", "ole", "ogt", "oge", "ueq",
                                      "une", "ult", "ule", "ugt", "uge", "ord", "uno"};
+    return p < sizeof k / sizeof k[0] ? k[p] : "?";
+}
+
+const char *atomic_text(uint32_t p) {
+    static const char *const k[] = {"add", "sub", "smin", "smax", "umin", "umax", "and", "or", "xor", "exchange"};
     return p < sizeof k / sizeof k[0] ? k[p] : "?";
 }
 
@@ -158,16 +166,25 @@ std::string value_use(const Module &m, uint32_t id) {
     return s;
 }
 
-std::string variable_text(const Module &m, const Type &t, uint32_t align, uint32_t ptr) {
-    return type_text(t) + ", align: " + hex(align) + " -> " + m.value(ptr).name + ": data_ptr";
+// (`size` is printed when the storage is larger than one value of the type:
+// arrays and structs that the IR keeps in memory)
+std::string variable_text(const Module &m, const Type &t, uint32_t align, uint32_t ptr, uint32_t size = 0) {
+    std::string s = type_text(t) + ", align: " + hex(align);
+    if (size && size != t.size_bytes()) s += ", size: " + hex(size);
+    return s + " -> " + m.value(ptr).name + ": data_ptr";
 }
 
-const char *builtin_text(BuiltIn b) {
+const char *builtin_text(BuiltIn b) {  // (builtin_of below is its inverse)
     switch (b) {
     case BuiltIn::VertexPosition: return "vertex_position";
     case BuiltIn::VertexIndex: return "vertex_index";
     case BuiltIn::InstanceIndex: return "instance_index";
     case BuiltIn::FragCoord: return "fragment_coord";
+    case BuiltIn::GlobalInvocationId: return "global_invocation_id";
+    case BuiltIn::LocalInvocationId: return "local_invocation_id";
+    case BuiltIn::WorkgroupId: return "workgroup_id";
+    case BuiltIn::NumWorkgroups: return "num_workgroups";
+    case BuiltIn::LocalInvocationIndex: return "local_invocation_index";
     }
     return "?";
 }
@@ -190,27 +207,100 @@ std::string type_text(const Type &t) {
     return scalar_text(t.scalar);
 }
 
+namespace {
+
+// `want`, or want_2, want_3, ... : the first not in `taken`
+template <class Taken>
+std::string unique_name(const std::string &want, const Taken &taken) {
+    if (!taken(want)) return want;
+    for (uint32_t k = 2;; ++k) {
+        const std::string n = want + "_" + std::to_string(k);
+        if (!taken(n)) return n;
+    }
+}
+
+}  // namespace
+
 uint32_t Module::new_value(const Type &t, const std::string &name) {
     Value v;
     v.id = (uint32_t)values.size() + 1;
     v.type = t;
-    v.name = name.empty() ? "v" + std::to_string(v.id) : name;
+    const std::string want = name.empty() ? "v" + std::to_string(v.id) : name;
+    v.name = unique_name(want, [&](const std::string &n) { return value_names.count(n) != 0; });
+    value_names.insert(v.name);
     values.push_back(v);
     return v.id;
 }
 
-uint32_t Module::new_const_f32(float f) {
-    const uint32_t id = new_value(Type::f32(), "");
-    Value &v = value(id);
+uint32_t Module::new_const(const Type &t) {
+    Value v;
+    v.id = (uint32_t)values.size() + 1;
+    v.type = t;
     v.is_const = true;
+    values.push_back(v);
+    return v.id;
+}
+
+uint32_t Function::new_block(const std::string &name, bool loop_body) {
+    const std::string n = unique_name(name, [&](const std::string &c) {
+        for (const Block &b : blocks)
+            if (b.name == c) return true;
+        return false;
+    });
+    blocks.push_back(Block{n, {}, loop_body});
+    return (uint32_t)blocks.size() - 1;
+}
+
+const char *op_name(Op op) { return op_text(op); }
+
+bool builtin_of(const std::string &s, BuiltIn &out) {
+    for (BuiltIn b : {BuiltIn::VertexPosition, BuiltIn::VertexIndex, BuiltIn::InstanceIndex, BuiltIn::FragCoord,
+                      BuiltIn::GlobalInvocationId, BuiltIn::LocalInvocationId, BuiltIn::WorkgroupId,
+                      BuiltIn::NumWorkgroups, BuiltIn::LocalInvocationIndex})
+        if (s == builtin_text(b)) {
+            out = b;
+            return true;
+        }
+    return false;
+}
+
+bool fcmp_of(const std::string &s, uint32_t &out) {
+    for (uint32_t p = 0; p <= FUno; ++p)
+        if (s == fcmp_text(p)) {
+            out = p;
+            return true;
+        }
+    return false;
+}
+
+bool atomic_of(const std::string &s, uint32_t &out) {
+    for (uint32_t p = 0; p <= AExchange; ++p)
+        if (s == atomic_text(p)) {
+            out = p;
+            return true;
+        }
+    return false;
+}
+
+bool icmp_of(const std::string &s, uint32_t &out) {
+    for (uint32_t p = 0; p <= IUGe; ++p)
+        if (s == icmp_text(p)) {
+            out = p;
+            return true;
+        }
+    return false;
+}
+
+uint32_t Module::new_const_f32(float f) {
+    const uint32_t id = new_const(Type::f32());
+    Value &v = value(id);
     std::memcpy(&v.const_bits[0], &f, 4);
     return id;
 }
 
 uint32_t Module::new_const_i32(int32_t i) {
-    const uint32_t id = new_value(Type::i32(), "");
+    const uint32_t id = new_const(Type::i32());
     Value &v = value(id);
-    v.is_const = true;
     v.const_bits[0] = (uint32_t)i;
     return id;
 }
@@ -245,6 +335,7 @@ std::string instruction_text(const Module &m, const Instruction &ins) {
         case Op::Extract: s += ", index: " + std::to_string(ins.fields[0]); break;
         case Op::FCmp: s += ", " + std::string(fcmp_text(ins.fields[0])); break;
         case Op::ICmp: s += ", " + std::string(icmp_text(ins.fields[0])); break;
+        case Op::Atomic: s += ", " + std::string(atomic_text(ins.fields[0])); break;
         case Op::Shuffle: {
             s += ", lanes: [";
             for (size_t i = 0; i < ins.fields.size(); ++i) s += (i ? ", " : "") + std::to_string(ins.fields[i]);
@@ -331,15 +422,24 @@ std::string module_text(const Module &m) {
     print_block(p, m, m.user_outputs, "user_outputs_block");
     p.line("invocation_global_variables {");
     ++p.indent;
-    for (const auto &v : m.invocation_globals) p.line(variable_text(m, v.type, v.alignment, v.pointer) + ";");
+    for (const auto &v : m.invocation_globals) p.line(variable_text(m, v.type, v.alignment, v.pointer, v.size) + ";");
     --p.indent;
     p.line("}");
+    if (!m.workgroup_variables.empty()) {
+        p.line("workgroup_variables {");
+        ++p.indent;
+        for (const auto &v : m.workgroup_variables)
+            p.line(variable_text(m, v.type, v.alignment, v.pointer, v.size) + ";");
+        --p.indent;
+        p.line("}");
+    }
     if (!m.resources.empty()) {
         p.line("resources {");
         ++p.indent;
         for (const auto &r : m.resources) {
-            const char *k = r.kind == ResourceKind::PushConstants ? "push_constants"
+            const char *k = r.kind == ResourceKind::PushConstants   ? "push_constants"
                             : r.kind == ResourceKind::UniformBuffer ? "uniform_buffer"
+                            : r.kind == ResourceKind::StorageBuffer ? "storage_buffer"
                                                                      : "combined_image_sampler";
             std::string s = std::string(k) + " -> " + m.value(r.pointer).name + ": " + type_text(m.value(r.pointer).type);
             if (r.kind != ResourceKind::PushConstants)
@@ -359,7 +459,7 @@ std::string module_text(const Module &m) {
     p.line("}");
     p.line("{");
     ++p.indent;
-    for (const auto &v : m.main.locals) p.line(variable_text(m, v.type, v.alignment, v.pointer) + ";");
+    for (const auto &v : m.main.locals) p.line(variable_text(m, v.type, v.alignment, v.pointer, v.size) + ";");
     --p.indent;
     p.line("}");
     p.line(m.main.body().name + " {");

@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -130,7 +131,11 @@ enum class Op : uint8_t {
     ContinueLoop,  // field: the loop's body block
     Switch,    // [selector] + fields: default block, then (literal, block) pairs: breaks from the chosen block
     Unreachable,
+    Barrier,   // workgroup control + memory barrier (compute)
+    Atomic,    // [ptr, value] + field: AtomicOp -> [old value: i32]
 };
+
+enum AtomicOp : uint32_t { AAdd, ASub, ASMin, ASMax, AUMin, AUMax, AAnd, AOr, AXor, AExchange };
 
 enum FCmpPred : uint32_t { FOEq, FONe, FOLt, FOLe, FOGt, FOGe, FUEq, FUNe, FULt, FULe, FUGt, FUGe, FOrd, FUno };
 enum ICmpPred : uint32_t { IEq, INe, ISLt, ISLe, ISGt, ISGe, IULt, IULe, IUGt, IUGe };
@@ -149,7 +154,18 @@ struct Instruction {
     Location loc;
 };
 
-enum class BuiltIn : uint8_t { VertexPosition, VertexIndex, InstanceIndex, FragCoord };
+enum class BuiltIn : uint8_t {
+    VertexPosition,
+    VertexIndex,
+    InstanceIndex,
+    FragCoord,
+    // compute (all <3 x i32> but the index)
+    GlobalInvocationId,
+    LocalInvocationId,
+    WorkgroupId,
+    NumWorkgroups,
+    LocalInvocationIndex,
+};
 
 struct InterfaceVariable {
     Type type;
@@ -170,7 +186,7 @@ struct InterfaceBlock {
     std::vector<BlockMember> members;
 };
 
-enum class ResourceKind : uint8_t { PushConstants, UniformBuffer, CombinedImageSampler };
+enum class ResourceKind : uint8_t { PushConstants, UniformBuffer, CombinedImageSampler, StorageBuffer };
 
 struct Resource {
     ResourceKind kind;
@@ -202,10 +218,9 @@ struct Function {
     std::vector<Block> blocks{Block{}};
     Block &body() { return blocks[0]; }
     const Block &body() const { return blocks[0]; }
-    uint32_t new_block(const std::string &name, bool loop_body = false) {
-        blocks.push_back(Block{name, {}, loop_body});
-        return (uint32_t)blocks.size() - 1;
-    }
+    // (a name already taken gets a suffix: names are unique in a function,
+    // so the text form names each break and continue target unambiguously)
+    uint32_t new_block(const std::string &name, bool loop_body = false);
 };
 
 enum class Stage : uint8_t { Vertex, Fragment, Compute };
@@ -215,13 +230,22 @@ struct Module {
     std::vector<InterfaceVariable> built_in_inputs, built_in_outputs;
     InterfaceBlock user_inputs, user_outputs;
     std::vector<Variable> invocation_globals;
+    std::vector<Variable> workgroup_variables;  // compute: shared by a workgroup's invocations
     std::vector<Resource> resources;
+    uint32_t local_size[3] = {0, 0, 0};         // compute: the workgroup's size (not in the text form)
     std::vector<Value> values;  // index = id - 1
     Function main;
 
+    std::set<std::string> value_names;
+
     Value &value(uint32_t id) { return values[id - 1]; }
     const Value &value(uint32_t id) const { return values[id - 1]; }
+    // (a name already taken gets a suffix: value names are unique in a
+    // module, so the text form names each value unambiguously; an empty
+    // name becomes v<id>)
     uint32_t new_value(const Type &t, const std::string &name);
+    // a constant: printed by its bits, it takes no name
+    uint32_t new_const(const Type &t);
     uint32_t new_const_f32(float v);
     uint32_t new_const_i32(int32_t v);
 };
@@ -229,6 +253,19 @@ struct Module {
 // The module in the reference's textual form.
 std::string module_text(const Module &m);
 std::string instruction_text(const Module &m, const Instruction &ins);
+
+// The text form parsed back (ir_text.cpp): a module, or value definitions
+// "[name: type, ...]" followed by one instruction. False with a message
+// (line number first) on malformed text.
+bool parse_module_text(const std::string &text, Module &m, std::string &err);
+bool parse_instruction_text(const std::string &text, Module &m, Instruction &ins, std::string &err);
+
+// Names of the text form.
+const char *op_name(Op op);
+bool builtin_of(const std::string &s, BuiltIn &out);
+bool fcmp_of(const std::string &s, uint32_t &out);
+bool icmp_of(const std::string &s, uint32_t &out);
+bool atomic_of(const std::string &s, uint32_t &out);
 
 }  // namespace ir
 }  // namespace kz

@@ -2,10 +2,12 @@
 #include "translate.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
 
+#include "spirv_id_map.h"
 #include "spirv_reader.h"
 
 namespace kz {
@@ -43,19 +45,25 @@ enum SOp : uint16_t {
     OpFOrdGreaterThanEqual = 190, OpFUnordGreaterThanEqual = 191, OpShiftRightLogical = 194,
     OpShiftRightArithmetic = 195, OpShiftLeftLogical = 196, OpBitwiseOr = 197, OpBitwiseXor = 198,
     OpBitwiseAnd = 199, OpNot = 200, OpPhi = 245, OpSwitch = 251, OpReturnValue = 254, OpUnreachable = 255,
+    OpTypeRuntimeArray = 29, OpControlBarrier = 224, OpMemoryBarrier = 225, OpAtomicLoad = 227, OpAtomicStore = 228,
+    OpAtomicExchange = 229, OpAtomicIIncrement = 232, OpAtomicIDecrement = 233, OpAtomicIAdd = 234,
+    OpAtomicISub = 235, OpAtomicSMin = 236, OpAtomicUMin = 237, OpAtomicSMax = 238, OpAtomicUMax = 239,
+    OpAtomicAnd = 240, OpAtomicOr = 241, OpAtomicXor = 242,
 };
 
 enum Storage : uint32_t {
-    SC_UniformConstant = 0, SC_Input = 1, SC_Uniform = 2, SC_Output = 3, SC_Private = 6, SC_Function = 7,
-    SC_PushConstant = 9,
+    SC_UniformConstant = 0, SC_Input = 1, SC_Uniform = 2, SC_Output = 3, SC_Workgroup = 4, SC_Private = 6,
+    SC_Function = 7, SC_PushConstant = 9, SC_StorageBuffer = 12,
 };
 
 enum Deco : uint32_t {
-    D_SpecId = 1, D_Block = 2, D_ColMajor = 5, D_ArrayStride = 6, D_MatrixStride = 7, D_BuiltIn = 11, D_Flat = 14,
+    D_SpecId = 1, D_Block = 2, D_BufferBlock = 3, D_ColMajor = 5, D_ArrayStride = 6, D_MatrixStride = 7, D_BuiltIn = 11, D_Flat = 14,
     D_Location = 30, D_Component = 31, D_Binding = 33, D_DescriptorSet = 34, D_Offset = 35,
 };
 
-enum BuiltInS : uint32_t { BI_Position = 0, BI_PointSize = 1, BI_FragCoord = 15, BI_VertexIndex = 42,
+enum BuiltInS : uint32_t { BI_Position = 0, BI_PointSize = 1, BI_FragCoord = 15, BI_NumWorkgroups = 24,
+                           BI_WorkgroupSize = 25, BI_WorkgroupId = 26, BI_LocalInvocationId = 27,
+                           BI_GlobalInvocationId = 28, BI_LocalInvocationIndex = 29, BI_VertexIndex = 42,
                            BI_InstanceIndex = 43 };
 
 struct SType {
@@ -117,16 +125,17 @@ struct Translator {
     const SpirvModule &sm;
     ir::Module &m;
     std::string &err;
-    std::map<uint32_t, SType> types;
-    std::map<uint32_t, std::string> names;
-    std::map<uint32_t, std::vector<std::vector<uint32_t>>> decos;  // id -> [deco, params...]
+    // per-id tables, bounded by the module header's `bound`
+    IdMap<SType> types;
+    IdMap<std::string> names;
+    IdMap<std::vector<std::vector<uint32_t>>> decos;  // id -> [deco, params...]
     std::map<std::pair<uint32_t, uint32_t>, std::vector<std::vector<uint32_t>>> mdecos;
-    std::map<uint32_t, Val> vals;
-    std::map<uint32_t, std::pair<uint32_t, std::vector<uint32_t>>> consts;  // id -> (type, words)
-    std::map<uint32_t, std::vector<uint8_t>> spec;
+    IdMap<Val> vals;
+    IdMap<std::pair<uint32_t, std::vector<uint32_t>>> consts;  // id -> (type, words)
+    std::map<uint32_t, std::vector<uint8_t>> spec;  // by SpecId
     uint32_t glsl_set = 0;
     std::string file_name;
-    std::map<uint32_t, std::string> strings;
+    IdMap<std::string> strings;
     // builtin member map for block variables: (var id, member) -> IR pointer
     std::map<std::pair<uint32_t, uint32_t>, uint32_t> block_builtin_ptr;
     ir::Location cur_loc;
@@ -136,13 +145,30 @@ struct Translator {
     FnCfg *fcfg = nullptr;
     std::vector<Construct> constructs;
     size_t construct_base = 0;  // constructs of the function being emitted start here
-    std::map<uint32_t, uint32_t> phi_local;  // phi result id -> IR local pointer
+    IdMap<uint32_t> phi_local;  // phi result id -> IR local pointer
     int inline_depth = 0;
 
-    Translator(const SpirvModule &s, ir::Module &mm, std::string &e) : sm(s), m(mm), err(e) {}
+    Translator(const SpirvModule &s, ir::Module &mm, std::string &e)
+        : sm(s), m(mm), err(e), types(s.bound), names(s.bound), decos(s.bound), vals(s.bound), consts(s.bound),
+          strings(s.bound), phi_local(s.bound) {}
 
+    // an id outside [1, bound) that any table was asked to hold, 0 if none
+    uint32_t out_of_bounds() const {
+        for (uint32_t id : {types.out_of_bounds(), names.out_of_bounds(), decos.out_of_bounds(), vals.out_of_bounds(),
+                            consts.out_of_bounds(), strings.out_of_bounds(), phi_local.out_of_bounds()})
+            if (id) return id;
+        return 0;
+    }
+
+    // (an id outside the header's bound, once seen, is the error to report:
+    // what fails after it is its consequence)
     bool fail(const std::string &s) {
-        if (err.empty()) err = s;
+        if (err.empty()) {
+            const uint32_t id = out_of_bounds();
+            err = id ? "SPIR-V id %" + std::to_string(id) + " is out of bounds (header bound " +
+                           std::to_string(sm.bound) + ")"
+                     : s;
+        }
         return false;
     }
 
@@ -293,6 +319,15 @@ struct Translator {
 bool Translator::collect() {
     for (const auto &ins : sm.instructions) {
         const auto &o = ins.operands;
+        // ids every table below is keyed by must lie in [1, bound)
+        // (spirv-id-map/src/lib.rs:94-210: IdOutOfBounds)
+        if ((ins.opcode == OpLabel || ins.opcode == OpName || ins.opcode == OpDecorate || ins.opcode == OpString) &&
+            !o.empty() && !types.in_bounds(o[0]))
+            return fail("SPIR-V id %" + std::to_string(o[0]) + " is out of bounds (header bound " +
+                        std::to_string(sm.bound) + ")");
+        if ((ins.opcode == OpFunction || ins.opcode == OpVariable) && o.size() > 1 && !types.in_bounds(o[1]))
+            return fail("SPIR-V id %" + std::to_string(o[1]) + " is out of bounds (header bound " +
+                        std::to_string(sm.bound) + ")");
         switch (ins.opcode) {
         case OpName: names[o[0]] = spirv_string(o, 1); break;
         case OpString: strings[o[0]] = spirv_string(o, 1); break;
@@ -338,11 +373,12 @@ bool Translator::collect() {
             types[o[0]] = t;
             break;
         }
-        case OpTypeArray: {
+        case OpTypeArray:
+        case OpTypeRuntimeArray: {
             SType t;
             t.k = SType::Array;
             t.elem = o[1];
-            t.count = o[2];  // constant id
+            t.count = ins.opcode == OpTypeArray ? o[2] : 0;  // constant id; 0: sized at run time (no storage here)
             types[o[0]] = t;
             break;
         }
@@ -473,6 +509,18 @@ bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> 
                     iv.kind = ir::BuiltIn::FragCoord;
                     iv.pointer = m.new_value(Type::ptr(), "fragment_coord_in");
                     m.built_in_inputs.push_back(iv);
+                } else if (!out && stage == ir::Stage::Compute &&
+                           (b == BI_GlobalInvocationId || b == BI_LocalInvocationId || b == BI_WorkgroupId ||
+                            b == BI_NumWorkgroups || b == BI_LocalInvocationIndex)) {
+                    static const std::map<uint32_t, std::pair<ir::BuiltIn, const char *>> kinds = {
+                        {BI_GlobalInvocationId, {ir::BuiltIn::GlobalInvocationId, "global_invocation_id_in"}},
+                        {BI_LocalInvocationId, {ir::BuiltIn::LocalInvocationId, "local_invocation_id_in"}},
+                        {BI_WorkgroupId, {ir::BuiltIn::WorkgroupId, "workgroup_id_in"}},
+                        {BI_NumWorkgroups, {ir::BuiltIn::NumWorkgroups, "num_workgroups_in"}},
+                        {BI_LocalInvocationIndex, {ir::BuiltIn::LocalInvocationIndex, "local_invocation_index_in"}}};
+                    iv.kind = kinds.at(b).first;
+                    iv.pointer = m.new_value(Type::ptr(), kinds.at(b).second);
+                    m.built_in_inputs.push_back(iv);
                 } else {
                     return fail("unsupported built-in " + std::to_string(b));
                 }
@@ -533,7 +581,7 @@ bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> 
             }
             v.ir.push_back(ptr_offset(out ? m.user_outputs.pointer : m.user_inputs.pointer, off));
             vals[id] = v;
-        } else if (sc == SC_PushConstant || sc == SC_Uniform || sc == SC_UniformConstant) {
+        } else if (sc == SC_PushConstant || sc == SC_Uniform || sc == SC_UniformConstant || sc == SC_StorageBuffer) {
             ir::Resource r;
             if (sc == SC_PushConstant) {
                 r.kind = ir::ResourceKind::PushConstants;
@@ -542,7 +590,12 @@ bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> 
                 auto set = deco(id, D_DescriptorSet), bind = deco(id, D_Binding);
                 r.set = set ? (*set)[1] : 0;
                 r.binding = bind ? (*bind)[1] : 0;
-                if (sc == SC_Uniform) {
+                // a storage buffer: StorageBuffer storage, or (SPIR-V 1.0
+                // style) a Uniform block decorated BufferBlock
+                if (sc == SC_StorageBuffer || (sc == SC_Uniform && deco(pointee, D_BufferBlock))) {
+                    r.kind = ir::ResourceKind::StorageBuffer;
+                    r.pointer = m.new_value(Type::ptr(), value_name(id));
+                } else if (sc == SC_Uniform) {
                     r.kind = ir::ResourceKind::UniformBuffer;
                     r.pointer = m.new_value(Type::ptr(), value_name(id));
                 } else {
@@ -555,6 +608,17 @@ bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> 
             m.resources.push_back(r);
             v.layout = 1;
             v.ir.push_back(r.pointer);
+            vals[id] = v;
+        } else if (sc == SC_Workgroup) {
+            if (stage != ir::Stage::Compute) return fail("Workgroup storage outside a compute shader");
+            ir::Variable wv;
+            wv.size = size_of(pointee, 0);
+            wv.type = ir_type(pointee);
+            wv.alignment = 4;
+            wv.pointer = m.new_value(Type::ptr(), value_name(id));
+            m.workgroup_variables.push_back(wv);
+            v.layout = 0;
+            v.ir.push_back(wv.pointer);
             vals[id] = v;
         } else if (sc == SC_Private) {
             ir::Variable gv;
@@ -570,7 +634,6 @@ bool Translator::build_interface(uint32_t entry_fn, const std::vector<uint32_t> 
             return fail("unsupported storage class " + std::to_string(sc));
         }
     }
-    (void)stage;
     return true;
 }
 
@@ -1069,7 +1132,10 @@ bool Translator::call(const std::vector<uint32_t> &o) {
     cur_block = outer;
     if (!ok) return false;
     // the callee's ids may be defined again by another call
-    for (auto it = vals.begin(); it != vals.end();) it = before.count(it->first) ? std::next(it) : vals.erase(it);
+    std::vector<uint32_t> callee_ids;
+    for (const auto &kv : vals)
+        if (!before.count(kv.first)) callee_ids.push_back(kv.first);
+    for (uint32_t id : callee_ids) vals.erase(id);
     if (rt.k != SType::Void) {
         Val r;
         r.type = o[0];
@@ -1124,6 +1190,52 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
     case OpPhi: {
         const Type t = ir_type(o[0]);
         return define(o[0], o[1], emit(Op::Load, {phi_local.at(o[1])}, t, {4}, value_name(o[1])));
+    }
+    case OpControlBarrier:
+        if (m.stage != ir::Stage::Compute) return fail("OpControlBarrier outside a compute shader");
+        emit(Op::Barrier, {}, Type{});
+        return true;
+    case OpMemoryBarrier:  // (one workgroup runs on one CU: its memory is ordered at the control barrier)
+        return true;
+    case OpAtomicLoad: {
+        const Val *p = V(o[2]);
+        return p && load(o[0], o[1], *p);
+    }
+    case OpAtomicStore: {
+        const Val *p = V(o[0]);
+        const Val *v = V(o[3]);
+        return p && v && store(*p, *v);
+    }
+    case OpAtomicExchange:
+    case OpAtomicIIncrement:
+    case OpAtomicIDecrement:
+    case OpAtomicIAdd:
+    case OpAtomicISub:
+    case OpAtomicSMin:
+    case OpAtomicUMin:
+    case OpAtomicSMax:
+    case OpAtomicUMax:
+    case OpAtomicAnd:
+    case OpAtomicOr:
+    case OpAtomicXor: {
+        const Val *p = V(o[2]);
+        if (!p) return false;
+        const Type t = ir_type(o[0]);
+        if (t != Type::i32()) return fail("atomics on 32-bit integers only");
+        uint32_t value = 0;
+        if (ins.opcode == OpAtomicIIncrement || ins.opcode == OpAtomicIDecrement) {
+            value = m.new_const_i32(1);
+        } else {
+            const Val *v = V(o[5]);
+            if (!v) return false;
+            value = v->ir[0];
+        }
+        static const std::map<uint32_t, uint32_t> kind = {
+            {OpAtomicExchange, ir::AExchange}, {OpAtomicIIncrement, ir::AAdd}, {OpAtomicIDecrement, ir::ASub},
+            {OpAtomicIAdd, ir::AAdd}, {OpAtomicISub, ir::ASub}, {OpAtomicSMin, ir::ASMin}, {OpAtomicUMin, ir::AUMin},
+            {OpAtomicSMax, ir::ASMax}, {OpAtomicUMax, ir::AUMax}, {OpAtomicAnd, ir::AAnd}, {OpAtomicOr, ir::AOr},
+            {OpAtomicXor, ir::AXor}};
+        return define(o[0], o[1], emit(Op::Atomic, {p->ir[0], value}, t, {kind.at(ins.opcode)}, value_name(o[1])));
     }
     case OpFunctionCall: return call(o);
     case OpCopyObject: {
@@ -1620,6 +1732,9 @@ bool translate_spirv(const uint32_t *words, size_t nwords, const std::string &en
     Translator tr(sm, out, err);
     for (const auto &s : spec) tr.spec[s.id] = s.bytes;
     if (!tr.collect()) return false;
+    if (const uint32_t id = tr.out_of_bounds())
+        return tr.fail("SPIR-V id %" + std::to_string(id) + " is out of bounds (header bound " + std::to_string(sm.bound) +
+                       ")");
     const uint32_t model = stage == ir::Stage::Vertex ? 0u : stage == ir::Stage::Fragment ? 4u : 5u;
     uint32_t fn = 0;
     std::vector<uint32_t> iface;
@@ -1642,12 +1757,37 @@ bool translate_spirv(const uint32_t *words, size_t nwords, const std::string &en
     for (const auto &ins : sm.instructions) {
         if (ins.opcode != OpExecutionMode || ins.operands[0] != fn) continue;
         const uint32_t mode = ins.operands[1];
+        if (mode == 17 && ins.operands.size() >= 5) {  // LocalSize x y z
+            for (int k = 0; k < 3; ++k) out.local_size[k] = ins.operands[2 + k];
+        }
         if (mode == 7) upper_left = true;
         else if (mode == 8) return tr.fail("OriginLowerLeft is not supported (Vulkan requires OriginUpperLeft)");
     }
     if (!upper_left) return tr.fail("fragment shader without OriginUpperLeft");
+    if (stage == ir::Stage::Compute) {
+        const uint64_t n = (uint64_t)out.local_size[0] * out.local_size[1] * out.local_size[2];
+        if (n == 0) return tr.fail("compute shader without a LocalSize");
+        if (n > 1024) return tr.fail("workgroup of " + std::to_string(n) + " invocations (at most 1024)");
+    }
     if (!tr.build_interface(fn, iface, stage)) return false;
-    return tr.translate_function(fn);
+    if (!tr.translate_function(fn)) return false;
+    if (const uint32_t id = tr.out_of_bounds())
+        return tr.fail("SPIR-V id %" + std::to_string(id) + " is out of bounds (header bound " + std::to_string(sm.bound) +
+                       ")");
+    // KAZAN_IR_ROUND_TRIP=1: hand on the module as its text form parsed back
+    // (what a pipeline built from IR text would get; tests render with it)
+    const char *rt = std::getenv("KAZAN_IR_ROUND_TRIP");
+    if (rt && std::strcmp(rt, "0") != 0) {
+        ir::Module back;
+        if (!ir::parse_module_text(ir::module_text(out), back, err)) {
+            err = "IR text round trip: " + err;
+            return false;
+        }
+        back.stage = stage;
+        for (int k = 0; k < 3; ++k) back.local_size[k] = out.local_size[k];
+        out = std::move(back);
+    }
+    return true;
 }
 
 }  // namespace kz

@@ -296,6 +296,45 @@ def stress_4k(n_draws=2048, total=8_388_608, width=3840, height=2160, samples=4,
     return s
 
 
+def camera_inside(n_small=4000, n_big=400, width=256, height=192, n_draws=4, samples=1, seed=rng.DEFAULT_SEED):
+    """A camera standing inside the geometry: n_small small triangles (edges
+    up to 4 % of their distance) and n_big large ones (edges up to 5 units)
+    centred over a 60-degree perspective camera's view (near 0.5, far 20)
+    from depth 0.3 to 16, drawn in n_draws draws
+    with slightly different views (MVP push constants). Many large
+    triangles cross the near plane or reach behind the eye, some leave the
+    guard band or pass the far plane: the clipping workload. Small and large
+    triangles are interleaved in primitive order. Depth LESS, alpha blend,
+    interpolated colour."""
+    n = n_small + n_big
+    u = rng.uniform_f32(seed, n * 3).reshape(n, 3).astype(np.float64)
+    # centres over the view frustum, widened by 20 %, from just behind the
+    # eye to depth 16 (log-spaced, so that the near field is well populated)
+    depth = np.exp(np.log(0.3) + u[:, 2] * (np.log(16.0) - np.log(0.3)))
+    t = np.tan(np.radians(30.0)) * 1.2
+    c = np.stack([(u[:, 0] * 2 - 1) * depth * t * width / height, (u[:, 1] * 2 - 1) * depth * t, -depth], axis=1)
+    size = np.full(n, 0.02) * depth
+    big = rng.splitmix64(seed + 1, n) % np.uint64(max(1, n // max(1, n_big))) == 0
+    size[big] = 2.5
+    off = (rng.uniform_f32(seed + 2, n * 9).reshape(n, 3, 3).astype(np.float64) * 2 - 1) * size[:, None, None]
+    pos = (c[:, None, :] + off).reshape(n * 3, 3)
+    col = rng.uniform_f32(seed + 3, n * 3 * 4).reshape(n * 3, 4)
+    v = np.concatenate([pos.astype(np.float32), col], axis=1).astype(np.float32)
+    proj = _perspective(60.0, width / height, 0.5, 20.0)
+    draws, per = [], (n + n_draws - 1) // n_draws
+    for d in range(n_draws):
+        a, b = d * per, min(n, (d + 1) * per)
+        view = np.eye(4)  # turned about the vertical axis by 0.06 rad per draw
+        a_y = 0.06 * (d - (n_draws - 1) / 2.0)
+        view[0, 0], view[0, 2], view[2, 0], view[2, 2] = np.cos(a_y), np.sin(a_y), -np.sin(a_y), np.cos(a_y)
+        m = proj @ view
+        draws.append(dict(count=3 * (b - a), first=3 * a, vertex_offset=0, mvp=m.T.astype(np.float32).ravel()))
+    s = Scene("camera_inside_%d_%d_%dx%d" % (n_small, n_big, width, height), width, height, VS_MVP_POS3_COLOR4,
+              FS_COLOR, v, draws=draws, samples=samples)
+    s.blend = (1, BLEND_SRC_ALPHA, BLEND_ONE_MINUS_SRC_ALPHA, 0, BLEND_ONE, BLEND_ONE_MINUS_SRC_ALPHA, 0)
+    return s
+
+
 WORKLOADS = {
     "triangles_256": lambda seed=rng.DEFAULT_SEED: random_triangles(1024, 256, 256, seed),
     "grid_1080p": lambda seed=rng.DEFAULT_SEED: jittered_grid(512, 1024, 1920, 1080, 1024, seed),

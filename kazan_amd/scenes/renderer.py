@@ -180,10 +180,14 @@ class Context:
 
 
 class SceneRenderer:
-    """Everything one Scene needs; render() records nothing new."""
+    """Everything one Scene needs; render() records nothing new.
+    `before_pass(renderer, cb)`, if given, records commands into the frame's
+    command buffer ahead of the render pass (a dispatch that writes the
+    vertex buffer, say)."""
 
-    def __init__(self, ctx, scene):
+    def __init__(self, ctx, scene, before_pass=None):
         self.ctx, self.s = ctx, scene
+        self.before_pass = before_pass
         d, dev = ctx.d, ctx.device
         s = scene
         self.keep = []
@@ -345,6 +349,8 @@ class SceneRenderer:
         ra = s.render_area or (0, 0, s.width, s.height)
         rbi = vk.VkRenderPassBeginInfo(vk.ST_RENDER_PASS_BEGIN_INFO, None, self.rp, self.fb,
                                        vk.VkRect2D(vk.VkOffset2D(ra[0], ra[1]), vk.VkExtent2D(ra[2], ra[3])), 2, clears)
+        if self.before_pass is not None:
+            self.before_pass(self, cb)
         d.vkCmdBeginRenderPass(cb, _ref(rbi), 0)
         d.vkCmdBindPipeline(cb, 0, self.pipeline)
         d.vkCmdBindVertexBuffers(cb, 0, 1, (vk.Handle * 1)(self.vbuf[0]), (ctypes.c_uint64 * 1)(0))

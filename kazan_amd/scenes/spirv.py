@@ -76,6 +76,8 @@ OpReturnValue = 254
 OpIAdd = 128
 OpISub = 130
 OpIMul = 132
+OpUDiv = 134
+OpUMod = 137
 OpSDiv = 135
 OpSMod = 139
 OpFMod = 141
@@ -89,6 +91,13 @@ OpFOrdGreaterThanEqual = 190
 OpBitwiseAnd = 199
 OpShiftRightLogical = 194
 OpConvertFToU = 109
+OpConvertUToF = 112
+OpTypeRuntimeArray = 29
+OpControlBarrier = 224
+OpAtomicIAdd = 234
+OpULessThan = 176
+OpUGreaterThan = 172
+OpShiftLeftLogical = 196
 
 # enumerants
 CapabilityShader = 1
@@ -97,6 +106,20 @@ MemoryGLSL450 = 1
 ExecutionModelVertex = 0
 ExecutionModelFragment = 4
 ExecutionModeOriginUpperLeft = 7
+ExecutionModelGLCompute = 5
+ExecutionModeLocalSize = 17
+StorageWorkgroup = 4
+DecorationBufferBlock = 3
+BuiltInNumWorkgroups = 24
+BuiltInWorkgroupId = 26
+BuiltInLocalInvocationId = 27
+BuiltInGlobalInvocationId = 28
+BuiltInLocalInvocationIndex = 29
+ScopeDevice = 1
+ScopeWorkgroup = 2
+SemanticsAcquireRelease = 0x8
+SemanticsUniformMemory = 0x40
+SemanticsWorkgroupMemory = 0x100
 StorageUniformConstant = 0
 StorageInput = 1
 StorageUniform = 2
@@ -231,6 +254,15 @@ class Module:
 
     def t_vec(self, n):
         return self._type(("vec", n), OpTypeVector, self.t_float(), n)
+
+    def t_uvec(self, n):
+        return self._type(("uvec", n), OpTypeVector, self.t_int(False), n)
+
+    def t_runtime_array(self, elem):
+        """A new runtime array type per call (its ArrayStride is decorated per use)."""
+        i = self.id()
+        self.globals += self._inst(OpTypeRuntimeArray, i, elem)
+        return i
 
     def t_mat4(self):
         return self._type(("mat4",), OpTypeMatrix, self.t_vec(4), 4)
@@ -432,3 +464,10 @@ class Function:
     def ret(self):
         self._emit(OpReturn)
         self._emit(OpFunctionEnd)
+
+    def barrier(self, exec_scope, mem_scope, semantics):
+        """OpControlBarrier; the operands are constant ids."""
+        self._emit(OpControlBarrier, exec_scope, mem_scope, semantics)
+
+    def atomic_iadd(self, type_id, ptr, scope, semantics, value):
+        return self._result(OpAtomicIAdd, type_id, ptr, scope, semantics, value)

@@ -54,6 +54,7 @@ ST_PIPELINE_DEPTH_STENCIL_STATE_CREATE_INFO = 25
 ST_PIPELINE_COLOR_BLEND_STATE_CREATE_INFO = 26
 ST_PIPELINE_DYNAMIC_STATE_CREATE_INFO = 27
 ST_GRAPHICS_PIPELINE_CREATE_INFO = 28
+ST_COMPUTE_PIPELINE_CREATE_INFO = 29
 ST_PIPELINE_LAYOUT_CREATE_INFO = 30
 ST_SAMPLER_CREATE_INFO = 31
 ST_DESCRIPTOR_SET_LAYOUT_CREATE_INFO = 32
@@ -80,8 +81,12 @@ MEMORY_PROPERTY_HOST_COHERENT = 0x4
 
 SHADER_STAGE_VERTEX = 0x1
 SHADER_STAGE_FRAGMENT = 0x10
+SHADER_STAGE_COMPUTE = 0x20
 DESCRIPTOR_TYPE_COMBINED_IMAGE_SAMPLER = 1
 DESCRIPTOR_TYPE_UNIFORM_BUFFER = 6
+DESCRIPTOR_TYPE_STORAGE_BUFFER = 7
+PIPELINE_BIND_POINT_GRAPHICS = 0
+PIPELINE_BIND_POINT_COMPUTE = 1
 ATTACHMENT_LOAD_OP_LOAD = 0
 ATTACHMENT_LOAD_OP_CLEAR = 1
 ATTACHMENT_UNUSED = 0xFFFFFFFF
@@ -408,6 +413,12 @@ class VkGraphicsPipelineCreateInfo(Structure):
                 ("basePipelineIndex", c_int32)]
 
 
+class VkComputePipelineCreateInfo(Structure):
+    _fields_ = [("sType", c_int32), ("pNext", c_void_p), ("flags", c_uint32),
+                ("stage", VkPipelineShaderStageCreateInfo), ("layout", Handle), ("basePipelineHandle", Handle),
+                ("basePipelineIndex", c_int32)]
+
+
 class VkAttachmentDescription(Structure):
     _fields_ = [("flags", c_uint32), ("format", c_int32), ("samples", c_uint32), ("loadOp", c_int32),
                 ("storeOp", c_int32), ("stencilLoadOp", c_int32), ("stencilStoreOp", c_int32),
@@ -509,7 +520,7 @@ ABI_STRUCTS = [VkApplicationInfo, VkPipelineCacheCreateInfo, VkInstanceCreateInf
                VkPipelineRasterizationStateCreateInfo, VkPipelineMultisampleStateCreateInfo,
                VkPipelineDepthStencilStateCreateInfo, VkPipelineColorBlendAttachmentState,
                VkPipelineColorBlendStateCreateInfo, VkPipelineDynamicStateCreateInfo, VkGraphicsPipelineCreateInfo,
-               VkAttachmentDescription, VkAttachmentReference, VkSubpassDescription, VkRenderPassCreateInfo,
+               VkComputePipelineCreateInfo, VkAttachmentDescription, VkAttachmentReference, VkSubpassDescription, VkRenderPassCreateInfo,
                VkFramebufferCreateInfo, VkCommandPoolCreateInfo, VkCommandBufferAllocateInfo,
                VkCommandBufferBeginInfo, VkClearValue, VkRenderPassBeginInfo, VkBufferCopy, VkBufferImageCopy,
                VkSubmitInfo, VkFenceCreateInfo]
@@ -564,6 +575,8 @@ _SIGS = {
     "vkDestroyFramebuffer": (None, [Handle, Handle, c_void_p]),
     "vkCreateGraphicsPipelines": (c_int32, [Handle, Handle, c_uint32, P(VkGraphicsPipelineCreateInfo), c_void_p,
                                             P(Handle)]),
+    "vkCreateComputePipelines": (c_int32, [Handle, Handle, c_uint32, P(VkComputePipelineCreateInfo), c_void_p,
+                                           P(Handle)]),
     "vkDestroyPipeline": (None, [Handle, Handle, c_void_p]),
     "vkCreateCommandPool": (c_int32, [Handle, P(VkCommandPoolCreateInfo), c_void_p, P(Handle)]),
     "vkDestroyCommandPool": (None, [Handle, Handle, c_void_p]),
@@ -582,6 +595,8 @@ _SIGS = {
     "vkCmdEndRenderPass": (None, [Handle]),
     "vkCmdDraw": (None, [Handle, c_uint32, c_uint32, c_uint32, c_uint32]),
     "vkCmdDrawIndexed": (None, [Handle, c_uint32, c_uint32, c_uint32, c_int32, c_uint32]),
+    "vkCmdDispatch": (None, [Handle, c_uint32, c_uint32, c_uint32]),
+    "vkCmdDispatchBase": (None, [Handle, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32]),
     "vkCmdCopyBuffer": (None, [Handle, Handle, Handle, c_uint32, P(VkBufferCopy)]),
     "vkCmdCopyBufferToImage": (None, [Handle, Handle, Handle, c_int32, c_uint32, P(VkBufferImageCopy)]),
     "vkCmdCopyImageToBuffer": (None, [Handle, Handle, c_int32, Handle, c_uint32, P(VkBufferImageCopy)]),
@@ -604,7 +619,7 @@ _SIGS = {
 
 KERNEL_NAMES = ["clear_counts", "bin_count", "bin_hist", "bin_scan", "bin_scatter", "bin_sort", "plan", "raster"]
 
-STAT_NAMES = ["triangles_in", "triangles_setup", "triangles_unsupported", "fragments", "depth_passed",
+STAT_NAMES = ["triangles_in", "triangles_setup", "triangles_clipped", "fragments", "depth_passed",
               "list_entries", "list_overflow", "split_entries"]
 
 

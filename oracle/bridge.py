@@ -35,7 +35,7 @@ class KzoDraw(ctypes.Structure):
 
 
 class KzoStats(ctypes.Structure):
-    _fields_ = [(n, c_u64) for n in ("triangles_in", "triangles_setup", "triangles_unsupported", "fragments",
+    _fields_ = [(n, c_u64) for n in ("triangles_in", "triangles_setup", "triangles_clipped", "fragments",
                                      "depth_passed")]
 
 
@@ -61,6 +61,8 @@ def lib():
         _lib.kzo_vertex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_u32, ctypes.c_void_p, ctypes.c_void_p]
         _lib.kzo_fragment.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.kzo_fragment.restype = ctypes.c_int
+        _lib.kzo_clip_polygon.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.kzo_clip_polygon.restype = ctypes.c_int
     return _lib
 
 
@@ -105,6 +107,17 @@ def vertex(scene, draw_index, vi):
     lib().kzo_vertex(ctypes.byref(st), ctypes.byref(draws[draw_index]), vi, pos.ctypes.data, var.ctypes.data)
     del keep
     return pos, var
+
+
+def clip_polygon(scene, pos):
+    """The oracle's R-CLIP on three clip positions (3, 4): the polygon's
+    vertex clip positions, shape (n, 4) float32."""
+    st, keep = make_state(scene)
+    p = np.ascontiguousarray(pos, np.float32).reshape(12)
+    poly = np.zeros((12, 4), np.float32)
+    n = lib().kzo_clip_polygon(ctypes.byref(st), p.ctypes.data, poly.ctypes.data)
+    del keep
+    return poly[:n].copy()
 
 
 def fragment(scene, var):

@@ -100,29 +100,36 @@ void run_vs(const KzoState &st, const KzoDraw &d, uint32_t vi, Vtx &o) {
 inline int64_t floor_div16(int64_t v) { return v >> 4; }
 
 // Returns 0 = drop (culled/degenerate/clip-rejected), 1 = set up,
-// 2 = unsupported (would need z or guard-band clipping).
-int setup_triangle(const KzoState &st, const Vtx *v, Tri &t) {
-    // Trivial reject against the clip volume -w<=x,y<=w, 0<=z<=w.
-    for (int plane = 0; plane < 6; ++plane) {
-        int out = 0;
+// 2 = needs clipping (R-CLIP: the caller clips it and sets up the pieces).
+// `piece`: v is a piece of a clipped triangle; the clip checks are skipped
+// and a vertex with w <= 0 or outside the guard band drops the piece.
+int setup_triangle(const KzoState &st, const Vtx *v, Tri &t, bool piece = false) {
+    if (!piece) {
+        // Trivial reject against the clip volume -w<=x,y<=w, 0<=z<=w.
+        for (int plane = 0; plane < 6; ++plane) {
+            int out = 0;
+            for (int k = 0; k < 3; ++k) {
+                const float *p = v[k].pos;
+                bool o = false;
+                switch (plane) {
+                case 0: o = p[0] < -p[3]; break;
+                case 1: o = p[0] > p[3]; break;
+                case 2: o = p[1] < -p[3]; break;
+                case 3: o = p[1] > p[3]; break;
+                case 4: o = p[2] < 0.0f; break;
+                case 5: o = p[2] > p[3]; break;
+                }
+                out += o;
+            }
+            if (out == 3) return 0;
+        }
         for (int k = 0; k < 3; ++k) {
             const float *p = v[k].pos;
-            bool o = false;
-            switch (plane) {
-            case 0: o = p[0] < -p[3]; break;
-            case 1: o = p[0] > p[3]; break;
-            case 2: o = p[1] < -p[3]; break;
-            case 3: o = p[1] > p[3]; break;
-            case 4: o = p[2] < 0.0f; break;
-            case 5: o = p[2] > p[3]; break;
-            }
-            out += o;
+            if (!(p[3] > 0.0f) || !(p[2] >= 0.0f) || !(p[2] <= p[3])) return 2;
         }
-        if (out == 3) return 0;
-    }
-    for (int k = 0; k < 3; ++k) {
-        const float *p = v[k].pos;
-        if (!(p[3] > 0.0f) || !(p[2] >= 0.0f) || !(p[2] <= p[3])) return 2;
+    } else {
+        for (int k = 0; k < 3; ++k)
+            if (!(v[k].pos[3] > 0.0f)) return 0;
     }
     const float hw = st.vp_w * 0.5f, hh = st.vp_h * 0.5f;
     const float ox = st.vp_x + hw, oy = st.vp_y + hh;
@@ -134,7 +141,7 @@ int setup_triangle(const KzoState &st, const Vtx *v, Tri &t) {
         float xf = xd * hw + ox;
         float yf = yd * hh + oy;
         zf[k] = zd * dz + st.vp_min_depth;
-        if (!(xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f)) return 2;
+        if (!(xf >= -8192.0f && xf <= 8191.0f && yf >= -8192.0f && yf <= 8191.0f)) return piece ? 0 : 2;
         t.x[k] = kzo_snap(xf);
         t.y[k] = kzo_snap(yf);
         t.iw[k] = 1.0f / p[3];
@@ -169,6 +176,89 @@ int setup_triangle(const KzoState &st, const Vtx *v, Tri &t) {
     t.py0 = (int32_t)std::max(py0, cy0);
     t.py1 = (int32_t)std::min(py1, cy1);
     return 1;
+}
+
+// R-CLIP: Sutherland-Hodgman clipping of a triangle that needs it (see
+// setup_triangle) against z >= 0, z <= w and the guard band's four planes
+// x = gx * w, y = gy * w (screen x, y = -8000 and 8000 px, inside
+// viewportBoundsRange [-8192, 8191], api_impl.rs:1568). A polygon vertex is
+// a vertex record: clip position and varyings. Per plane the vertices are
+// visited in order: an inside vertex (d >= 0) is kept, and an edge whose
+// ends are on different sides adds its intersection, always interpolated
+// from the inside end, every component of the record alike:
+// t = d_in / (d_in - d_out), x = x_in + t * (x_out - x_in) (so the shared
+// edge of two triangles gets the same point from both). At most kClipMax
+// vertices are kept; a plane that leaves fewer than 3 drops the triangle.
+// The polygon is drawn as the fan (0, j + 1, j + 2), j = 0 .. n - 3, each
+// piece set up like a triangle, in place of the original in primitive
+// order.
+constexpr int kClipMax = 12;
+
+struct ClipBand {
+    float xl, xh, yl, yh;
+};
+
+ClipBand clip_band(const KzoState &st) {
+    const float hw = st.vp_w * 0.5f, hh = st.vp_h * 0.5f;
+    const float ox = st.vp_x + hw, oy = st.vp_y + hh;
+    const float ax = (-8000.0f - ox) / hw, bx = (8000.0f - ox) / hw;
+    const float ay = (-8000.0f - oy) / hh, by = (8000.0f - oy) / hh;
+    return {std::fmin(ax, bx), std::fmax(ax, bx), std::fmin(ay, by), std::fmax(ay, by)};
+}
+
+inline float clip_distance(int plane, const ClipBand &g, const float P[4]) {
+    switch (plane) {
+    case 0: return P[2];
+    case 1: return P[3] - P[2];
+    case 2: return P[0] - g.xl * P[3];
+    case 3: return g.xh * P[3] - P[0];
+    case 4: return P[1] - g.yl * P[3];
+    default: return g.yh * P[3] - P[1];
+    }
+}
+
+inline float clip_lerp(float in, float out, float t) { return in + t * (out - in); }
+
+// The clipped polygon's vertex records; returns its vertex count (0: nothing left).
+int clip_polygon(const KzoState &st, const Vtx *v, Vtx poly[kClipMax]) {
+    const ClipBand g = clip_band(st);
+    Vtx a[kClipMax], b[kClipMax];
+    for (int k = 0; k < 3; ++k) a[k] = v[k];
+    int n = 3;
+    for (int plane = 0; plane < 6; ++plane) {
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            const int j = i + 1 == n ? 0 : i + 1;
+            const float di = clip_distance(plane, g, a[i].pos), dj = clip_distance(plane, g, a[j].pos);
+            const bool ini = di >= 0.0f, inj = dj >= 0.0f;
+            if (ini && m < kClipMax) b[m++] = a[i];
+            if (ini != inj && m < kClipMax) {
+                const Vtx &vin = ini ? a[i] : a[j], &vout = ini ? a[j] : a[i];
+                const float din = ini ? di : dj, dout = ini ? dj : di;
+                const float tt = din / (din - dout);
+                for (int c = 0; c < 4; ++c) b[m].pos[c] = clip_lerp(vin.pos[c], vout.pos[c], tt);
+                for (int c = 0; c < kMaxVar; ++c) b[m].var[c] = clip_lerp(vin.var[c], vout.var[c], tt);
+                ++m;
+            }
+        }
+        if (m < 3) return 0;
+        for (int i = 0; i < m; ++i) a[i] = b[i];
+        n = m;
+    }
+    for (int i = 0; i < n; ++i) poly[i] = a[i];
+    return n;
+}
+
+// The fan pieces of a triangle that needs clipping that set up (R-CLIP).
+int clip_pieces(const KzoState &st, const Vtx *v, Tri *out) {
+    Vtx poly[kClipMax];
+    const int n = clip_polygon(st, v, poly);
+    int k = 0;
+    for (int j = 0; j + 2 < n; ++j) {
+        const Vtx p[3] = {poly[0], poly[j + 1], poly[j + 2]};
+        if (setup_triangle(st, p, out[k], true) == 1) ++k;
+    }
+    return k;
 }
 
 struct Edges {
@@ -449,6 +539,17 @@ void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4],
 
 int kzo_fragment(const KzoState *st, const float *var, float out[4]) { return run_fs(*st, var, out) ? 1 : 0; }
 
+int kzo_clip_polygon(const KzoState *st, const float *pos, float *poly) {
+    Vtx v[3]{};
+    for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 4; ++c) v[k].pos[c] = pos[4 * k + c];
+    Vtx p[kClipMax];
+    const int n = clip_polygon(*st, v, p);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < 4; ++c) poly[4 * i + c] = p[i].pos[c];
+    return n;
+}
+
 int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8_t *color, float *depth,
                uint8_t *resolved, KzoStats *stats, int threads) {
     const KzoState &st = *stp;
@@ -475,7 +576,9 @@ int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8
         for (uint32_t d = 0; d < ndraws; ++d)
             for (uint32_t t = 0; t < draws[d].count / 3; ++t) where[k++] = {d, t};
     }
-    auto setup_range = [&](size_t a, size_t b) {
+    // the pieces of clipped triangles, per setup thread, in primitive order
+    std::vector<std::vector<std::pair<size_t, std::vector<Tri>>>> clipped(threads);
+    auto setup_range = [&](size_t a, size_t b, int tid) {
         for (size_t k = a; k < b; ++k) {
             const KzoDraw &d = draws[where[k].first];
             const uint32_t t = where[k].second;
@@ -489,6 +592,11 @@ int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8
                 run_vs(st, d, vi, v[j]);
             }
             status[k] = (uint8_t)setup_triangle(st, v, all[k]);
+            if (status[k] == 2) {
+                Tri pieces[kClipMax];
+                const int np = clip_pieces(st, v, pieces);
+                clipped[tid].emplace_back(k, std::vector<Tri>(pieces, pieces + np));
+            }
         }
     };
     {
@@ -496,16 +604,28 @@ int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8
         const size_t chunk = (total + threads - 1) / threads;
         for (int i = 0; i < threads; ++i) {
             size_t a = i * chunk, b = std::min(total, a + chunk);
-            if (a < b) pool.emplace_back(setup_range, a, b);
+            if (a < b) pool.emplace_back(setup_range, a, b, i);
         }
         for (auto &th : pool) th.join();
     }
     std::vector<Tri> tris;
     tris.reserve(total);
-    uint64_t unsupported = 0;
-    for (size_t k = 0; k < total; ++k) {
-        if (status[k] == 1) tris.push_back(all[k]);
-        else if (status[k] == 2) ++unsupported;
+    uint64_t n_clipped = 0, n_setup = 0;
+    {
+        int ci = 0;
+        size_t cj = 0;
+        for (size_t k = 0; k < total; ++k) {
+            if (status[k] == 1) {
+                tris.push_back(all[k]);
+                ++n_setup;
+            } else if (status[k] == 2) {
+                while (cj >= clipped[ci].size()) ++ci, cj = 0;
+                const auto &pc = clipped[ci][cj++];
+                ++n_clipped;
+                n_setup += pc.second.empty() ? 0 : 1;
+                for (const Tri &q : pc.second) tris.push_back(q);
+            }
+        }
     }
     all.clear();
     all.shrink_to_fit();
@@ -536,8 +656,8 @@ int kzo_render(const KzoState *stp, const KzoDraw *draws, uint32_t ndraws, uint8
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->triangles_in = total;
-        stats->triangles_setup = tris.size();
-        stats->triangles_unsupported = unsupported;
+        stats->triangles_setup = n_setup;
+        stats->triangles_clipped = n_clipped;
         for (auto &b : bstats) stats->fragments += b.fragments, stats->depth_passed += b.depth_passed;
     }
     return 0;

@@ -47,6 +47,13 @@
  *  R-ORD  fragments are applied in primitive order (draw order, then
  *         triangle order within a draw).
  *  R-RES  4x resolve: (s0 + s1 + s2 + s3 + 2) >> 2 per channel.
+ *  R-CLIP a triangle not trivially rejected with a vertex outside
+ *         0 <= z <= w (or w <= 0) or outside the guard band [-8192, 8191]
+ *         is clipped in homogeneous space against z >= 0, z <= w and the
+ *         planes of screen x, y = -8000 and 8000 px, vertex weights
+ *         interpolated from the inside end of each crossing edge; the
+ *         polygon is drawn as a fan of pieces in the triangle's place in
+ *         primitive order (raster_oracle.cpp, clip_polygon).
  *  R-KILL a fragment shader that discards (OpKill) runs before the depth
  *         test: a discarded fragment is counted as a fragment but takes no
  *         part in the depth test, depth write or blend.
@@ -115,8 +122,9 @@ typedef struct KzoDraw {
 
 typedef struct KzoStats {
     uint64_t triangles_in;       /* primitives assembled */
-    uint64_t triangles_setup;    /* survived clip/cull/degenerate */
-    uint64_t triangles_unsupported; /* needed z/guard-band clipping: dropped */
+    uint64_t triangles_setup;    /* survived clip/cull/degenerate (a clipped
+                                    triangle: at least one piece did) */
+    uint64_t triangles_clipped;  /* went through R-CLIP */
     uint64_t fragments;          /* covered (triangle, sample) pairs */
     uint64_t depth_passed;       /* samples that passed the depth test */
 } KzoStats;
@@ -141,6 +149,10 @@ uint32_t kzo_encode_unorm8(float v);
  * interpolated varyings (unclamped RGBA). */
 void kzo_vertex(const KzoState *st, const KzoDraw *d, uint32_t vi, float pos[4], float var[8]);
 int kzo_fragment(const KzoState *st, const float *var, float out[4]); /* 0: discarded */
+/* R-CLIP on the clip positions pos[3][4]: writes the polygon's vertex
+ * clip positions to poly[12][4] and returns its vertex count (0: clipped
+ * away). */
+int kzo_clip_polygon(const KzoState *st, const float *pos, float *poly);
 
 #ifdef __cplusplus
 }

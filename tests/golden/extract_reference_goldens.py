@@ -8,6 +8,9 @@ tree, so tests reassemble them from the dumps' own offsets and opcodes):
       the expected text of the parser's dump -> spirv_parse_test{,2,3}.dump
   shader-compiler-translate-spirv-to-ir/test_data/trivial_test.kazan-ir:
       the IR text of a translated empty GLCompute shader -> trivial_test.kazan-ir
+  shader-compiler-ir/test_data/test_assert_ir_matches_file.kazan-ir: the IR
+      text of the module the IR crate's own text test builds
+      -> test_assert_ir_matches_file.kazan-ir
 """
 import os
 import re
@@ -26,11 +29,12 @@ def main(ref):
         with open(out, "w") as f:
             f.write(m.group(1))
         print("wrote", out)
-    ir = os.path.join(ref, "shader-compiler-translate-spirv-to-ir/test_data/trivial_test.kazan-ir")
-    out = os.path.join(HERE, "trivial_test.kazan-ir")
-    with open(ir) as f, open(out, "w") as g:
-        g.write(f.read())
-    print("wrote", out)
+    for rel in ("shader-compiler-translate-spirv-to-ir/test_data/trivial_test.kazan-ir",
+                "shader-compiler-ir/test_data/test_assert_ir_matches_file.kazan-ir"):
+        out = os.path.join(HERE, os.path.basename(rel))
+        with open(os.path.join(ref, rel)) as f, open(out, "w") as g:
+            g.write(f.read())
+        print("wrote", out)
 
 
 if __name__ == "__main__":

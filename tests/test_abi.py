@@ -60,8 +60,9 @@ def test_exported_symbols():
     icd = {"vk_icdGetInstanceProcAddr", "vk_icdNegotiateLoaderICDInterfaceVersion",
            "vk_icdGetPhysicalDeviceProcAddr"}
     tools = _header_functions("kazan_tools.h", r"\bint (kazan_\w+)\(")
-    assert icd <= exports and tools <= exports, (icd | tools) - exports
-    assert len(tools) >= 5
+    backend = _header_functions("kazan_backend.h", r"KAZAN_EXPORT [\w\s\*]*?\b(kazan_\w+)\(")
+    assert icd <= exports and tools <= exports and backend <= exports, (icd | tools | backend) - exports
+    assert len(tools) >= 5 and len(backend) == 8, backend
     # nothing else with a C name: every vk* entry point goes through the loader
     assert {e for e in exports if e.startswith("vk")} <= icd, sorted(e for e in exports if e.startswith("vk"))
 
@@ -99,3 +100,11 @@ def test_cost_model_eval_rejects_bad_blob(driver):
     assert fn(bytes(blob[:-4]), len(blob) - 4, None, 0, None) < 0   # wrong size
     blob[4] = 9                                                     # wrong version
     assert fn(bytes(blob), len(blob), None, 0, None) < 0
+
+
+def test_backend_name():
+    """Compiler::name (shader-compiler-backend/src/lib.rs): the backend's
+    name, without a device."""
+    lib = ctypes.CDLL(vk.DRIVER_PATH)
+    lib.kazan_backend_name.restype = ctypes.c_char_p
+    assert lib.kazan_backend_name() == b"hip-gfx950"

@@ -47,7 +47,7 @@ def check_parity(ctx, oracle, scene, frames=1):
     oc, od, ores, ost = oracle.render(scene)
     assert st["fragments"] == frames * ost["fragments"], (st, ost)
     assert st["triangles_setup"] == frames * ost["triangles_setup"]
-    assert st["triangles_unsupported"] == frames * ost["triangles_unsupported"]
+    assert st["triangles_clipped"] == frames * ost["triangles_clipped"]
     cov_gpu, cov_cpu = d != np.float32(scene.clear_depth), od != np.float32(scene.clear_depth)
     np.testing.assert_array_equal(cov_gpu, cov_cpu)
     np.testing.assert_array_equal(d.view(np.uint32), od.view(np.uint32))  # depth bit-exact
@@ -236,7 +236,7 @@ def test_offscreen_degenerate_and_clipped(mctx, oracle):
     v[3:6, 0:2] = v[3, 0:2]                  # a zero-area triangle
     s = g.Scene("mixed_clip", 64, 64, g.VS_POS4_COLOR4, g.FS_COLOR, v)
     st, ost, _ = check_parity(mctx, oracle, s)
-    assert ost["triangles_unsupported"] > 0
+    assert ost["triangles_clipped"] > 0
 
 
 def test_repeated_submissions_accumulate(mctx, oracle):

@@ -62,7 +62,7 @@ def test_full_size_config(form_ctx, oracle, name):
           flush=True)
     oc, od, ores, ost = _oracle(oracle, name)
     print("  oracle: %d fragments per frame" % ost["fragments"], flush=True)
-    assert ost["triangles_unsupported"] == 0
+    assert ost["triangles_clipped"] == 0
     assert st["fragments"] == 2 * ost["fragments"]
     assert st["triangles_setup"] == 2 * ost["triangles_setup"]
     cov_gpu, cov_cpu = d != np.float32(s.clear_depth), od != np.float32(s.clear_depth)
