@@ -88,3 +88,48 @@ def test_error_stats():
     s = T.error_stats(f * 1.1, f)
     assert s["tiles_ge_100_frags"] == 3 and abs(s["median_rel_err_ge_100"] - 0.1) < 1e-9
     assert T.error_stats(f, f)["r2_log2"] == 1.0
+
+
+def _feat_of(tris):
+    f = np.zeros((len(tris), M.FEATURES), np.float32)
+    f[:, 0] = np.log2(1.0 + np.asarray(tris, np.float64))
+    return f
+
+
+def test_tile_study_when_fragments_follow_triangles():
+    """Fragments proportional to triangles: correlation 1, the two plans
+    rank the tiles alike, cycles follow both."""
+    from kazan_amd.balancer.tile_study import study
+    r = np.random.default_rng(4)
+    tris = r.integers(1, 2000, 3000).astype(np.float64)
+    frags = 0.5 * tris
+    cycles = 1000 + 300 * frags
+    res = study(_feat_of(tris), frags, cycles, frags.copy(), 1.0)
+    assert res["tiles"] == 3000
+    assert abs(res["fragments_per_triangle"] - 0.5) < 1e-3
+    assert res["pearson_frags_tris"] > 0.999 and res["spearman_frags_tris"] > 0.999
+    assert res["r2_frags_from_tris_through_origin"] > 0.999
+    assert res["r2_cycles_from_tris"] > 0.999 and res["r2_model_pred_vs_frags"] > 0.999
+    assert res["spearman_static_vs_model_cost"] > 0.999
+    assert res["top5%_overlap_static_model"] > 0.95 and res["top2%_overlap_static_cycles"] > 0.95
+
+
+def test_tile_study_when_fragments_do_not_follow_triangles():
+    """Two populations: many tiny triangles with few fragments, few big ones
+    with many. The triangle count misranks the expensive tiles, the model's
+    cost (given its predictions) does not."""
+    from kazan_amd.balancer.tile_study import study
+    r = np.random.default_rng(5)
+    fine = r.integers(800, 1200, 1500).astype(np.float64)    # many triangles, ~0.3 px each
+    coarse = r.integers(20, 60, 1500).astype(np.float64)     # few triangles, ~60 px each
+    tris = np.concatenate([fine, coarse])
+    frags = np.concatenate([0.3 * fine, 60.0 * coarse])
+    cycles = 500 + 350 * frags
+    res = study(_feat_of(tris), frags, cycles, frags.copy(), 4.0)
+    assert res["pearson_frags_tris"] < 0.0
+    assert res["r2_cycles_from_frags"] > 0.999 and res["r2_cycles_from_tris"] < 0.9
+    assert res["top5%_overlap_static_cycles"] < 0.1
+    assert res["top5%_overlap_model_cycles"] > 0.9
+    # tiles without triangles are left out
+    res2 = study(_feat_of(np.r_[tris, 0, 0]), np.r_[frags, 0, 0], np.r_[cycles, 0, 0], np.r_[frags, 0, 0], 4.0)
+    assert res2["tiles"] == res["tiles"]
