@@ -22,7 +22,6 @@ own kernels can only add time. The plain form profiles its last batch, one
 per draw: study single-draw workloads (a multi-draw one shows its last draw).
 """
 import argparse
-import ctypes
 import json
 import os
 import sys

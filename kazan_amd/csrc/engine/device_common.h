@@ -91,6 +91,7 @@ struct KzFrame {
     uint32_t *tile_offsets;  // [tiles] first entry of the tile's list (any value when empty)
     uint32_t *tile_cursor;   // [tiles] scatter cursors
     uint32_t *tile_flag;     // [tiles] nonzero: cut by the list capacity (radix binning), or null
+    const uint32_t *bin_ctrl;  // KZ_BIN_* words of radix binning (the first cut chunk), or null
     uint32_t *tile_lists;    // [list_capacity] triangle ids, per tile, sorted
     uint32_t *sort_tmp;      // [list_capacity] merge scratch for long lists
     uint32_t list_capacity;

@@ -339,6 +339,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
     f.tile_offsets = tile_offsets_;
     f.tile_cursor = tile_cursor_;
     f.tile_flag = radix_ ? tile_flag_ : nullptr;
+    f.bin_ctrl = radix_ ? bin_ctrl_ : nullptr;
     f.tile_lists = tile_lists_;
     f.sort_tmp = sort_tmp_;
     f.list_capacity = list_capacity_;

@@ -1113,10 +1113,11 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
 // did not fit its storage: the parent tile's list, or every triangle).
 // Returns the square's fragment count (to every lane).
 __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32_t Y0, uint32_t RS,
-                                             const uint32_t *ids, uint32_t n, bool filter, uint32_t *s_color,
-                                             float *s_depth, uint32_t *s_claim, uint32_t *s_frag) {
+                                             const uint32_t *ids, uint32_t n, bool filter, uint32_t clear_mask,
+                                             uint32_t *s_color, float *s_depth, uint32_t *s_claim,
+                                             uint32_t *s_frag) {
     const uint32_t lane = kz_lane();
-    if (n == 0 && !f.clear_mask) return 0;  // (with folded clears the square is written regardless)
+    if (n == 0 && !clear_mask) return 0;  // (with folded clears the square is written regardless)
     const int32_t RW = 1 << RS;
     const uint32_t *gcolor = (const uint32_t *)f.color;
 
@@ -1131,8 +1132,8 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             const size_t gp = ((size_t)y * f.width + x) * KZ_SAMPLES;
 #pragma unroll
             for (int s = 0; s < KZ_SAMPLES; ++s) {
-                s_color[i * KZ_SAMPLES + s] = (f.clear_mask & 1u) ? f.clear_color : gcolor[gp + s];
-                s_depth[i * KZ_SAMPLES + s] = (f.clear_mask & 2u) ? f.clear_depth : f.depth[gp + s];
+                s_color[i * KZ_SAMPLES + s] = (clear_mask & 1u) ? f.clear_color : gcolor[gp + s];
+                s_depth[i * KZ_SAMPLES + s] = (clear_mask & 2u) ? f.clear_depth : f.depth[gp + s];
             }
         }
         s_claim[i] = KZ_NO_CLAIM;
@@ -1140,7 +1141,9 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
     __syncthreads();
 
     unsigned long long nfrag = 0, npass = 0;
-    uint32_t k = 0;
+    // a scan of the triangles (filter without ids) starts at the first cut
+    // chunk's first triangle: the ones before it were in the tile's list
+    uint32_t k = filter && !ids && f.bin_ctrl ? min(f.bin_ctrl[KZ_BIN_CUT] * KZ_PAIR_CHUNK, n) : 0u;
     uint32_t ks = 0;  // first piece still to draw of a clipped triangle at entry k
     if (!filter) kz_ahead_indices(nx, f);
 
@@ -1426,19 +1429,24 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFram
     uint32_t off, n;
     kz_tile_list(f, tile, off, n);
     const uint64_t t0 = f.tile_cycles ? clock64() : 0;
-    unsigned long long frags = 0;
-    // entries past list_capacity were dropped by the scatter: such a tile
-    // takes its triangles from an in-order scan of all pixel boxes
-    if (n == 0)  // nothing to draw; with folded clears the tile is still written
-        frags = f.clear_mask ? kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, nullptr, 0, false, s_color, s_depth, s_claim,
-                                              s_frag)
-                             : 0;
-    else if ((uint64_t)off + n <= (uint64_t)f.list_capacity)
-        frags = kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, f.tile_lists + off, n, false, s_color, s_depth, s_claim,
-                               s_frag);
-    else
-        frags = kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, nullptr, f.total_tris, true, s_color, s_depth, s_claim,
-                               s_frag);
+    // A tile flagged by a cut (the pairs of the triangles from the first
+    // cut chunk on were dropped) draws its list, then the triangles from
+    // the cut on by an in-order scan of their pixel boxes; a list past the
+    // capacity without a flag (sort binning) is a scan of every triangle.
+    // (An empty tile returns at once unless clears are folded.)
+    const bool cut = f.tile_flag && f.tile_flag[tile];
+    const bool scan = cut || (uint64_t)off + n > (uint64_t)f.list_capacity;
+    uint32_t lcnt = scan ? 0u : n;
+    const uint32_t *lids = f.tile_lists + off;
+    if (cut) {
+        lcnt = f.tile_counts[tile];
+        lids = f.tile_lists + (lcnt ? f.tile_offsets[tile] : 0u);
+    }
+    unsigned long long frags =
+        kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, lids, lcnt, false, f.clear_mask, s_color, s_depth, s_claim, s_frag);
+    if (scan)
+        frags += kz_raster_rect(f, X0, Y0, KZ_TILE_SHIFT, nullptr, f.total_tris, true, 0u, s_color, s_depth, s_claim,
+                                s_frag);
     // tile profile: the cost model's label (fragments) and raster cycles
     if (f.tile_cycles && kz_lane() == 0) {
         const uint64_t dt = n ? clock64() - t0 : 0;
@@ -1466,7 +1474,7 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persis
         int32_t X0, Y0;
         uint32_t RS, n;
         const uint32_t *ids;
-        bool filter = false;
+        bool filter = false, cut = false;
         const bool item = q < n_items;
         if (item) {
             const uint32_t geo = f.queue[q];
@@ -1477,9 +1485,15 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persis
             uint32_t toff, tn;
             kz_tile_list(f, tile, toff, tn);
             if (level == 0) {
-                filter = (uint64_t)toff + tn > (uint64_t)f.list_capacity;
-                ids = filter ? nullptr : f.tile_lists + toff;
-                n = filter ? f.total_tris : tn;
+                cut = f.tile_flag && f.tile_flag[tile];
+                if (cut) {  // its list, then (pass 1) the triangles from the cut on
+                    n = f.tile_counts[tile];
+                    ids = n ? f.tile_lists + f.tile_offsets[tile] : nullptr;
+                } else {
+                    filter = (uint64_t)toff + tn > (uint64_t)f.list_capacity;
+                    ids = filter ? nullptr : f.tile_lists + toff;
+                    n = filter ? f.total_tris : tn;
+                }
             } else {
                 // a split tile's list always fits; its sub-tile lists may
                 // not, and then the sub-tile filters the tile's list
@@ -1504,7 +1518,17 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persis
             n = 0;
             e += gridDim.x;
         }
-        kz_raster_rect(f, X0, Y0, RS, ids, n, filter, s_color, s_depth, s_claim, s_frag);
+        // (a tile flagged by a cut: its list, then the triangles from the
+        // cut on by an in-order scan of their pixel boxes)
+        for (uint32_t pass = 0; pass < (cut ? 2u : 1u); ++pass) {
+            if (pass == 1) {
+                ids = nullptr;
+                n = f.total_tris;
+                filter = true;
+            }
+            kz_raster_rect(f, X0, Y0, RS, ids, n, filter, pass == 0 ? f.clear_mask : 0u, s_color, s_depth, s_claim,
+                           s_frag);
+        }
         if (item) {
             uint32_t next = 0;
             if (lane == 0) next = atomicAdd(&f.ctrl[KZ_CTRL_HEAD], 1u);

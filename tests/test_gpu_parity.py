@@ -149,6 +149,36 @@ def test_huge_triangles_list_overflow(small_list_ctx, oracle):
     assert st["list_overflow"] >= 1
 
 
+@pytest.fixture(params=["balanced", "plain"])
+def cut_list_ctx(driver, monkeypatch, request):
+    # tile lists of at most 20,000 (triangle, tile) pairs, in both forms
+    from kazan_amd.scenes.renderer import Context
+    monkeypatch.setenv("KAZAN_LIST_CAPACITY_LIMIT", "20000")
+    monkeypatch.setenv("KAZAN_PIPELINE", request.param)
+    monkeypatch.setenv("KAZAN_AUTOTUNE", "0")  # 16x16 tiles: the small triangles' pairs fit, the large ones' do not
+    return Context(driver)
+
+
+def test_cut_after_the_first_chunks(cut_list_ctx, oracle):
+    """8,192 small triangles (two pair chunks, one tile each) and then 64
+    large ones (about a thousand tiles each) in a third chunk: the lists hold
+    the small triangles' pairs, the third chunk is cut, and each tile it
+    touches draws its list and then scans the triangles from the cut on, in
+    primitive order (translucent, so order shows)."""
+    r = np.random.default_rng(16)
+    W = H = 512
+    c = r.uniform(4, W - 4, (8192, 1, 2))
+    small = np.round((c + r.uniform(-3, 3, (8192, 3, 2))) * 16) / 16
+    big = np.round(r.uniform(-40, W + 40, (64, 3, 2)) * 16) / 16
+    tris = np.concatenate([small, big])
+    n = len(tris)
+    s = scene_from_pixels(tris, colors=r.uniform(0, 1, (n, 3, 4)), z=r.uniform(0, 1, (n, 3)), width=W, height=H)
+    s.blend = (1, g.BLEND_SRC_ALPHA, g.BLEND_ONE_MINUS_SRC_ALPHA, 0, g.BLEND_ONE, g.BLEND_ONE_MINUS_SRC_ALPHA, 0)
+    s.depth_test = 0
+    st, ost, _ = check_parity(cut_list_ctx, oracle, s)
+    assert st["list_overflow"] >= 1
+
+
 @pytest.fixture
 def tiny_list_ctx(driver, monkeypatch):
     # balanced form; tile lists and sub-tile lists hold 10,000 entries
