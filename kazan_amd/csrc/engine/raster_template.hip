@@ -1106,6 +1106,43 @@ __device__ __forceinline__ void kz_resolve(const KzFrame &f, const KzLaneTri &L,
     }
 }
 
+// The pieces ks + lane of the clip polygon of triangle g0 (one that needs
+// clipping), set up, with their varyings' base and deltas; ok: the lane's
+// piece exists and set up. The whole wave calls it, and only for a batch
+// that starts with such a triangle, so it stays out of line: its
+// temporaries do not weigh on the raster's registers.
+struct KzClipBatch {
+    KzTri t;
+    float a0[KZ_NVA], da1[KZ_NVA], da2[KZ_NVA];
+    uint32_t ok, npieces;
+};
+
+__device__ __forceinline__ KzClipBatch kz_clip_batch(const KzFrame &f, uint32_t g0, uint32_t ks, uint32_t lane) {
+    float p[3][4], vary[3][KZ_NVA];
+    if (KZ_SETUP_RECORDS && f.tri_rec) {  // its record holds the vertex shader outputs
+        kz_load_clip_rec(f, g0, p, vary);
+    } else {
+        const uint32_t draw = kz_draw_of(f, g0);
+        uint32_t idx[3];
+        kz_triangle_indices(f, g0, draw, idx);
+        kz_fetch_vertices(f, draw, idx, p, vary);
+    }
+    float v[4 + KZ_NV];
+    const int np = kz_clip_polygon<4 + KZ_NV>(f, p, vary, 0, lane, v);
+    KzClipBatch o;
+    o.npieces = np >= 3 ? (uint32_t)(np - 2) : 0u;
+    kz_clip_piece<4 + KZ_NV>(v, ks + lane, p, vary);
+    const int st = kz_setup_rt(f, p, o.t, true);
+    o.ok = ks + lane < o.npieces && st == 1 ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < KZ_NVA; ++i) {
+        o.a0[i] = i < KZ_NV ? vary[0][i] : 0.0f;
+        o.da1[i] = i < KZ_NV ? vary[1][i] - vary[0][i] : 0.0f;
+        o.da2[i] = i < KZ_NV ? vary[2][i] - vary[0][i] : 0.0f;
+    }
+    return o;
+}
+
 // Rasterises n triangles into the square of side 1 << RS at (X0, Y0) (a
 // whole tile, or a sub-tile of the balanced mode), in primitive order. The
 // triangles are ids[0, n) (ids = null: triangles 0 .. n-1); with `filter`,
@@ -1195,44 +1232,38 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         } else {
             kz_fetch_vertices(f, draw, idx, p, vary);
         }
-        // A triangle that needs clipping is drawn as its own batch of
-        // pieces (lane j: piece ks + j), in its place: a batch whose first
-        // triangle needs it draws that triangle's pieces, any other batch
-        // ends before its first such triangle. One setup serves the lane's
-        // triangle and, in a second pass, its piece.
-        bool clip_batch = false;
-        uint32_t first_clip = 64u, npieces = 0;
-        for (int pass = 0;; ++pass) {
-            if (!(KZ_SETUP_RECORDS && f.tri_rec) || pass) {
-                const int st = kz_setup_rt(f, p, L.t, pass != 0);
-                if (pass) {
-                    ok = ks + lane < npieces && st == 1;
-                    break;
-                }
-                ok = valid && st == 1;
-                clip = valid && st == 2;
-            }
-            const uint64_t clipm = kz_ballot(clip);
-            clip_batch = (clipm & 1ull) != 0;
-            first_clip = clipm ? (uint32_t)__builtin_ctzll(clipm) : 64u;
-            if (!clip_batch) {
-                ok = ok && lane < first_clip;
-                break;
-            }
-            if (KZ_SETUP_RECORDS && f.tri_rec)  // lane 0's triangle: its record holds the shader outputs
-                kz_load_clip_rec(f, (uint32_t)kz_readlane_i((int32_t)g, 0), p, vary);
-            float v[4 + KZ_NV];
-            const int np = kz_clip_polygon<4 + KZ_NV>(f, p, vary, 0, lane, v);
-            npieces = np >= 3 ? (uint32_t)(np - 2) : 0u;
-            kz_clip_piece<4 + KZ_NV>(v, ks + lane, p, vary);
-        }
-        if (!(KZ_SETUP_RECORDS && f.tri_rec) || clip_batch) {
+        if (!(KZ_SETUP_RECORDS && f.tri_rec)) {
+            const int st = kz_setup_rt(f, p, L.t, false);
+            ok = valid && st == 1;
+            clip = valid && st == 2;
 #pragma unroll
             for (int i = 0; i < KZ_NVA; ++i) {
                 L.a0[i] = i < KZ_NV ? vary[0][i] : 0.0f;
                 L.da1[i] = i < KZ_NV ? vary[1][i] - vary[0][i] : 0.0f;
                 L.da2[i] = i < KZ_NV ? vary[2][i] - vary[0][i] : 0.0f;
             }
+        }
+        // A triangle that needs clipping is drawn as its own batch of
+        // pieces (lane j: piece ks + j), in its place: a batch whose first
+        // triangle needs it draws that triangle's pieces (out of line), any
+        // other batch ends before its first such triangle.
+        const uint64_t clipm = kz_ballot(clip);
+        const bool clip_batch = (clipm & 1ull) != 0;
+        const uint32_t first_clip = clipm ? (uint32_t)__builtin_ctzll(clipm) : 64u;
+        uint32_t npieces = 0;
+        if (clip_batch) {
+            const KzClipBatch cb = kz_clip_batch(f, (uint32_t)kz_readlane_i((int32_t)g, 0), ks, lane);
+            L.t = cb.t;
+#pragma unroll
+            for (int i = 0; i < KZ_NVA; ++i) {
+                L.a0[i] = cb.a0[i];
+                L.da1[i] = cb.da1[i];
+                L.da2[i] = cb.da2[i];
+            }
+            ok = cb.ok != 0;
+            npieces = cb.npieces;
+        } else {
+            ok = ok && lane < first_clip;
         }
         L.cx0 = max(L.t.px0, X0);
         L.cx1 = min(L.t.px1, X0 + RW - 1);
