@@ -1440,6 +1440,8 @@ __global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
 }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8_bits __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // f32 -> bf16 bits, round to nearest even (inputs are finite)
@@ -1453,10 +1455,13 @@ constexpr uint32_t kMlpThreads = 256;
 
 // One 64-unit layer for a wave's 16 rows (tiles or samples): in (bf16
 // [row][K], row stride in_stride) x W (bf16 [64][K], transposed) on MFMA
-// 16x16x16 bf16 with fp32 accumulation, + bias, ReLU, to bf16 out ([row][64];
-// may alias `in`: every read precedes every write). Operand layouts of
-// v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l % 16][4 (l / 16) + j] and
-// B[4 (l / 16) + j][l % 16]; it receives D[4 (l / 16) + i][l % 16].
+// with fp32 accumulation, + bias, ReLU, to bf16 out ([row][64]; may alias
+// `in`: every read precedes every write). K a multiple of 32 (the hidden
+// layers) runs on gfx950's v_mfma_f32_16x16x32_bf16: lane l supplies
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8 (one
+// 16-byte LDS read each); K = 16 (the input layer) on
+// v_mfma_f32_16x16x16bf16_1k: A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j]
+// [l % 16], j < 4. Both give D[4 (l / 16) + i][l % 16] to lane l.
 template <uint32_t K>
 __device__ __forceinline__ void mlp_layer(const short *in, uint32_t in_stride, short *out, const short *w,
                                           const float *bias, uint32_t lane) {
@@ -1465,15 +1470,25 @@ __device__ __forceinline__ void mlp_layer(const short *in, uint32_t in_stride, s
 #pragma unroll
     for (uint32_t nb = 0; nb < 4; ++nb) {
         d[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (K % 32 == 0) {
 #pragma unroll
-        for (uint32_t kb = 0; kb < K / 16; ++kb) {
-            bf16x4 a, b;
-#pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-                a[j] = in[row * in_stride + kb * 16 + quad * 4 + j];
-                b[j] = w[(nb * 16 + row) * K + kb * 16 + quad * 4 + j];
+            for (uint32_t kb = 0; kb < K / 32; ++kb) {
+                const bf16x8_bits a = *(const bf16x8_bits *)&in[row * in_stride + kb * 32 + quad * 8];
+                const bf16x8_bits b = *(const bf16x8_bits *)&w[(nb * 16 + row) * K + kb * 32 + quad * 8];
+                d[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                               __builtin_bit_cast(bf16x8, b), d[nb], 0, 0, 0);
             }
-            d[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d[nb], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (uint32_t kb = 0; kb < K / 16; ++kb) {
+                bf16x4 a, b;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    a[j] = in[row * in_stride + kb * 16 + quad * 4 + j];
+                    b[j] = w[(nb * 16 + row) * K + kb * 16 + quad * 4 + j];
+                }
+                d[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d[nb], 0, 0, 0);
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1492,9 +1507,11 @@ __device__ __forceinline__ void mlp_layer(const short *in, uint32_t in_stride, s
 // workgroup.
 __global__ __launch_bounds__(kMlpThreads) void kz_cost_mlp(const KzCostModel *m, const float *feat, uint32_t ntiles,
                                                            float *pred) {
-    __shared__ short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN], s_w3[KZ_HIDDEN * KZ_HIDDEN];
+    // (16-byte aligned: the hidden layers read 8 bf16 at a time)
+    __shared__ __attribute__((aligned(16))) short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN],
+        s_w3[KZ_HIDDEN * KZ_HIDDEN];
     __shared__ float s_b[3][KZ_HIDDEN], s_w4[KZ_HIDDEN];
-    __shared__ short s_act[kMlpThreads / 64][16 * KZ_HIDDEN];
+    __shared__ __attribute__((aligned(16))) short s_act[kMlpThreads / 64][16 * KZ_HIDDEN];
     for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_FEATURES; i += kMlpThreads) s_w1[i] = (short)m->w1[i];
     for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_HIDDEN; i += kMlpThreads) {
         s_w2[i] = (short)m->w2[i];
@@ -1627,11 +1644,13 @@ __device__ __forceinline__ void flush_wgrad(const f32x4 *g, float *s_g, uint32_t
 __global__ __launch_bounds__(kTrainThreads) void kz_cost_train(float *state, const float *x, const float *y,
                                                                KzTrainParams p, float *loss_out) {
     float *W = state, *M = state + KZ_PARAMS, *V = state + 2 * KZ_PARAMS;
-    __shared__ short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN], s_w3[KZ_HIDDEN * KZ_HIDDEN];
+    // (16-byte aligned: the hidden layers read 8 bf16 at a time)
+    __shared__ __attribute__((aligned(16))) short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN],
+        s_w3[KZ_HIDDEN * KZ_HIDDEN];
     __shared__ float s_b[3][KZ_HIDDEN], s_w4[KZ_HIDDEN], s_b4, s_loss;
     __shared__ float s_g[KZ_PARAMS];
     __shared__ short s_h0[kTrainThreads / 64][16 * KZ_FEATURES];
-    __shared__ short s_h[kTrainThreads / 64][3][16 * KZ_HIDDEN];
+    __shared__ __attribute__((aligned(16))) short s_h[kTrainThreads / 64][3][16 * KZ_HIDDEN];
     __shared__ short s_dz[kTrainThreads / 64][16 * KZ_HIDDEN];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t row = lane & 15, quad = lane >> 4;
