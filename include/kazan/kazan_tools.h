@@ -116,7 +116,10 @@ KAZAN_EXPORT int kazan_cost_model_eval_noalloc(const void *blob, size_t bytes, c
 KAZAN_EXPORT int kazan_ir_round_trip(const char *text, uint32_t what, char *out, size_t cap, size_t *out_len);
 
 /* what: 0 = generated shader source, 1 = full kernel source, 2 = compile to a
- * code object and report its size and whether the cache held it */
+ * code object and report its size, whether the cache held it, and per kernel
+ * (kz_bin_count, kz_raster, kz_raster_persistent) <kernel>_vgprs, _scratch
+ * (bytes per lane), _lds (bytes per workgroup) and _waves (resident waves per
+ * SIMD) */
 KAZAN_EXPORT int kazan_compile_pipeline_spirv(const KazanPipelineDesc *desc, uint32_t what, char *out, size_t cap,
                                               size_t *out_len);
 

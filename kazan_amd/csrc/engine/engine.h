@@ -61,6 +61,15 @@ uint64_t fnv1a64(const void *data, size_t n, uint64_t h = 1469598103934665603ull
 // hiprtc writes; anything else in a cache is damaged or foreign.
 bool is_amdgpu_code_object(const char *data, size_t n);
 std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source);
+// What one kernel of a gfx950 code object takes, from its kernel descriptor
+// (symbol <kernel>.kd): VGPRs including AGPRs (allocation granules of 8),
+// scratch bytes per lane, LDS bytes per workgroup, and the resident waves
+// per SIMD those allow a one-wave workgroup (512 VGPRs per SIMD lane, 160 KB
+// of LDS per CU). False if the code object has no such kernel.
+struct KernelResources {
+    uint32_t vgprs = 0, scratch = 0, lds = 0, waves = 0;
+};
+bool kernel_resources(const std::vector<char> &elf, const char *kernel, KernelResources &r);
 // A compute pipeline's kernel (generate_compute_source's output with the
 // shared device structs), compiled through the same code-object cache.
 std::string compute_full_source(const std::string &compute_source);

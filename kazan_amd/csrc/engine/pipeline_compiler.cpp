@@ -60,7 +60,25 @@ std::mutex g_compile_mutex;
 
 }  // namespace
 
-std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source) {
+namespace {
+
+// Occupancy hints of the two raster kernels (resident waves per SIMD the
+// compiler must allow for; 0: its own choice). KAZAN_RASTER_WAVES=n sets
+// both to n and turns the occupancy step of compile_pipeline_code off.
+// Default 3: binds only the heaviest shaders (an 8-light Phong raster would
+// otherwise take a few registers more than 3 waves allow).
+struct WaveHints {
+    int raster = 3, persist = 3;
+};
+
+bool waves_from_env(WaveHints &h) {
+    const char *waves = std::getenv("KAZAN_RASTER_WAVES");
+    if (!waves) return false;
+    h.raster = h.persist = std::max(0, std::min(8, std::atoi(waves)));
+    return true;
+}
+
+std::string full_source(const FixedState &fs, const std::string &shader_source, const WaveHints &h) {
     std::ostringstream os;
     os << "// generated pipeline kernel source\n";
     os << "#define KZ_TILE_SHIFT " << fs.tile_shift << "\n";
@@ -83,17 +101,70 @@ std::string pipeline_full_source(const FixedState &fs, const std::string &shader
     // compiled out otherwise, so that binning drops what only records need
     const char *rec = std::getenv("KAZAN_SETUP_RECORDS");
     os << "#define KZ_SETUP_RECORDS " << (rec && std::strcmp(rec, "0") != 0 ? 1 : 0) << "\n";
-    // KAZAN_RASTER_WAVES=n: ask the compiler for at least n resident raster
-    // waves per SIMD (registers beyond the budget spill); 0 = its own
-    // choice. Default 3: binds only the heaviest shaders (an 8-light Phong
-    // raster would otherwise take a few registers more than 3 waves allow)
-    const char *waves = std::getenv("KAZAN_RASTER_WAVES");
-    os << "#define KZ_RASTER_WAVES " << (waves ? std::max(0, std::min(8, std::atoi(waves))) : 3) << "\n";
+    os << "#define KZ_RASTER_WAVES " << h.raster << "\n";
+    os << "#define KZ_PERSIST_WAVES " << h.persist << "\n";
     os << kz_src_device_common << "\n";
     os << "__device__ void kz_sample(const KzTexture &t, float u, float v, float out[4]);\n";
     os << shader_source << "\n";
     os << kz_src_raster_template << "\n";
     return os.str();
+}
+
+WaveHints default_hints() {
+    WaveHints h;
+    waves_from_env(h);
+    return h;
+}
+
+template <typename T>
+T get_le(const std::vector<char> &b, size_t off) {
+    T v{};
+    std::memcpy(&v, b.data() + off, sizeof v);
+    return v;
+}
+
+}  // namespace
+
+std::string pipeline_full_source(const FixedState &fs, const std::string &shader_source) {
+    return full_source(fs, shader_source, default_hints());
+}
+
+bool kernel_resources(const std::vector<char> &elf, const char *kernel, KernelResources &r) {
+    if (!is_amdgpu_code_object(elf.data(), elf.size())) return false;
+    const size_t n = elf.size();
+    const uint64_t shoff = get_le<uint64_t>(elf, 0x28);
+    const uint16_t shentsize = get_le<uint16_t>(elf, 0x3A), shnum = get_le<uint16_t>(elf, 0x3C);
+    if (shentsize < 64 || shoff > n || (uint64_t)shnum * shentsize > n - shoff) return false;
+    auto sh = [&](uint32_t i, size_t field) { return shoff + (size_t)i * shentsize + field; };
+    const std::string want = std::string(kernel) + ".kd";
+    for (uint32_t i = 0; i < shnum; ++i) {
+        if (get_le<uint32_t>(elf, sh(i, 4)) != 2) continue;  // SHT_SYMTAB
+        const uint64_t so = get_le<uint64_t>(elf, sh(i, 24)), ss = get_le<uint64_t>(elf, sh(i, 32));
+        const uint32_t link = get_le<uint32_t>(elf, sh(i, 40));
+        if (link >= shnum || so > n || ss > n - so) return false;
+        const uint64_t stro = get_le<uint64_t>(elf, sh(link, 24)), strs = get_le<uint64_t>(elf, sh(link, 32));
+        if (stro > n || strs > n - stro) return false;
+        for (uint64_t k = 0; k + 24 <= ss; k += 24) {
+            const uint32_t name = get_le<uint32_t>(elf, so + k);
+            if (name >= strs || strs - name < want.size() + 1 ||
+                std::memcmp(elf.data() + stro + name, want.c_str(), want.size() + 1) != 0)
+                continue;
+            const uint16_t shndx = get_le<uint16_t>(elf, so + k + 6);
+            const uint64_t value = get_le<uint64_t>(elf, so + k + 8);
+            if (shndx == 0 || shndx >= shnum) return false;
+            const uint64_t addr = get_le<uint64_t>(elf, sh(shndx, 16)), off = get_le<uint64_t>(elf, sh(shndx, 24));
+            if (value < addr || off + (value - addr) > n || n - (off + (value - addr)) < 64) return false;
+            const size_t kd = (size_t)(off + (value - addr));
+            r.lds = get_le<uint32_t>(elf, kd + 0);
+            r.scratch = get_le<uint32_t>(elf, kd + 4);
+            r.vgprs = ((get_le<uint32_t>(elf, kd + 48) & 63u) + 1u) * 8u;
+            const uint32_t by_vgpr = 512u / r.vgprs;
+            const uint32_t by_lds = r.lds ? (160u * 1024u / r.lds) / 4u : 8u;
+            r.waves = std::max(1u, std::min(std::min(by_vgpr, by_lds), 8u));
+            return true;
+        }
+    }
+    return false;
 }
 
 uint64_t fnv1a64(const void *data, size_t n, uint64_t h) {
@@ -131,22 +202,27 @@ std::string code_name(const std::string &full_source, const char *opt) {
     return name;
 }
 
-// hiprtc through the on-disk code-object cache
-bool compile_cached(const std::string &src, const char *opt, std::vector<char> &code, std::string &log,
-                    bool *cache_hit, bool read_cache) {
-    const std::string dir = cache_dir();
-    const std::string path = dir + "/" + code_name(src, opt);
-    if (cache_hit) *cache_hit = false;
-    if (read_cache) {
-        std::ifstream in(path, std::ios::binary);
-        if (in) {
-            code.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
-            if (is_amdgpu_code_object(code.data(), code.size())) {
-                if (cache_hit) *cache_hit = true;
-                return true;
-            }
-        }
+// the cached code object at `path`, if it is one
+bool read_cached(const std::string &path, std::vector<char> &code) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    code.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+    return is_amdgpu_code_object(code.data(), code.size());
+}
+
+// publish atomically: write a temporary file, then rename
+void publish(const std::string &dir, const std::string &path, const std::vector<char> &code) {
+    mkdir(dir.c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream out(tmp, std::ios::binary);
+        if (out) out.write(code.data(), (std::streamsize)code.size());
     }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+// hiprtc, no cache
+bool rtc_compile(const std::string &src, const char *opt, std::vector<char> &code, std::string &log) {
     std::lock_guard<std::mutex> lock(g_compile_mutex);
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "kazan_pipeline.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
@@ -172,15 +248,33 @@ bool compile_cached(const std::string &src, const char *opt, std::vector<char> &
     code.resize(cs);
     hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
-    // publish atomically: write a temporary file, then rename
-    mkdir(dir.c_str(), 0755);
-    const std::string tmp = path + ".tmp" + std::to_string(getpid());
-    {
-        std::ofstream out(tmp, std::ios::binary);
-        if (out) out.write(code.data(), (std::streamsize)code.size());
-    }
-    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
     return true;
+}
+
+// hiprtc through the on-disk code-object cache
+bool compile_cached(const std::string &src, const char *opt, std::vector<char> &code, std::string &log,
+                    bool *cache_hit, bool read_cache) {
+    const std::string dir = cache_dir();
+    const std::string path = dir + "/" + code_name(src, opt);
+    if (cache_hit) *cache_hit = false;
+    if (read_cache && read_cached(path, code)) {
+        if (cache_hit) *cache_hit = true;
+        return true;
+    }
+    if (!rtc_compile(src, opt, code, log)) return false;
+    publish(dir, path, code);
+    return true;
+}
+
+// The occupancy step for one raster kernel: the hint that asks for one more
+// resident wave per SIMD when the kernel's registers are within one
+// allocation granule (8 VGPRs) of that level and LDS allows it, else 0.
+int next_wave_hint(const KernelResources &r) {
+    const uint32_t next = 512u / r.vgprs + 1u;
+    const uint32_t by_lds = r.lds ? (160u * 1024u / r.lds) / 4u : 8u;
+    if (next > std::min(by_lds, 8u) || r.scratch) return 0;
+    const uint32_t budget = (512u / next) & ~7u;
+    return r.vgprs <= budget + 8u ? (int)next : 0;
 }
 
 }  // namespace
@@ -189,9 +283,50 @@ std::string pipeline_code_name(const FixedState &fs, const std::string &shader_s
     return code_name(pipeline_full_source(fs, shader_source), opt_level(fs));
 }
 
+// The raster kernels are compiled at the default hints first. A kernel whose
+// registers are one granule above the next occupancy level is compiled again
+// with that level as its hint, and the result kept if the kernel then uses
+// no scratch and reaches the level (the raster is latency bound: one more
+// resident wave hides more of it). The code object kept is cached under the
+// name of the default source, so the choice is made once per pipeline.
 bool compile_pipeline_code(const FixedState &fs, const std::string &shader_source, std::vector<char> &code,
                            std::string &log, bool *cache_hit, bool read_cache) {
-    return compile_cached(pipeline_full_source(fs, shader_source), opt_level(fs), code, log, cache_hit, read_cache);
+    WaveHints h;
+    const bool forced = waves_from_env(h);
+    const std::string src = full_source(fs, shader_source, h);
+    const char *opt = opt_level(fs);
+    if (forced || !fs.optimize) return compile_cached(src, opt, code, log, cache_hit, read_cache);
+    const std::string dir = cache_dir();
+    const std::string path = dir + "/" + code_name(src, opt);
+    if (cache_hit) *cache_hit = false;
+    if (read_cache && read_cached(path, code)) {
+        if (cache_hit) *cache_hit = true;
+        return true;
+    }
+    if (!rtc_compile(src, opt, code, log)) return false;
+    KernelResources r0, p0;
+    WaveHints up = h;
+    if (kernel_resources(code, "kz_raster", r0)) up.raster = std::max(h.raster, next_wave_hint(r0));
+    if (kernel_resources(code, "kz_raster_persistent", p0)) up.persist = std::max(h.persist, next_wave_hint(p0));
+    for (int attempt = 0; attempt < 2 && (up.raster != h.raster || up.persist != h.persist); ++attempt) {
+        std::vector<char> alt;
+        std::string alt_log;
+        if (!rtc_compile(full_source(fs, shader_source, up), opt, alt, alt_log)) break;
+        KernelResources r1, p1;
+        const bool r_ok = up.raster == h.raster ||
+                          (kernel_resources(alt, "kz_raster", r1) && !r1.scratch && (int)r1.waves >= up.raster);
+        const bool p_ok = up.persist == h.persist || (kernel_resources(alt, "kz_raster_persistent", p1) &&
+                                                      !p1.scratch && (int)p1.waves >= up.persist);
+        if (r_ok && p_ok) {
+            code.swap(alt);
+            break;
+        }
+        // keep the raise that held, if any, and compile once more with it alone
+        if (!r_ok) up.raster = h.raster;
+        if (!p_ok) up.persist = h.persist;
+    }
+    publish(dir, path, code);
+    return true;
 }
 
 std::string compute_full_source(const std::string &compute_source) {

@@ -471,21 +471,29 @@ struct KzWalk {
 // the walk of the box of width w whose first pixel is (px, py)
 __device__ __forceinline__ void kz_walk_init(KzWalk &w, const KzTri &t, const KzCov &c, int32_t px, int32_t py,
                                              int32_t width) {
-    const int32_t sg = c.flip ? -1 : 1;
-    const int32_t A[3] = {-t.d1y * sg, (t.d1y - t.d2y) * sg, t.d2y * sg};
-    const int32_t B[3] = {t.d1x * sg, (t.d2x - t.d1x) * sg, -t.d2x * sg};
-    const int32_t C[3] = {0, (int32_t)t.S * sg, 0};
+    // (wrapping 32-bit arithmetic: a lane may run this on a triangle that
+    // is not small, and then never uses the values)
+    const uint32_t sg = c.flip ? ~0u : 1u;
+    const uint32_t d1x = (uint32_t)t.d1x, d1y = (uint32_t)t.d1y, d2x = (uint32_t)t.d2x, d2y = (uint32_t)t.d2y;
+    const int32_t A[3] = {(int32_t)((0u - d1y) * sg), (int32_t)((d1y - d2y) * sg), (int32_t)(d2y * sg)};
+    const int32_t B[3] = {(int32_t)(d1x * sg), (int32_t)((d2x - d1x) * sg), (int32_t)((0u - d2x) * sg)};
+    const uint32_t C[3] = {0u, (uint32_t)t.S * sg, 0u};
     const int32_t th[3] = {c.th01, c.th12, c.th20};
     const int32_t rx = px * 16 + kz_sample_x[0] - t.x0, ry = py * 16 + kz_sample_y[0] - t.y0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        w.e[k] = __mul24(A[k], rx) + __mul24(B[k], ry) + C[k] - th[k];
-        w.ax[k] = A[k] * 16;
-        w.wy[k] = (B[k] - __mul24(width - 1, A[k])) * 16;
+        w.e[k] = (int32_t)((uint32_t)__mul24(A[k], rx) + (uint32_t)__mul24(B[k], ry) + C[k] - (uint32_t)th[k]);
+        w.ax[k] = (int32_t)((uint32_t)A[k] << 4);
+        w.wy[k] = (int32_t)(((uint32_t)B[k] - (uint32_t)__mul24(width - 1, A[k])) << 4);
 #pragma unroll
         for (int s = 0; s < KZ_SAMPLES; ++s)
             w.o[k][s] = __mul24(A[k], kz_sample_x[s] - kz_sample_x[0]) + __mul24(B[k], kz_sample_y[s] - kz_sample_y[0]);
     }
+}
+
+// at one sample per pixel: bit 31 set iff the current pixel is outside
+__device__ __forceinline__ uint32_t kz_walk_or(const KzWalk &w) {
+    return (uint32_t)(w.e[0] | w.e[1] | w.e[2]);
 }
 
 __device__ __forceinline__ uint32_t kz_walk_mask(const KzWalk &w) {
@@ -501,7 +509,7 @@ __device__ __forceinline__ uint32_t kz_walk_mask(const KzWalk &w) {
 // to the next pixel; `wrap`: the current one ends its row
 __device__ __forceinline__ void kz_walk_step(KzWalk &w, bool wrap) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) w.e[k] += wrap ? w.wy[k] : w.ax[k];
+    for (int k = 0; k < 3; ++k) w.e[k] = (int32_t)((uint32_t)w.e[k] + (uint32_t)(wrap ? w.wy[k] : w.ax[k]));
 }
 
 // The draw of batch triangle g: the per-triangle table binning wrote, the
@@ -1293,8 +1301,30 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
         uint32_t cnt = 0;
         KzCandMask cmask = 0;
         {
-            const int32_t maxc = kz_wave_max(small ? cand : 0, lane);
+            // (the wave's longest walk, a scalar loop bound)
+            const int32_t maxc = __builtin_amdgcn_readfirstlane(kz_wave_max(small ? cand : 0, lane));
             KzWalk wk;
+#if KZ_SAMPLES == 1
+            // Every lane steps maxc times (a lane that walks nothing, or
+            // less, steps on past its box, in wrapping 32-bit arithmetic;
+            // those steps are dropped below), shifting each pixel's sign bit
+            // (set: outside) in at bit 31, so that pixel j of maxc ends at bit
+            // 32 - maxc + j: two instructions a pixel, no mask switching.
+            kz_walk_init(wk, L.t, L.c, L.cx0, L.cy0, cw);
+            int32_t dx = 0;
+            uint32_t out = 0;
+            for (int32_t j = 0; j < maxc; ++j) {
+                const uint32_t v = kz_walk_or(wk);
+                const bool wrap = dx == cw - 1;
+                kz_walk_step(wk, wrap);
+                dx = wrap ? 0 : dx + 1;
+                out = (out >> 1) | (v & 0x80000000u);
+            }
+            if (small && maxc > 0) {
+                cmask = ~(out >> (32 - maxc)) & ((1u << cand) - 1u);
+                cnt = (uint32_t)__popc(cmask);
+            }
+#else
             if (small) kz_walk_init(wk, L.t, L.c, L.cx0, L.cy0, cw);
             int32_t dx = 0;
             for (int32_t j = 0; j < maxc; ++j) {
@@ -1309,6 +1339,7 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
                     }
                 }
             }
+#endif
         }
         uint64_t bigm = kz_ballot(L.live && !small);
         while (bigm) {
@@ -1459,12 +1490,20 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
     return nfrag;
 }
 
-// Static mode: one workgroup per screen tile, in tile order.
+// Occupancy hints (resident waves per SIMD the compiler must allow for;
+// 0: its own choice), one per raster kernel
 #if KZ_RASTER_WAVES
 #define KZ_RASTER_ATTR __attribute__((amdgpu_waves_per_eu(KZ_RASTER_WAVES)))
 #else
 #define KZ_RASTER_ATTR
 #endif
+#if KZ_PERSIST_WAVES
+#define KZ_PERSIST_ATTR __attribute__((amdgpu_waves_per_eu(KZ_PERSIST_WAVES)))
+#else
+#define KZ_PERSIST_ATTR
+#endif
+
+// Static mode: one workgroup per screen tile, in tile order.
 
 extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
@@ -1508,7 +1547,7 @@ extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster(KzFram
 // round, then the next untaken item from a counter. With folded clears,
 // the tiles without triangles (they have no items) follow, strided over
 // the workgroups. One call site of kz_raster_rect serves every case.
-extern "C" __global__ __launch_bounds__(64) KZ_RASTER_ATTR void kz_raster_persistent(KzFrame f) {
+extern "C" __global__ __launch_bounds__(64) KZ_PERSIST_ATTR void kz_raster_persistent(KzFrame f) {
     __shared__ uint32_t s_color[KZ_TP * KZ_SAMPLES];
     __shared__ float s_depth[KZ_TP * KZ_SAMPLES];
     __shared__ uint32_t s_claim[KZ_TP];

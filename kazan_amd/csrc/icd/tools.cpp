@@ -125,9 +125,16 @@ extern "C" int kazan_compile_pipeline_spirv(const KazanPipelineDesc *d, uint32_t
     std::string log;
     bool hit = false;
     if (!kz::compile_pipeline_code(fx, source, code, log, &hit)) return put(log, out, cap, out_len, -1);
-    return put("code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
-                   " varyings=" + std::to_string(fx.num_varyings),
-               out, cap, out_len, 0);
+    std::string rep = "code_object_bytes=" + std::to_string(code.size()) + " cache_hit=" + (hit ? "1" : "0") +
+                      " varyings=" + std::to_string(fx.num_varyings);
+    for (const char *k : {"kz_bin_count", "kz_raster", "kz_raster_persistent"}) {
+        kz::KernelResources r;
+        if (!kz::kernel_resources(code, k, r)) continue;
+        rep += std::string(" ") + k + "_vgprs=" + std::to_string(r.vgprs) + " " + k + "_scratch=" +
+               std::to_string(r.scratch) + " " + k + "_lds=" + std::to_string(r.lds) + " " + k +
+               "_waves=" + std::to_string(r.waves);
+    }
+    return put(rep, out, cap, out_len, 0);
 }
 
 extern "C" int kazan_compile_compute_spirv(const uint32_t *words, size_t nwords, const char *entry, uint32_t what,

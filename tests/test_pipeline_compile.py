@@ -55,3 +55,37 @@ def test_second_compile_hits_the_cache(driver):
     compile_offline(driver, s, 2)
     rc, txt = compile_offline(driver, s, 2)
     assert rc == 0 and "cache_hit=1" in txt
+
+
+def _resources(driver, scene, tile_shift):
+    rc, txt = compile_offline(driver, scene, 2, tile_shift=tile_shift)
+    assert rc == 0, txt
+    return {k: int(v) for k, v in (kv.split("=") for kv in txt.split())}
+
+
+@pytest.mark.parametrize("name,tile_shift,samples", [("grid_textured_blend", 4, 1), ("grid_textured_blend", 5, 1),
+                                                      ("phong_msaa4", 4, 4)])
+def test_raster_kernel_resources(driver, name, tile_shift, samples):
+    """The kernel-descriptor reader against what the raster's LDS must be: per
+    pixel colour and depth per sample and one claim word, plus the fragment
+    queue (4 entries per pixel), 4 bytes each; registers in granules of 8 and
+    the occupancy they and the LDS allow (512 VGPRs per SIMD lane, 160 KB of
+    LDS per CU, one-wave workgroups)."""
+    r = _resources(driver, _scenes()[name], tile_shift)
+    tp = 1 << (2 * tile_shift)
+    for k in ("kz_raster", "kz_raster_persistent"):
+        assert r[k + "_lds"] == tp * samples * 8 + tp * 4 + tp * 4 * 4
+        v = r[k + "_vgprs"]
+        assert v % 8 == 0 and 8 <= v <= 512
+        assert r[k + "_waves"] == max(1, min(512 // v, 160 * 1024 // r[k + "_lds"] // 4, 8))
+    assert r["kz_raster_scratch"] == 0
+
+
+def test_occupancy_step_raises_only_without_spills(driver):
+    """A raster kernel compiled again for one more resident wave keeps it only
+    without scratch: the grid pipeline's one-workgroup-per-tile raster (one
+    granule above 5 waves at the default hint) runs at 5 waves per SIMD with
+    no scratch."""
+    r = _resources(driver, _scenes()["grid_textured_blend"], 4)
+    assert r["kz_raster_waves"] >= 5 and r["kz_raster_scratch"] == 0
+    assert r["kz_raster_persistent_scratch"] == 0
