@@ -1613,8 +1613,12 @@ extern "C" __global__ __launch_bounds__(64) KZ_PERSIST_ATTR void kz_raster_persi
                 n = f.total_tris;
                 filter = true;
             }
-            kz_raster_rect(f, X0, Y0, RS, ids, n, filter, pass == 0 ? f.clear_mask : 0u, s_color, s_depth, s_claim,
-                           s_frag);
+            // (a call per filter mode: each inlined copy is specialised)
+            const uint32_t clear = pass == 0 ? f.clear_mask : 0u;
+            if (filter)
+                kz_raster_rect(f, X0, Y0, RS, ids, n, true, clear, s_color, s_depth, s_claim, s_frag);
+            else
+                kz_raster_rect(f, X0, Y0, RS, ids, n, false, clear, s_color, s_depth, s_claim, s_frag);
         }
         if (item) {
             uint32_t next = 0;
