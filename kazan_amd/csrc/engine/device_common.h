@@ -115,9 +115,12 @@ struct KzFrame {
     uint32_t *tile_level;       // [tiles] split level
     uint32_t *split_slot;       // [tiles] split slot of a split tile
     uint32_t *split_info;       // [2 * split slots] (first chunk, tile) of a slot
-    uint32_t *chunk_tile;       // [max_chunks] tile of each KZ_SPLIT_CHUNK-entry chunk
+    uint32_t *chunk_desc;       // [max_chunks * 4] per KZ_SPLIT_CHUNK-entry chunk of a split tile: the list
+                                // position of its first entry, the tile, slot << 2 | level, its entries
     uint32_t *chunk_counts;     // [chunks * 16] entries per sub-tile, then bases within the slot's lists
     uint16_t *split_mask;       // [max_chunks * KZ_SPLIT_CHUNK] sub-tiles each chunk entry overlaps
+    const uint16_t *list_mask;  // [list capacity] the 4 x 4 sub-tiles (bit sy * 4 + sx) each list entry's
+                                // pixel box overlaps in its tile (radix binning wrote them), or null
     uint32_t *split_counts;     // [split slots * 16]
     uint32_t *split_offsets;    // [split slots * 16]
     uint32_t *split_lists;      // [split_capacity]

@@ -51,9 +51,10 @@ Engine::~Engine() {
                         pair_vals_[0], pair_vals_[1], radix_table_, radix_totals_})
         hipFree(p);
     for (uint32_t *p : {tile_level_, split_slot_, split_info_, queue_, ctrl_, split_counts_, split_offsets_,
-                        chunk_tile_, chunk_counts_, split_lists_, plan_table_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
+                        chunk_desc_, chunk_counts_, split_lists_, plan_table_, tile_acc_, tile_cycles_, tile_frags_, tri_z_})
         hipFree(p);
     hipFree(split_mask_);
+    hipFree(list_mask_);
     hipFree(tile_feat_);
     hipFree(tile_pred_);
     hipFree(cost_model_d_);
@@ -216,7 +217,7 @@ hipError_t Engine::ensure_split_scratch(uint32_t tiles) {
     if (want_chunks > max_chunks_) {
         const uint64_t c = std::min<uint64_t>(std::max<uint64_t>(want_chunks, (uint64_t)max_chunks_ * 3 / 2),
                                               0x0FFFFFFFull);
-        KZ_TRY(regrow(chunk_tile_, c));
+        KZ_TRY(regrow(chunk_desc_, 4 * c));
         KZ_TRY(regrow(chunk_counts_, 16 * c));
         KZ_TRY(regrow(split_mask_, (uint64_t)KZ_SPLIT_CHUNK * c));
         max_chunks_ = (uint32_t)c;
@@ -272,6 +273,7 @@ hipError_t Engine::alloc_lists(uint64_t capacity) {
             KZ_TRY(regrow(pair_vals_[i], capacity));
         }
         KZ_TRY(regrow(radix_table_, (uint64_t)kz_radix_digits_max() * kz_radix_blocks(capacity)));
+        if (balanced_) KZ_TRY(regrow(list_mask_, capacity));
     } else {
         KZ_TRY(regrow(sort_tmp_, capacity));
     }
@@ -356,7 +358,7 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         f.tile_level = tile_level_;
         f.split_slot = split_slot_;
         f.split_info = split_info_;
-        f.chunk_tile = chunk_tile_;
+        f.chunk_desc = chunk_desc_;
         f.chunk_counts = chunk_counts_;
         f.split_mask = split_mask_;
         f.split_counts = split_counts_;
@@ -365,6 +367,10 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         f.max_chunks = max_chunks_;
         f.raster_slots = std::max(1u, kern->persistent_blocks / std::max(grid_div, 1u));
         f.split_max_level = std::min(cfg.max_level, 2u);
+        // the sub-tile masks travel in the pairs' keys (above 16 tile bits)
+        // through the radix sort, so the split reads them beside the lists
+        // instead of gathering pixel boxes
+        if (radix_ && list_mask_ && f.split_max_level > 0 && shift >= 2 && ntiles < 65536u) f.list_mask = list_mask_;
     }
     // the cost model's inputs (and, in the static form, its labels) when a
     // model will run or tiles are being profiled
@@ -467,7 +473,8 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         const uint32_t grid = std::min(f.raster_slots, 16 * ntiles);
         KZ_TRY(hipModuleLaunchKernel(kern->raster_persistent, grid, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
         KZ_TRY(mark(KS_RASTER, true));
-        launches += 7;
+        // the plan (2, or 3 with the cost pass), the split (4), the raster
+        launches += (f.tile_pred || !f.bin_ctrl) ? 8 : 7;
     }
     if (acc) {
         last_profile_tiles_ = ntiles;

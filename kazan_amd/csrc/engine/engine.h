@@ -311,7 +311,7 @@ class Engine {
     // balanced-mode scratch: up to 16 work items per tile; chunk tables
     // sized from the list capacity; sub-tile lists grown on demand
     uint32_t *tile_level_ = nullptr, *split_slot_ = nullptr, *split_info_ = nullptr, *queue_ = nullptr,
-             *ctrl_ = nullptr, *split_counts_ = nullptr, *split_offsets_ = nullptr, *chunk_tile_ = nullptr,
+             *ctrl_ = nullptr, *split_counts_ = nullptr, *split_offsets_ = nullptr, *chunk_desc_ = nullptr,
              *chunk_counts_ = nullptr, *split_lists_ = nullptr, *plan_table_ = nullptr;
     uint16_t *split_mask_ = nullptr;  // [max_chunks * KZ_SPLIT_CHUNK]
     uint32_t cap_split_tiles_ = 0, max_chunks_ = 0, split_capacity_ = 0;
@@ -344,6 +344,8 @@ class Engine {
     bool radix_ = true;
     uint32_t *block_pairs_ = nullptr, *chunk_offsets_ = nullptr, *bin_ctrl_ = nullptr, *tile_flag_ = nullptr;
     uint32_t *pair_keys_[2] = {nullptr, nullptr}, *pair_vals_[2] = {nullptr, nullptr};
+    // (balanced form) the list entries' 4 x 4 sub-tile masks
+    uint16_t *list_mask_ = nullptr;
     uint32_t *radix_table_ = nullptr, *radix_totals_ = nullptr;
     // the buffers sized by the list capacity
     hipError_t alloc_lists(uint64_t capacity);

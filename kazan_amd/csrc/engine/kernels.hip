@@ -598,6 +598,21 @@ __device__ __forceinline__ uint32_t box_tiles(uint64_t r, uint32_t sh) {
     return ((x1 >> sh) - (x0 >> sh) + 1) * ((y1 >> sh) - (y0 >> sh) + 1);
 }
 
+// The 4 x 4 sub-tiles (bit sy * 4 + sx; sub-tiles of 2^(sh - 2) px, sh >= 2)
+// of tile (tx, ty) that the pixel box (x0, y0) - (x1, y1) overlaps, the box
+// overlapping the tile: what a split into 16 sub-tiles sends the pair to
+// (a split into 4 takes the OR of each 2 x 2 group).
+__device__ __forceinline__ uint32_t box_sub_mask16(uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t tx,
+                                                   uint32_t ty, uint32_t sh) {
+    const uint32_t X0 = tx << sh, Y0 = ty << sh, last = (1u << sh) - 1u, ssh = sh - 2u;
+    const uint32_t sx0 = (max(x0, X0) - X0) >> ssh, sx1 = (min(x1, X0 + last) - X0) >> ssh;
+    const uint32_t sy0 = (max(y0, Y0) - Y0) >> ssh, sy1 = (min(y1, Y0 + last) - Y0) >> ssh;
+    const uint32_t row = (2u << sx1) - (1u << sx0);
+    uint32_t m = 0;
+    for (uint32_t sy = sy0; sy <= sy1; ++sy) m |= row << (4u * sy);
+    return m;
+}
+
 // Pair offsets of the KZ_PAIR_CHUNK-triangle chunks (one workgroup; chunk c
 // sums blocks [c * kBlocksPerChunk, ...) of kz_bin_count). When the pairs
 // exceed `capacity`, the chunks from the first one that does not fit on are
@@ -663,11 +678,13 @@ __global__ __launch_bounds__(kScanThreads) void kz_pair_scan(const uint32_t *blo
 // ...)) from its offset on: the chunk's q-th pair is the k-th tile of the
 // triangle whose prefix range holds q. A cut chunk instead flags every tile
 // its triangles' boxes touch; kz_tile_list reads a flagged tile as one for
-// the in-order scan of all pixel boxes.
+// the in-order scan of all pixel boxes. With key_masks (fewer than 65536
+// tiles) a key carries the pair's box_sub_mask16 in its upper 16 bits, for
+// the split of the tile's list; the sort's digits read only the tile bits.
 __global__ __launch_bounds__(kPairThreads) void kz_pair_write(const uint64_t *rect, uint32_t ntris, uint32_t sh,
                                                               uint32_t tiles_x, const uint32_t *chunk_offsets,
                                                               const uint32_t *ctrl, uint32_t *keys, uint32_t *vals,
-                                                              uint32_t *tile_flag) {
+                                                              uint32_t key_masks, uint32_t *tile_flag) {
     __shared__ uint64_t s_rect[KZ_PAIR_CHUNK];
     __shared__ uint32_t s_pfx[KZ_PAIR_CHUNK];
     __shared__ uint32_t s_wave[kPairThreads / 64];
@@ -732,7 +749,8 @@ __global__ __launch_bounds__(kPairThreads) void kz_pair_write(const uint64_t *re
         unpack_box(s_rect[lo], x0, y0, x1, y1);
         const uint32_t tx0 = x0 >> sh, tw = (x1 >> sh) - tx0 + 1;
         const uint32_t ky = k / tw, kx = k - ky * tw;
-        keys[out + q] = ((y0 >> sh) + ky) * tiles_x + tx0 + kx;
+        const uint32_t tile = ((y0 >> sh) + ky) * tiles_x + tx0 + kx;
+        keys[out + q] = key_masks ? tile | (box_sub_mask16(x0, y0, x1, y1, tx0 + kx, (y0 >> sh) + ky, sh) << 16) : tile;
         vals[out + q] = t0 + lo;
     }
 }
@@ -854,10 +872,13 @@ struct RadixAcc {
 // writes only the triangle ids, into the tile lists, adds each tile's run
 // lengths to tile_counts and lowers tile_offsets (all ones before) to the
 // runs' first positions, so that the lists need no scan of the counts
-// (and, with acc, adds the pair statistics).
+// (and, with acc, adds the pair statistics). With list_mask (keys that
+// carry a sub-tile mask above 16 tile bits) the last pass also writes each
+// entry's mask beside the list.
 __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t *keys_in, const uint32_t *vals_in,
                                                                   uint32_t *keys_out, uint32_t *vals_out,
-                                                                  uint32_t *lists, uint32_t *tile_counts,
+                                                                  uint32_t *lists, uint16_t *list_mask,
+                                                                  uint32_t *tile_counts,
                                                                   uint32_t *tile_offsets,
                                                                   const uint32_t *ctrl, const uint32_t *table,
                                                                   uint32_t tstride, const uint32_t *totals,
@@ -931,21 +952,23 @@ __global__ __launch_bounds__(kRadixThreads) void kz_radix_scatter(const uint32_t
     for (uint32_t j0 = 0; j0 < nitems; j0 += kRadixThreads) {
         const uint32_t j = j0 + tid;
         const bool ok = j < nitems;
-        const uint32_t k = ok ? s_key[j] : kNone, v = ok ? s_val[j] : 0u;
+        const uint32_t kk = ok ? s_key[j] : kNone, v = ok ? s_val[j] : 0u;
         uint32_t g = 0;
         if (ok) {
-            const uint32_t d = (k >> shift) & dmask;
+            const uint32_t d = (kk >> shift) & dmask;
             g = s_gbase[d] + (j - s_lstart[d]);  // < npairs by construction
             if (g < npairs) {
                 if (lists) {
                     lists[g] = v;
+                    if (list_mask) list_mask[g] = (uint16_t)(kk >> 16);
                 } else {
-                    keys_out[g] = k;
+                    keys_out[g] = kk;
                     vals_out[g] = v;
                 }
             }
         }
         if (!lists) continue;
+        const uint32_t k = ok && list_mask ? kk & 0xFFFFu : kk;  // the tile
         // the last pass's block is sorted by tile: one update per run of a tile
         const Run run = run_of(k, lane);
         if (run.leader) {
@@ -1050,7 +1073,9 @@ __device__ __forceinline__ uint32_t bucket_reserve(uint32_t *s, uint32_t bucket,
 
 // The plan runs on many workgroups, in three kernels over the same tile
 // partition (workgroup w: tiles [w * kPlanTiles, ...), kPlanPer per thread):
-//   kz_plan_cost   each workgroup's part of the frame's total cost;
+//   kz_plan_cost   each workgroup's part of the frame's total cost (only
+//                  with a model or sort binning: otherwise the total is
+//                  the binned pairs);
 //   kz_plan_count  the fair share (every workgroup sums the parts in the
 //                  same order, so all agree on it), then per workgroup the
 //                  items of each LPT bucket, its split tiles and their chunks;
@@ -1064,22 +1089,23 @@ constexpr uint32_t kPlanTiles = kPlanWg * kPlanPer;
 constexpr uint32_t kPlanCols = 34;  // 32 buckets, split tiles, chunks
 constexpr uint32_t kPlanGroups = kPlanWg / kPlanCols;  // threads per column when summing the table
 
-__device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, uint32_t *t, uint32_t *n, float *pred) {
+__device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, uint32_t *t, uint32_t *n, float *pred,
+                                           uint32_t *off) {
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j) {
         t[j] = blockIdx.x * kPlanTiles + j * kPlanWg + threadIdx.x;
-        uint32_t off = 0;
+        off[j] = 0;
         n[j] = 0;
-        if (t[j] < ntiles) kz_tile_list(f, t[j], off, n[j]);
+        if (t[j] < ntiles) kz_tile_list(f, t[j], off[j], n[j]);
         pred[j] = t[j] < ntiles && f.tile_pred ? f.tile_pred[t[j]] : 0.0f;
     }
 }
 
 __global__ __launch_bounds__(kPlanWg) void kz_plan_cost(KzFrame f, uint32_t ntiles, uint32_t *table) {
     __shared__ float s_part[kPlanWg / 64];
-    uint32_t t[kPlanPer], n[kPlanPer];
+    uint32_t t[kPlanPer], n[kPlanPer], off[kPlanPer];
     float pred[kPlanPer];
-    plan_loads(f, ntiles, t, n, pred);
+    plan_loads(f, ntiles, t, n, pred, off);
     float part = 0.0f;
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j)
@@ -1094,18 +1120,25 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_cost(KzFrame f, uint32_t ntil
     }
 }
 
+// Without a model a tile's cost is its list entries, and the frame's total
+// is the pairs radix binning wrote: kz_plan_cost is not needed.
+__device__ __forceinline__ bool plan_cost_is_pairs(const KzFrame &f) { return !f.tile_pred && f.bin_ctrl; }
+
 // The fair share of one resident raster workgroup. Every workgroup sums
 // the parts by the same tree (lane-strided, then waves in order), so all
 // of them compute the same float.
 __device__ __forceinline__ float plan_share(const KzFrame &f, const uint32_t *table, float *s_wave) {
     const float *part = (const float *)(table + gridDim.x * kPlanCols);
+    const bool pairs = plan_cost_is_pairs(f);
     float v = 0.0f;
-    for (uint32_t w = threadIdx.x; w < gridDim.x; w += kPlanWg) v += part[w];
+    if (!pairs)
+        for (uint32_t w = threadIdx.x; w < gridDim.x; w += kPlanWg) v += part[w];
     for (uint32_t d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = v;
     __syncthreads();
     float total = 0.0f;
     for (uint32_t w = 0; w < kPlanWg / 64; ++w) total += s_wave[w];
+    if (pairs) total = (float)f.bin_ctrl[KZ_BIN_PAIRS];
     return fmaxf(total / (float)max(f.raster_slots, 1u), KZ_SPLIT_MIN);
 }
 
@@ -1113,9 +1146,9 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_count(KzFrame f, uint32_t nti
     __shared__ uint32_t s_cnt[kPlanCols];
     __shared__ float s_wave[kPlanWg / 64];
     if (threadIdx.x < kPlanCols) s_cnt[threadIdx.x] = 0;
-    uint32_t t[kPlanPer], n[kPlanPer];
+    uint32_t t[kPlanPer], n[kPlanPer], off[kPlanPer];
     float pred[kPlanPer];
-    plan_loads(f, ntiles, t, n, pred);
+    plan_loads(f, ntiles, t, n, pred, off);
     const float share = plan_share(f, table, s_wave);
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j) {
@@ -1136,9 +1169,9 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t nti
     __shared__ uint32_t s_cur[kPlanCols], s_tot[kPlanCols];
     __shared__ uint32_t s_gb[kPlanGroups][kPlanCols], s_gt[kPlanGroups][kPlanCols];
     __shared__ float s_wave[kPlanWg / 64];
-    uint32_t t[kPlanPer], n[kPlanPer];
+    uint32_t t[kPlanPer], n[kPlanPer], off[kPlanPer];
     float pred[kPlanPer];
-    plan_loads(f, ntiles, t, n, pred);
+    plan_loads(f, ntiles, t, n, pred, off);
     // this workgroup's start in every column: the columns' sums over the
     // workgroups before it, and their totals over all (kPlanGroups threads
     // per column, each taking every kPlanGroups-th workgroup)
@@ -1196,30 +1229,30 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t nti
             f.split_info[2 * slot] = c0;
             f.split_info[2 * slot + 1] = t[j];
             for (uint32_t s = 0; s < 16; ++s) f.split_counts[slot * 16 + s] = 0;  // kz_split_totals writes
-            for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c) f.chunk_tile[c0 + c] = t[j];
+            for (uint32_t c = 0; c < nc && c0 + c < f.max_chunks; ++c)
+                ((uint4 *)f.chunk_desc)[c0 + c] = make_uint4(off[j] + c * KZ_SPLIT_CHUNK, t[j], (slot << 2) | level,
+                                                             min(KZ_SPLIT_CHUNK, n[j] - c * KZ_SPLIT_CHUNK));
         }
     }
 }
 
-// A KZ_SPLIT_CHUNK-entry piece of a split tile's (sorted) list.
+// A KZ_SPLIT_CHUNK-entry piece of a split tile's (sorted) list, from the
+// descriptor kz_plan_queue wrote (one load).
 struct Chunk {
     const uint32_t *ids;
     uint32_t n, nsub, level, ssh;
-    uint32_t slot, c0, j;  // split slot, its first chunk, this chunk's index in the tile
+    uint32_t slot;  // split slot
     int32_t X0, Y0;
 };
 
 __device__ __forceinline__ Chunk chunk_at(const KzFrame &f, uint32_t c) {
     Chunk k;
-    const uint32_t t = f.chunk_tile[c];
-    k.slot = f.split_slot[t];
-    k.c0 = f.split_info[2 * k.slot];
-    const uint32_t j = c - k.c0;
-    k.j = j;
-    const uint32_t n = f.tile_counts[t];
-    k.ids = f.tile_lists + f.tile_offsets[t] + j * KZ_SPLIT_CHUNK;
-    k.n = min(KZ_SPLIT_CHUNK, n - j * KZ_SPLIT_CHUNK);
-    k.level = f.tile_level[t];
+    const uint4 d = ((const uint4 *)f.chunk_desc)[c];
+    const uint32_t t = d.y;
+    k.ids = f.tile_lists + d.x;
+    k.n = d.w;
+    k.slot = d.z >> 2;
+    k.level = d.z & 3u;
     k.nsub = 1u << (2 * k.level);
     k.ssh = f.tile_shift - k.level;
     k.X0 = (int32_t)((t % f.tiles_x) << f.tile_shift);
@@ -1245,11 +1278,32 @@ __device__ __forceinline__ uint32_t sub_mask(uint64_t r, const Chunk &k) {
 // the masks instead of gathering pixel boxes again.
 constexpr uint32_t kSplitRounds = KZ_SPLIT_CHUNK / 64;
 
-// The sub-tile masks of a chunk's entries (lane l: entries r * 64 + l): all
-// ids, then all pixel boxes, are requested before any is used, so a wave
-// waits for two loads per chunk rather than two per 64 entries.
-__device__ __forceinline__ void chunk_masks(const KzFrame &f, const Chunk &k, uint32_t lane, uint32_t *id,
-                                            uint32_t *m) {
+// A split tile's sub-tiles of an entry from its 4 x 4 mask (box_sub_mask16):
+// at level 2 the mask itself, at level 1 the OR of each 2 x 2 group (bit
+// sy * 2 + sx).
+__device__ __forceinline__ uint32_t fold_sub_mask(uint32_t m16, uint32_t level) {
+    if (level >= 2) return m16;
+    const uint32_t c = m16 | (m16 >> 1);  // bit 4 sy + 2 sx: columns 2 sx and 2 sx + 1
+    const uint32_t r = c | (c >> 4);      // bit 8 sy + 2 sx: rows 2 sy and 2 sy + 1 as well
+    return (r & 1u) | ((r >> 1) & 2u) | ((r >> 6) & 4u) | ((r >> 7) & 8u);
+}
+
+// The sub-tile masks of a chunk's entries (lane l: entries r * 64 + l):
+// from the masks radix binning wrote beside the list, else from the
+// entries' pixel boxes (all ids, then all boxes, are requested before any
+// is used, so a wave waits for two loads per chunk rather than two per 64
+// entries).
+__device__ __forceinline__ void chunk_masks(const KzFrame &f, const Chunk &k, uint32_t lane, uint32_t *m) {
+    if (f.list_mask) {
+        const uint16_t *lm = f.list_mask + (k.ids - f.tile_lists);
+#pragma unroll
+        for (uint32_t r = 0; r < kSplitRounds; ++r) {
+            const uint32_t e = r * 64 + lane;
+            m[r] = e < k.n ? fold_sub_mask(lm[e], k.level) : 0u;
+        }
+        return;
+    }
+    uint32_t id[kSplitRounds];
     uint64_t box[kSplitRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSplitRounds; ++r) {
@@ -1268,8 +1322,8 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     const uint32_t waves = gridDim.x * (kSplitThreads / 64);
     for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
         const Chunk k = chunk_at(f, c);
-        uint32_t id[kSplitRounds], m[kSplitRounds];
-        chunk_masks(f, k, lane, id, m);
+        uint32_t m[kSplitRounds];
+        chunk_masks(f, k, lane, m);
         uint32_t cnt = 0;  // lane s < 16: entries of sub-tile s
 #pragma unroll
         for (uint32_t r = 0; r < kSplitRounds; ++r) {
@@ -1924,7 +1978,8 @@ hipError_t kz_launch_bin_pairs(hipStream_t st, const KzFrame &f, const KzRadixSc
     kz_pair_scan<<<1, kScanThreads, 0, st>>>(f.block_pairs, nblocks, s.chunk_offsets, nchunks, f.list_capacity,
                                              s.ctrl, f.stats);
     kz_pair_write<<<nchunks, kPairThreads, 0, st>>>(f.tri_rect, f.total_tris, f.tile_shift, f.tiles_x,
-                                                    s.chunk_offsets, s.ctrl, s.keys[0], s.vals[0], s.tile_flag);
+                                                    s.chunk_offsets, s.ctrl, s.keys[0], s.vals[0],
+                                                    f.list_mask ? 1u : 0u, s.tile_flag);
     return hipGetLastError();
 }
 
@@ -1943,7 +1998,8 @@ hipError_t kz_launch_bin_radix(hipStream_t st, const KzFrame &f, const KzRadixSc
         kz_radix_scan<<<1u << pb, kScanThreads, 0, st>>>(s.table, s.tstride, s.ctrl, s.totals);
         kz_radix_scatter<<<s.tstride, kRadixThreads, 0, st>>>(
             s.keys[in], s.vals[in], last ? nullptr : s.keys[out], last ? nullptr : s.vals[out],
-            last ? f.tile_lists : nullptr, last ? f.tile_counts : nullptr, last ? f.tile_offsets : nullptr, s.ctrl,
+            last ? f.tile_lists : nullptr, last ? (uint16_t *)f.list_mask : nullptr, last ? f.tile_counts : nullptr,
+            last ? f.tile_offsets : nullptr, s.ctrl,
             s.table, s.tstride, s.totals,
             shift, pb, last ? acc : RadixAcc{nullptr, nullptr, nullptr, 0, 0});
     }
@@ -2022,7 +2078,7 @@ uint32_t kz_plan_table_words(uint32_t ntiles) {
 
 hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table) {
     const uint32_t grid = std::max<uint32_t>((ntiles + kPlanTiles - 1) / kPlanTiles, 1);
-    kz_plan_cost<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
+    if (f.tile_pred || !f.bin_ctrl) kz_plan_cost<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     kz_plan_count<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     kz_plan_queue<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     return hipGetLastError();
