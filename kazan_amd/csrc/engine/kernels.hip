@@ -1316,6 +1316,27 @@ __device__ __forceinline__ void chunk_masks(const KzFrame &f, const Chunk &k, ui
     for (uint32_t r = 0; r < kSplitRounds; ++r) m[r] = box[r] != KZ_RECT_EMPTY ? sub_mask(box[r], k) : 0u;
 }
 
+// A chunk's entries per sub-tile (NSUB = 4 or 16 sub-tiles: the counts are
+// wave-uniform, held in scalar registers), and its entries' masks.
+template <uint32_t NSUB>
+__device__ __forceinline__ void split_count_chunk(const KzFrame &f, const Chunk &k, uint32_t c, uint32_t lane,
+                                                  const uint32_t *m) {
+    uint32_t cnt[NSUB];
+#pragma unroll
+    for (uint32_t s = 0; s < NSUB; ++s) cnt[s] = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) {
+        if (r * 64 >= k.n) break;
+        f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + r * 64 + lane] = (uint16_t)m[r];
+#pragma unroll
+        for (uint32_t s = 0; s < NSUB; ++s) cnt[s] += (uint32_t)__popcll(__ballot((m[r] >> s) & 1u));
+    }
+    uint32_t mine = 0;  // lane s: sub-tile s's count
+#pragma unroll
+    for (uint32_t s = 0; s < NSUB; ++s) mine = lane == s ? cnt[s] : mine;
+    if (lane < 16) f.chunk_counts[c * 16 + lane] = mine;
+}
+
 __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
     const uint32_t lane = lane_id();
     const uint32_t nch = f.ctrl[KZ_CTRL_CHUNKS];
@@ -1324,17 +1345,8 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_count(KzFrame f) {
         const Chunk k = chunk_at(f, c);
         uint32_t m[kSplitRounds];
         chunk_masks(f, k, lane, m);
-        uint32_t cnt = 0;  // lane s < 16: entries of sub-tile s
-#pragma unroll
-        for (uint32_t r = 0; r < kSplitRounds; ++r) {
-            if (r * 64 >= k.n) break;
-            f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + r * 64 + lane] = (uint16_t)m[r];
-            for (uint32_t s = 0; s < k.nsub; ++s) {
-                const uint32_t c_s = (uint32_t)__popcll(__ballot((m[r] >> s) & 1u));
-                if (lane == s) cnt += c_s;
-            }
-        }
-        if (lane < 16) f.chunk_counts[c * 16 + lane] = cnt;
+        if (k.level == 2) split_count_chunk<16>(f, k, c, lane, m);
+        else split_count_chunk<4>(f, k, c, lane, m);
     }
 }
 
@@ -1420,6 +1432,39 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_offsets(KzFrame f) {
     }
 }
 
+// One chunk's entries into the sub-tile lists (NSUB = 4 or 16 sub-tiles).
+template <uint32_t NSUB>
+__device__ __forceinline__ void split_scatter_chunk(const KzFrame &f, const Chunk &k, uint32_t c, uint32_t lane) {
+    // next position in sub-tile s's list (the slot's list offset plus the
+    // entries of the tile's earlier chunks), wave-uniform: scalar registers
+    const uint32_t start =
+        lane < NSUB ? f.split_offsets[k.slot * 16 + lane] + f.chunk_counts[c * 16 + lane] : 0u;
+    uint32_t base[NSUB];
+#pragma unroll
+    for (uint32_t s = 0; s < NSUB; ++s) base[s] = readlane(start, s);
+    uint32_t id[kSplitRounds], m[kSplitRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) {
+        const uint32_t e = r * 64 + lane;
+        id[r] = e < k.n ? k.ids[e] : kNone;
+        m[r] = e < k.n ? f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + e] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kSplitRounds; ++r) {
+        if (r * 64 >= k.n) break;
+#pragma unroll
+        for (uint32_t s = 0; s < NSUB; ++s) {
+            const bool in = (m[r] >> s) & 1u;
+            const uint64_t bal = __ballot(in);
+            if (in) {
+                const uint32_t pos = base[s] + rank_in(bal);
+                if (pos < f.split_capacity) f.split_lists[pos] = id[r];
+            }
+            base[s] += (uint32_t)__popcll(bal);
+        }
+    }
+}
+
 // Writes the sub-tile lists: entries keep their list order, so each
 // sub-tile list is in primitive order.
 __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
@@ -1428,29 +1473,8 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
     const uint32_t waves = gridDim.x * (kSplitThreads / 64);
     for (uint32_t c = blockIdx.x * (kSplitThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
         const Chunk k = chunk_at(f, c);
-        // lane s < 16: next position in sub-tile s's list (the slot's list
-        // offset plus the entries of the tile's earlier chunks)
-        uint32_t base = lane < 16 ? f.split_offsets[k.slot * 16 + lane] + f.chunk_counts[c * 16 + lane] : 0u;
-        uint32_t id[kSplitRounds], m[kSplitRounds];
-#pragma unroll
-        for (uint32_t r = 0; r < kSplitRounds; ++r) {
-            const uint32_t e = r * 64 + lane;
-            id[r] = e < k.n ? k.ids[e] : kNone;
-            m[r] = e < k.n ? f.split_mask[(size_t)c * KZ_SPLIT_CHUNK + e] : 0u;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < kSplitRounds; ++r) {
-            if (r * 64 >= k.n) break;
-            for (uint32_t s = 0; s < k.nsub; ++s) {
-                const uint64_t bal = __ballot((m[r] >> s) & 1u);
-                const uint32_t b = readlane(base, s);
-                if ((m[r] >> s) & 1u) {
-                    const uint32_t pos = b + rank_in(bal);
-                    if (pos < f.split_capacity) f.split_lists[pos] = id[r];
-                }
-                if (lane == s) base += (uint32_t)__popcll(bal);
-            }
-        }
+        if (k.level == 2) split_scatter_chunk<16>(f, k, c, lane);
+        else split_scatter_chunk<4>(f, k, c, lane);
     }
 }
 
