@@ -536,6 +536,26 @@ int fastest(const BatchTuning &t) {
         if (t.ms[i] >= 0 && (best < 0 || t.ms[i] < t.ms[best])) best = (int)i;
     return best;
 }
+
+// The choice: the finalist with the lowest median of its runs (one lucky
+// run does not decide), else the fastest run.
+int final_choice(const BatchTuning &t) {
+    int best = -1;
+    float best_ms = 0.0f;
+    for (const std::vector<size_t> &runs : t.finals) {
+        std::vector<float> v;
+        for (size_t i : runs)
+            if (t.ms[i] >= 0) v.push_back(t.ms[i]);
+        if (v.empty()) continue;
+        std::sort(v.begin(), v.end());
+        const float med = v[v.size() / 2];
+        if (best < 0 || med < best_ms) {
+            best = (int)runs[0];
+            best_ms = med;
+        }
+    }
+    return best >= 0 ? best : fastest(t);
+}
 }  // namespace
 
 // Candidates, one HIP-event-timed trial per submission: every tile size
@@ -549,7 +569,8 @@ int fastest(const BatchTuning &t) {
 // at the fastest planned one, every other pair of tile size and grid size
 // (both were chosen with the deepest splits and the model; with fewer
 // splits smaller tiles on a larger grid may pay); then the three fastest
-// once more. The fastest time of any candidate decides.
+// twice more each. The finalist with the lowest median of its three runs
+// is kept.
 hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
     cfg = default_config(op);
     timed = false;
@@ -606,13 +627,17 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
                             if ((s != b.shift || d != b.grid_div) && op.variants->fits(s))
                                 t.cand.push_back({s, d, b.model, true, b.max_level, b.frag_scale});
             } else {
-                // the three fastest once more (a near tie is decided by the
-                // faster of two runs each, not by one run's noise)
+                // the three fastest twice more each (a near tie is decided
+                // by the median of three runs, not by one run's noise)
                 std::vector<size_t> order;
                 for (size_t i = 0; i < t.cand.size(); ++i)
                     if (t.ms[i] >= 0) order.push_back(i);
                 std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return t.ms[x] < t.ms[y]; });
-                for (size_t k = 0; k < order.size() && k < 3; ++k) t.cand.push_back(t.cand[order[k]]);
+                for (size_t k = 0; k < order.size() && k < 3; ++k) {
+                    t.finals.push_back({order[k], t.cand.size(), t.cand.size() + 1});
+                    t.cand.push_back(t.cand[order[k]]);
+                    t.cand.push_back(t.cand[order[k]]);
+                }
             }
             t.ms.resize(t.cand.size(), -1.0f);
         }
@@ -624,7 +649,7 @@ hipError_t Engine::pick_config(Op &op, BatchConfig &cfg, bool &timed) {
         timed = true;
         return hipSuccess;
     }
-    const int best = fastest(t);
+    const int best = final_choice(t);
     t.done = true;
     if (best >= 0) t.choice = t.cand[best];
     else t.choice = default_config(op);

@@ -142,9 +142,10 @@ struct BatchTuning {
     ~BatchTuning();
     bool done = false;
     int phase = 0;  // 0: tile sizes, 1: grid sizes, 2: plan and raster forms, 3: split depth, 4: fragment weight,
-                    // 5: every tile size x grid size again, 6: the three fastest again
+                    // 5: every tile size x grid size again, 6: the three fastest twice more each
     std::vector<BatchConfig> cand;
     std::vector<float> ms;           // per candidate, -1 = not timed
+    std::vector<std::vector<size_t>> finals;  // the finalists' runs (indices into cand)
     uint32_t next = 0;               // next candidate to try
     int pending = -1;                // candidate whose events are in flight
     hipEvent_t ev[2] = {nullptr, nullptr};

@@ -60,9 +60,9 @@ def test_autotune_trials_then_graph_replays(driver, monkeypatch, oracle):
     # timed trials: the 4 tile sizes 8..64, two grid sizes, the plan without
     # the model and the plain raster; when a planned raster is fastest, two
     # split depths, two fragment weights (with the model) and the 11 other
-    # pairs of tile size and grid size; the three fastest again; then the
-    # choice, a capture and graph launches
-    trials = 8 + (15 if pr.value else 0) + 3
+    # pairs of tile size and grid size; the three fastest twice more each;
+    # then the choice, a capture and graph launches
+    trials = 8 + (15 if pr.value else 0) + 6
     captures, launches = _graph_stats(ctx)
     assert captures == 1 and launches == frames - trials - 1, (captures, launches, trials)
 
