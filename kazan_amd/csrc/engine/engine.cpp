@@ -458,23 +458,22 @@ hipError_t Engine::run_batch(const Op &op, const BatchConfig &cfg) {
         // predicted costs, split expensive tiles, cut their sub-tile lists,
         // order the items
         KZ_TRY(mark(KS_PLAN, false));
-        if (acc) {
+        if (model) {  // features, predictions and the plan's cost parts: one launch
+            KZ_TRY(kz_launch_cost_plan(stream_, cost_model_d_, f, ntiles, plan_table_));
+            launches += 1;
+        } else if (acc) {
             KZ_TRY(kz_launch_tile_features(stream_, f, ntiles));
             launches += 1;
         }
-        if (model) {
-            KZ_TRY(kz_launch_cost_mlp(stream_, cost_model_d_, tile_feat_, ntiles, tile_pred_));
-            launches += 1;
-        }
-        KZ_TRY(kz_launch_plan(stream_, f, ntiles, plan_table_));
+        KZ_TRY(kz_launch_plan(stream_, f, ntiles, plan_table_, model));
         KZ_TRY(kz_launch_split(stream_, f, split_grid_, ntiles));
         KZ_TRY(mark(KS_PLAN, true));
         KZ_TRY(mark(KS_RASTER, false));
         const uint32_t grid = std::min(f.raster_slots, 16 * ntiles);
         KZ_TRY(hipModuleLaunchKernel(kern->raster_persistent, grid, 1, 1, 64, 1, 1, 0, stream_, args, nullptr));
         KZ_TRY(mark(KS_RASTER, true));
-        // the plan (2, or 3 with the cost pass), the split (4), the raster
-        launches += (f.tile_pred || !f.bin_ctrl) ? 8 : 7;
+        // the plan (2, or 3 with its own cost pass), the split (4), the raster
+        launches += (!model && !f.bin_ctrl) ? 8 : 7;
     }
     if (acc) {
         last_profile_tiles_ = ntiles;

@@ -61,7 +61,12 @@ hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntile
 // kz_plan_table_words(ntiles) words of per-workgroup counts), then the
 // sub-tile lists of split tiles on `grid` workgroups.
 uint32_t kz_plan_table_words(uint32_t ntiles);
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table);
+// parts_ready: kz_launch_cost_plan wrote the workgroups' cost parts
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table, bool parts_ready);
+// The balanced form's cost pass with a model: features, predictions and the
+// plan's cost parts in one launch (table: the plan's table)
+hipError_t kz_launch_cost_plan(hipStream_t st, const KzCostModel *model, const KzFrame &f, uint32_t ntiles,
+                               uint32_t *table);
 hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid, uint32_t ntiles);
 hipError_t kz_launch_stats_reduce(hipStream_t st, unsigned long long *stats);
 hipError_t kz_launch_resolve4(hipStream_t st, const uint32_t *src, uint32_t *dst, uint32_t width, int32_t x0,

@@ -1082,12 +1082,20 @@ __device__ __forceinline__ uint32_t bucket_reserve(uint32_t *s, uint32_t bucket,
 //   kz_plan_queue  from those counts, each workgroup's place in every bucket
 //                  (buckets most expensive first) and its split slots and
 //                  chunks, then the queue, the levels and the split tables.
-// table: [workgroups * kPlanCols] counts, then [workgroups] float parts.
+// table: [workgroups * kPlanCols] counts, then a float part of the frame's
+// cost per kPartTiles tiles.
 constexpr uint32_t kPlanWg = 256;
 constexpr uint32_t kPlanPer = 1;
 constexpr uint32_t kPlanTiles = kPlanWg * kPlanPer;
+constexpr uint32_t kPartTiles = 64;
+static_assert(kPlanPer == 1 && kPartTiles == 64, "a wave's tiles are one part");
 constexpr uint32_t kPlanCols = 34;  // 32 buckets, split tiles, chunks
 constexpr uint32_t kPlanGroups = kPlanWg / kPlanCols;  // threads per column when summing the table
+
+// the parts, after the counts of `plan_wgs` workgroups
+__device__ __forceinline__ float *plan_parts(uint32_t *table, uint32_t plan_wgs) {
+    return (float *)(table + plan_wgs * kPlanCols);
+}
 
 __device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, uint32_t *t, uint32_t *n, float *pred,
                                            uint32_t *off) {
@@ -1102,7 +1110,6 @@ __device__ __forceinline__ void plan_loads(const KzFrame &f, uint32_t ntiles, ui
 }
 
 __global__ __launch_bounds__(kPlanWg) void kz_plan_cost(KzFrame f, uint32_t ntiles, uint32_t *table) {
-    __shared__ float s_part[kPlanWg / 64];
     uint32_t t[kPlanPer], n[kPlanPer], off[kPlanPer];
     float pred[kPlanPer];
     plan_loads(f, ntiles, t, n, pred, off);
@@ -1111,13 +1118,8 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_cost(KzFrame f, uint32_t ntil
     for (uint32_t j = 0; j < kPlanPer; ++j)
         if (n[j]) part += tile_cost(f, n[j], pred[j]);
     for (uint32_t d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
-    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float s = 0.0f;
-        for (uint32_t w = 0; w < kPlanWg / 64; ++w) s += s_part[w];
-        ((float *)(table + gridDim.x * kPlanCols))[blockIdx.x] = s;
-    }
+    // (wave w of workgroup b: tiles [(4 b + w) * 64, ...))
+    if ((threadIdx.x & 63) == 0) plan_parts(table, gridDim.x)[blockIdx.x * (kPlanWg / 64) + (threadIdx.x >> 6)] = part;
 }
 
 // Without a model a tile's cost is its list entries, and the frame's total
@@ -1127,12 +1129,14 @@ __device__ __forceinline__ bool plan_cost_is_pairs(const KzFrame &f) { return !f
 // The fair share of one resident raster workgroup. Every workgroup sums
 // the parts by the same tree (lane-strided, then waves in order), so all
 // of them compute the same float.
-__device__ __forceinline__ float plan_share(const KzFrame &f, const uint32_t *table, float *s_wave) {
-    const float *part = (const float *)(table + gridDim.x * kPlanCols);
+__device__ __forceinline__ float plan_share(const KzFrame &f, uint32_t ntiles, const uint32_t *table,
+                                            float *s_wave) {
+    const float *part = (const float *)(table + gridDim.x * kPlanCols);  // (plan_parts)
     const bool pairs = plan_cost_is_pairs(f);
+    const uint32_t nparts = (ntiles + kPartTiles - 1) / kPartTiles;
     float v = 0.0f;
     if (!pairs)
-        for (uint32_t w = threadIdx.x; w < gridDim.x; w += kPlanWg) v += part[w];
+        for (uint32_t w = threadIdx.x; w < nparts; w += kPlanWg) v += part[w];
     for (uint32_t d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -1149,7 +1153,7 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_count(KzFrame f, uint32_t nti
     uint32_t t[kPlanPer], n[kPlanPer], off[kPlanPer];
     float pred[kPlanPer];
     plan_loads(f, ntiles, t, n, pred, off);
-    const float share = plan_share(f, table, s_wave);
+    const float share = plan_share(f, ntiles, table, s_wave);
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j) {
         float cost = 0.0f;
@@ -1211,7 +1215,7 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t nti
             f.ctrl[KZ_CTRL_CHUNKS] = min(s_tot[33], f.max_chunks);
         }
     }
-    const float share = plan_share(f, table, s_wave);  // (its barrier also publishes s_cur)
+    const float share = plan_share(f, ntiles, table, s_wave);  // (its barrier also publishes s_cur)
 #pragma unroll
     for (uint32_t j = 0; j < kPlanPer; ++j) {
         float cost = 0.0f;
@@ -1491,9 +1495,7 @@ __global__ __launch_bounds__(kSplitThreads) void kz_split_scatter(KzFrame f) {
 //   9..14 pipeline: log2(samples) / 2, blend, depth test, varyings / 16,
 //         textures / 4, uniform buffers / 4
 //   15    fraction of the tile inside the framebuffer
-__global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
+__device__ __forceinline__ void tile_features(const KzFrame &f, uint32_t t, float *o) {
     const uint32_t cnt = f.tile_counts[t];
     const float n = (float)cnt;
     const uint32_t *a = f.tile_acc + (size_t)t * KZ_ACC;
@@ -1501,7 +1503,6 @@ __global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
     const uint32_t ts = 1u << f.tile_shift, p = f.pipe_info;
     const uint32_t X0 = (t % f.tiles_x) * ts, Y0 = (t / f.tiles_x) * ts;
     const float inside = (float)(min(ts, f.width - X0) * min(ts, f.height - Y0)) / (float)(ts * ts);
-    float *o = f.tile_feat + (size_t)t * KZ_FEATURES;
     o[0] = log2f(1.0f + n);
     o[1] = log2f(1.0f + (float)a[KZ_ACC_AREA]);
     for (uint32_t b = 0; b < 5; ++b) o[2 + b] = (float)a[KZ_ACC_HIST + b] * inv;
@@ -1515,6 +1516,16 @@ __global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
     o[13] = (float)KZ_PIPE_TEXTURES(p) * 0.25f;
     o[14] = (float)KZ_PIPE_UBOS(p) * 0.25f;
     o[15] = inside;
+}
+
+__global__ void kz_tile_features(KzFrame f, uint32_t ntiles) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    float o[KZ_FEATURES];
+    tile_features(f, t, o);
+    float *out = f.tile_feat + (size_t)t * KZ_FEATURES;
+#pragma unroll
+    for (uint32_t k = 0; k < KZ_FEATURES; ++k) out[k] = o[k];
 }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -1583,54 +1594,128 @@ __device__ __forceinline__ void mlp_layer(const short *in, uint32_t in_stride, s
 // Predicted fragments per tile: pred[t] = max(exp2(MLP(features)) - 1, 0).
 // Each wave evaluates 16 tiles at a time; weights are staged in LDS once per
 // workgroup.
-__global__ __launch_bounds__(kMlpThreads) void kz_cost_mlp(const KzCostModel *m, const float *feat, uint32_t ntiles,
-                                                           float *pred) {
-    // (16-byte aligned: the hidden layers read 8 bf16 at a time)
-    __shared__ __attribute__((aligned(16))) short s_w1[KZ_HIDDEN * KZ_FEATURES], s_w2[KZ_HIDDEN * KZ_HIDDEN],
-        s_w3[KZ_HIDDEN * KZ_HIDDEN];
-    __shared__ float s_b[3][KZ_HIDDEN], s_w4[KZ_HIDDEN];
-    __shared__ __attribute__((aligned(16))) short s_act[kMlpThreads / 64][16 * KZ_HIDDEN];
-    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_FEATURES; i += kMlpThreads) s_w1[i] = (short)m->w1[i];
-    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_HIDDEN; i += kMlpThreads) {
-        s_w2[i] = (short)m->w2[i];
-        s_w3[i] = (short)m->w3[i];
+// The model's weights in LDS (bf16 as stored, fp32 biases and output row)
+// and each wave's activations, for a workgroup of THREADS threads; 16-byte
+// aligned: the hidden layers read 8 bf16 at a time.
+template <uint32_t THREADS>
+struct MlpShared {
+    __attribute__((aligned(16))) short w1[KZ_HIDDEN * KZ_FEATURES], w2[KZ_HIDDEN * KZ_HIDDEN], w3[KZ_HIDDEN * KZ_HIDDEN];
+    float b[3][KZ_HIDDEN], w4[KZ_HIDDEN];
+    __attribute__((aligned(16))) short act[THREADS / 64][16 * KZ_HIDDEN];
+};
+
+template <uint32_t THREADS>
+__device__ __forceinline__ void mlp_stage(const KzCostModel *m, MlpShared<THREADS> &sh) {
+    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_FEATURES; i += THREADS) sh.w1[i] = (short)m->w1[i];
+    for (uint32_t i = threadIdx.x; i < KZ_HIDDEN * KZ_HIDDEN; i += THREADS) {
+        sh.w2[i] = (short)m->w2[i];
+        sh.w3[i] = (short)m->w3[i];
     }
     if (threadIdx.x < KZ_HIDDEN) {
-        s_b[0][threadIdx.x] = m->b1[threadIdx.x];
-        s_b[1][threadIdx.x] = m->b2[threadIdx.x];
-        s_b[2][threadIdx.x] = m->b3[threadIdx.x];
-        s_w4[threadIdx.x] = m->w4[threadIdx.x];
+        sh.b[0][threadIdx.x] = m->b1[threadIdx.x];
+        sh.b[1][threadIdx.x] = m->b2[threadIdx.x];
+        sh.b[2][threadIdx.x] = m->b3[threadIdx.x];
+        sh.w4[threadIdx.x] = m->w4[threadIdx.x];
     }
     __syncthreads();
+}
+
+// The prediction for tile `row` of a wave's 16 (lane: row = lane % 16, its
+// normalised input features quad * 4 .. + 3 in x, quad = lane / 16); every
+// lane of the wave takes part, lanes row + 16 q return the same value.
+template <uint32_t THREADS>
+__device__ __forceinline__ float mlp_rows(const KzCostModel *m, MlpShared<THREADS> &sh, const float *x, uint32_t lane) {
+    const uint32_t row = lane & 15, quad = lane >> 4;
+    short *act = sh.act[threadIdx.x >> 6];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) act[row * KZ_HIDDEN + quad * 4 + j] = to_bf16(x[j]);
+    __builtin_amdgcn_wave_barrier();
+    mlp_layer<KZ_FEATURES>(act, KZ_HIDDEN, act, sh.w1, sh.b[0], lane);
+    mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, sh.w2, sh.b[1], lane);
+    mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, sh.w3, sh.b[2], lane);
+    // output unit: lanes row, row + 16, row + 32, row + 48 share tile `row`
+    float y = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t u = quad * 16 + j;
+        y += __uint_as_float((uint32_t)(unsigned short)act[row * KZ_HIDDEN + u] << 16) * sh.w4[u];
+    }
+    y += __shfl_xor(y, 16, 64);
+    y += __shfl_xor(y, 32, 64);
+    __builtin_amdgcn_wave_barrier();
+    return fmaxf(exp2f(y + m->b4) - 1.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kMlpThreads) void kz_cost_mlp(const KzCostModel *m, const float *feat, uint32_t ntiles,
+                                                           float *pred) {
+    __shared__ MlpShared<kMlpThreads> sh;
+    mlp_stage(m, sh);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 15, quad = lane >> 4;
-    short *act = s_act[wave];
     const uint32_t stride = gridDim.x * (kMlpThreads / 64) * 16;
     for (uint32_t base = (blockIdx.x * (kMlpThreads / 64) + wave) * 16; base < ntiles; base += stride) {
         // normalised features of tile base + row, 4 per lane, as layer-1 input
         const uint32_t t = base + row;
+        float x[4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             const uint32_t k = quad * 4 + j;
-            const float x = t < ntiles ? (feat[(size_t)t * KZ_FEATURES + k] - m->feat_mean[k]) * m->feat_scale[k]
-                                       : 0.0f;
-            act[row * KZ_HIDDEN + k] = to_bf16(x);
+            x[j] = t < ntiles ? (feat[(size_t)t * KZ_FEATURES + k] - m->feat_mean[k]) * m->feat_scale[k] : 0.0f;
         }
-        __builtin_amdgcn_wave_barrier();
-        mlp_layer<KZ_FEATURES>(act, KZ_HIDDEN, act, s_w1, s_b[0], lane);
-        mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, s_w2, s_b[1], lane);
-        mlp_layer<KZ_HIDDEN>(act, KZ_HIDDEN, act, s_w3, s_b[2], lane);
-        // output unit: lanes row, row + 16, row + 32, row + 48 share tile `row`
-        float y = 0.0f;
+        const float p = mlp_rows(m, sh, x, lane);
+        if (quad == 0 && t < ntiles) pred[t] = p;
+    }
+}
+
+// The balanced form's cost pass with a model, in the plan's tile partition
+// (workgroup w: tiles [w * kPlanTiles, ...)): each tile's features (also
+// kept in tile_feat for the tile profile), the model's prediction
+// (tile_pred) and the workgroup's part of the frame's cost for
+// kz_plan_count / kz_plan_queue, summed in a fixed order. Replaces
+// kz_tile_features, kz_cost_mlp and kz_plan_cost. A workgroup takes one
+// part's tiles, a wave 16 of them.
+constexpr uint32_t kCostPlanThreads = kPartTiles / 16 * 64;
+__global__ __launch_bounds__(kCostPlanThreads) void kz_cost_plan(const KzCostModel *m, KzFrame f, uint32_t ntiles,
+                                                                 uint32_t *table) {
+    __shared__ MlpShared<kCostPlanThreads> sh;
+    __shared__ float s_part[kCostPlanThreads / 64];
+    mlp_stage(m, sh);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t row = lane & 15, quad = lane >> 4;
+    float part = 0.0f;  // (lanes of quad 0: their tile's cost)
+    {
+        const uint32_t t = blockIdx.x * kPartTiles + wave * 16 + row;
+        float o[KZ_FEATURES];
+        if (t < ntiles) tile_features(f, t, o);
+        else
+            for (uint32_t k = 0; k < KZ_FEATURES; ++k) o[k] = 0.0f;
+        float x[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 16; ++j) {
-            const uint32_t u = quad * 16 + j;
-            y += __uint_as_float((uint32_t)(unsigned short)act[row * KZ_HIDDEN + u] << 16) * s_w4[u];
+        for (uint32_t j = 0; j < 4; ++j) {
+            // o[quad * 4 + j] without indexing a register array by a lane value
+            float v = o[j];
+            v = quad == 1 ? o[4 + j] : v;
+            v = quad == 2 ? o[8 + j] : v;
+            v = quad == 3 ? o[12 + j] : v;
+            const uint32_t k = quad * 4 + j;
+            x[j] = t < ntiles ? (v - m->feat_mean[k]) * m->feat_scale[k] : 0.0f;
+            if (t < ntiles) f.tile_feat[(size_t)t * KZ_FEATURES + k] = v;
         }
-        y += __shfl_xor(y, 16, 64);
-        y += __shfl_xor(y, 32, 64);
-        if (quad == 0 && t < ntiles) pred[t] = fmaxf(exp2f(y + m->b4) - 1.0f, 0.0f);
-        __builtin_amdgcn_wave_barrier();
+        const float p = mlp_rows(m, sh, x, lane);
+        if (quad == 0 && t < ntiles) {
+            f.tile_pred[t] = p;
+            uint32_t off, n;
+            kz_tile_list(f, t, off, n);
+            if (n) part += tile_cost(f, n, p);
+        }
+    }
+    for (uint32_t d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+    if (lane == 0) s_part[wave] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (uint32_t w = 0; w < kCostPlanThreads / 64; ++w) s += s_part[w];
+        plan_parts(table, (ntiles + kPlanTiles - 1) / kPlanTiles)[blockIdx.x] = s;
     }
 }
 
@@ -2097,12 +2182,19 @@ hipError_t kz_launch_sort_lists(hipStream_t st, const KzFrame &f, uint32_t ntile
 }
 
 uint32_t kz_plan_table_words(uint32_t ntiles) {
-    return ((ntiles + kPlanTiles - 1) / kPlanTiles) * (kPlanCols + 1);
+    return ((ntiles + kPlanTiles - 1) / kPlanTiles) * (kPlanCols + kPlanTiles / kPartTiles);
 }
 
-hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table) {
+hipError_t kz_launch_cost_plan(hipStream_t st, const KzCostModel *model, const KzFrame &f, uint32_t ntiles,
+                               uint32_t *table) {
+    const uint32_t grid = std::max<uint32_t>((ntiles + kPartTiles - 1) / kPartTiles, 1);
+    kz_cost_plan<<<grid, kCostPlanThreads, 0, st>>>(model, f, ntiles, table);
+    return hipGetLastError();
+}
+
+hipError_t kz_launch_plan(hipStream_t st, const KzFrame &f, uint32_t ntiles, uint32_t *table, bool parts_ready) {
     const uint32_t grid = std::max<uint32_t>((ntiles + kPlanTiles - 1) / kPlanTiles, 1);
-    if (f.tile_pred || !f.bin_ctrl) kz_plan_cost<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
+    if (!parts_ready && (f.tile_pred || !f.bin_ctrl)) kz_plan_cost<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     kz_plan_count<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     kz_plan_queue<<<grid, kPlanWg, 0, st>>>(f, ntiles, table);
     return hipGetLastError();

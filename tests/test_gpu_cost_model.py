@@ -205,6 +205,32 @@ def test_features_do_not_depend_on_the_binning(driver, monkeypatch):
     np.testing.assert_array_equal(feats[0][1], feats[1][1])
 
 
+def test_fused_cost_pass_matches_the_separate_kernels(driver, monkeypatch):
+    # the balanced form's one-launch cost pass (features, predictions and the
+    # plan's cost parts) against the plain form's feature and MLP kernels on
+    # the same frame, and the predictions against numpy
+    from kazan_amd.scenes.renderer import SceneRenderer
+    monkeypatch.setenv("KAZAN_AUTOTUNE", "0")
+    blob = _random_blob(5)
+    s = g.gaussian_cluster(100_000, 320, 240)
+    out = {}
+    for form in ("plain", "balanced"):
+        ctx = _ctx(driver, monkeypatch, form)
+        ctx.d.check("vkSetCostModelKAZAN", ctx.device, blob, len(blob))
+        ctx.d.vkSetTileProfileKAZAN(ctx.device, 1)
+        SceneRenderer(ctx, s).render(1)
+        out[form] = _profile(ctx)
+    fp, pp = out["plain"][0], out["plain"][3]
+    fb, pb = out["balanced"][0], out["balanced"][3]
+    assert fb.shape == (20 * 15, 16) and (fb[:, 0] > 0).any()
+    np.testing.assert_allclose(fb, fp, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(pb, pp, rtol=1e-5, atol=1e-6)
+    assert (pb > 0).any()
+    ref = M.predict_reference(blob, fb)
+    d = np.abs(np.log2(1 + pb) - np.log2(1 + ref))
+    assert np.percentile(d, 99) < 0.02 and d.max() < 0.1, (np.percentile(d, 99), d.max())
+
+
 @pytest.mark.parametrize("b4", [20.0, 200.0, -20.0])
 def test_balanced_pixels_do_not_depend_on_the_model(driver, monkeypatch, oracle, b4):
     # b4 = 20: every tile predicted at a million fragments, so every tile is
