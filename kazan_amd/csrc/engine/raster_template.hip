@@ -829,12 +829,19 @@ __device__ __forceinline__ float kz_clamp_coord(float v) {
 // weights (subTexelPrecisionBits = 4, vulkan-driver/src/api_impl.rs:1560):
 // f = u*(W*16) - 8, i = floor(f), texel = i >> 4, weight = i & 15, integer
 // lerp over 16ths, result = sum * (1/65280). NEAREST takes floor(u*W).
+// Texels are addressed by a 32-bit byte offset from the image's base (an
+// image is at most 16384 x 16384, maxImageDimension2D, so rows and offsets
+// fit 24-bit multiplies and 32 bits): one vector offset per load against a
+// wave-uniform base, no 64-bit address arithmetic.
+__device__ __forceinline__ uint32_t kz_texel(const KzTexture &t, uint32_t row, uint32_t x) {
+    return *(const uint32_t *)(t.data + ((row + x) << 2));
+}
+
 __device__ __forceinline__ void kz_sample(const KzTexture &t, float u, float v, float out[4]) {
     if (t.filter == 0) {
         const int32_t x = (int32_t)floorf(kz_clamp_coord(u * (float)t.w));
         const int32_t y = (int32_t)floorf(kz_clamp_coord(v * (float)t.h));
-        const uint32_t texel =
-            *(const uint32_t *)(t.data + ((size_t)kz_wrap(y, t.h, t.addr_v) * t.w + kz_wrap(x, t.w, t.addr_u)) * 4);
+        const uint32_t texel = kz_texel(t, __umul24(kz_wrap(y, t.h, t.addr_v), t.w), kz_wrap(x, t.w, t.addr_u));
 #pragma unroll
         for (int c = 0; c < 4; ++c) out[c] = (float)((texel >> (8 * c)) & 255u) * (1.0f / 255.0f);
         return;
@@ -846,9 +853,9 @@ __device__ __forceinline__ void kz_sample(const KzTexture &t, float u, float v, 
     const uint32_t ax = (uint32_t)(iu & 15), ay = (uint32_t)(iv & 15);
     const uint32_t xa = kz_wrap(x0, t.w, t.addr_u), xb = kz_wrap(x0 + 1, t.w, t.addr_u);
     const uint32_t ya = kz_wrap(y0, t.h, t.addr_v), yb = kz_wrap(y0 + 1, t.h, t.addr_v);
-    const uint32_t *row_a = (const uint32_t *)(t.data + (size_t)ya * t.w * 4);
-    const uint32_t *row_b = (const uint32_t *)(t.data + (size_t)yb * t.w * 4);
-    const uint32_t t00 = row_a[xa], t10 = row_a[xb], t01 = row_b[xa], t11 = row_b[xb];
+    const uint32_t row_a = __umul24(ya, t.w), row_b = __umul24(yb, t.w);
+    const uint32_t t00 = kz_texel(t, row_a, xa), t10 = kz_texel(t, row_a, xb), t01 = kz_texel(t, row_b, xa),
+                   t11 = kz_texel(t, row_b, xb);
     // two channels per 32-bit lerp: R and B (then G and A) in 16-bit fields;
     // the weights of each lerp sum to 16, so a field stays <= 255 * 16 * 16
     // = 65280 and never carries into the next (the same integers per channel)
