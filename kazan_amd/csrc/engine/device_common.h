@@ -231,14 +231,19 @@ enum {
 
 // ctrl words of the balanced mode
 enum {
-    KZ_CTRL_HEAD = 0,      // persistent raster: items taken beyond the first round
     KZ_CTRL_ITEMS = 1,     // queue length
     KZ_CTRL_SPLITS = 2,    // split slots used
     KZ_CTRL_CHUNKS = 3,    // chunks of split tiles' lists
     KZ_CTRL_SPLIT_MAX = 4, // largest sub-tile list total of any batch so far (sizing)
     KZ_CTRL_SPLIT_CURSOR = 5, // sub-tile list storage reserved so far this batch
-    KZ_CTRL_WORDS = 64
+    // persistent raster: per sub-queue, the items taken beyond the first
+    // round (KZ_QUEUES counters, KZ_CTRL_HEAD_STRIDE words apart: 128 bytes)
+    KZ_CTRL_HEADS = 64,
+    KZ_CTRL_HEAD_STRIDE = 32,
+    KZ_CTRL_WORDS = 64 + 8 * 32
 };
+// sub-queues of the persistent raster (one per XCD)
+#define KZ_QUEUES 8u
 
 // Entries of a split tile's list that one wave cuts into sub-tile lists
 // (small enough that a frame's split tiles keep every SIMD busy).

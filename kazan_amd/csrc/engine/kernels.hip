@@ -1208,7 +1208,7 @@ __global__ __launch_bounds__(kPlanWg) void kz_plan_queue(KzFrame f, uint32_t nti
             start += s_tot[b];
         }
         if (blockIdx.x == 0) {
-            f.ctrl[KZ_CTRL_HEAD] = 0;
+            for (uint32_t x = 0; x < KZ_QUEUES; ++x) f.ctrl[KZ_CTRL_HEADS + x * KZ_CTRL_HEAD_STRIDE] = 0;
             f.ctrl[KZ_CTRL_SPLIT_CURSOR] = 0;
             f.ctrl[KZ_CTRL_ITEMS] = start;
             f.ctrl[KZ_CTRL_SPLITS] = s_tot[32];
