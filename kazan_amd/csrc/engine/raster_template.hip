@@ -18,7 +18,9 @@
 // :1584, :1609): 1/16-px snapping with round-half-even, int64 edge
 // functions, top-left tie-break with y pointing down, standard 4x sample
 // positions, f32 barycentrics with a fixed operation order. The code is
-// compiled with -ffp-contract=off and IEEE division/square root.
+// compiled with -ffp-contract=off and IEEE division/square root; the
+// fragment shaders' own division and square root take Vulkan's precision
+// (kz_fs_div, kz_fs_sqrt).
 //
 // Kernels:
 //   kz_bin_count  one thread per triangle: vertex fetch + position part of
