@@ -498,6 +498,19 @@ __device__ __forceinline__ uint32_t kz_walk_or(const KzWalk &w) {
     return (uint32_t)(w.e[0] | w.e[1] | w.e[2]);
 }
 
+// at four samples per pixel: bit s set iff sample s of the current pixel is
+// outside (wrapping 32-bit sums, as the walk's steps)
+__device__ __forceinline__ uint32_t kz_walk_out4(const KzWalk &w) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < KZ_SAMPLES && s < 4; ++s) {
+        const uint32_t v = ((uint32_t)w.e[0] + (uint32_t)w.o[0][s]) | ((uint32_t)w.e[1] + (uint32_t)w.o[1][s]) |
+                           ((uint32_t)w.e[2] + (uint32_t)w.o[2][s]);
+        m |= (v >> 31) << s;
+    }
+    return m;
+}
+
 __device__ __forceinline__ uint32_t kz_walk_mask(const KzWalk &w) {
     uint32_t m = 0;
 #pragma unroll
@@ -1332,6 +1345,28 @@ __device__ unsigned long long kz_raster_rect(const KzFrame &f, int32_t X0, int32
             if (small && maxc > 0) {
                 cmask = ~(out >> (32 - maxc)) & ((1u << cand) - 1u);
                 cnt = (uint32_t)__popc(cmask);
+            }
+#elif KZ_SAMPLES == 4
+            // The same at four samples: a pixel's four sign bits (set:
+            // sample outside) shift in at the top of a 64-bit word, so that
+            // pixel j of maxc (at most 16) ends at bits 4 (16 - maxc + j) ..
+            // + 3; the count is of pixels with any sample covered.
+            kz_walk_init(wk, L.t, L.c, L.cx0, L.cy0, cw);
+            int32_t dx = 0;
+            uint64_t out = 0;
+            for (int32_t j = 0; j < maxc; ++j) {
+                const uint32_t om = kz_walk_out4(wk);
+                const bool wrap = dx == cw - 1;
+                kz_walk_step(wk, wrap);
+                dx = wrap ? 0 : dx + 1;
+                out = (out >> 4) | ((uint64_t)om << 60);
+            }
+            if (small && maxc > 0) {
+                const uint64_t in = ~(out >> (64 - 4 * maxc));  // pixel j at bits 4 j .. 4 j + 3
+                cmask = cand >= 16 ? in : in & ((1ull << (4 * cand)) - 1ull);
+                uint64_t any = cmask | (cmask >> 1);
+                any |= any >> 2;
+                cnt = (uint32_t)__popcll(any & 0x1111111111111111ull);
             }
 #else
             if (small) kz_walk_init(wk, L.t, L.c, L.cx0, L.cy0, cw);
