@@ -323,6 +323,30 @@ __device__ __forceinline__ int32_t kz_uclamp(int32_t x, int32_t lo, int32_t hi) 
 __device__ __forceinline__ int32_t kz_sabs(int32_t a) { return a < 0 ? (int32_t)(0u - (uint32_t)a) : a; }
 __device__ __forceinline__ float kz_fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x); }
 __device__ __forceinline__ float kz_fmod_glsl(float x, float y) { return x - y * __builtin_floorf(x / y); }
+__device__ __forceinline__ int32_t kz_bit_count(int32_t x) { return __builtin_popcount((uint32_t)x); }
+__device__ __forceinline__ int32_t kz_bit_reverse(int32_t x) { return (int32_t)__builtin_bitreverse32((uint32_t)x); }
+__device__ __forceinline__ int32_t kz_find_lsb(int32_t x) { return x == 0 ? -1 : __builtin_ctz((uint32_t)x); }
+__device__ __forceinline__ int32_t kz_find_umsb(int32_t x) { return x == 0 ? -1 : 31 - __builtin_clz((uint32_t)x); }
+__device__ __forceinline__ int32_t kz_find_smsb(int32_t x) { return kz_find_umsb(x < 0 ? ~x : x); }
+// Bit fields of offset + count <= 32 bits (SPIR-V leaves the rest undefined;
+// here a field is cut at bit 31 and an offset of 32 or more selects nothing).
+__device__ __forceinline__ uint32_t kz_field_mask(int32_t count) {
+    return (uint32_t)count >= 32u ? 0xFFFFFFFFu : (1u << (uint32_t)count) - 1u;
+}
+__device__ __forceinline__ int32_t kz_bitfield_insert(int32_t base, int32_t ins, int32_t off, int32_t count) {
+    if ((uint32_t)off >= 32u || count == 0) return base;
+    const uint32_t m = kz_field_mask(count) << (uint32_t)off;
+    return (int32_t)(((uint32_t)base & ~m) | (((uint32_t)ins << (uint32_t)off) & m));
+}
+__device__ __forceinline__ int32_t kz_bitfield_uextract(int32_t base, int32_t off, int32_t count) {
+    if ((uint32_t)off >= 32u || count == 0) return 0;
+    return (int32_t)(((uint32_t)base >> (uint32_t)off) & kz_field_mask(count));
+}
+__device__ __forceinline__ int32_t kz_bitfield_sextract(int32_t base, int32_t off, int32_t count) {
+    if ((uint32_t)off >= 32u || count == 0) return 0;
+    const uint32_t c = (uint32_t)count >= 32u ? 32u : (uint32_t)count;
+    return (int32_t)(((uint32_t)base >> (uint32_t)off) << (32u - c)) >> (32u - c);
+}
 __device__ __forceinline__ uint32_t kz_f2u(float x) {
     return !(x > 0.0f) ? 0u : (x >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)x);
 }

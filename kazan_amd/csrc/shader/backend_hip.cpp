@@ -359,6 +359,28 @@ struct StageGen {
             lanes_expr(res(), [&](int c) { return f + "(" + comp(ins.args[0], c) + ")"; });
             return true;
         }
+        case Op::Atan2:
+        case Op::BitCount:
+        case Op::BitReverse:
+        case Op::FindILsb:
+        case Op::FindUMsb:
+        case Op::FindSMsb:
+        case Op::BitFieldInsert:
+        case Op::BitFieldSExtract:
+        case Op::BitFieldUExtract: {
+            static const std::map<Op, const char *> fn = {
+                {Op::Atan2, "atan2f"}, {Op::BitCount, "kz_bit_count"}, {Op::BitReverse, "kz_bit_reverse"},
+                {Op::FindILsb, "kz_find_lsb"}, {Op::FindUMsb, "kz_find_umsb"}, {Op::FindSMsb, "kz_find_smsb"},
+                {Op::BitFieldInsert, "kz_bitfield_insert"}, {Op::BitFieldSExtract, "kz_bitfield_sextract"},
+                {Op::BitFieldUExtract, "kz_bitfield_uextract"}};
+            const std::string f = fn.at(ins.op);
+            lanes_expr(res(), [&](int c) {
+                std::string e = f + "(";
+                for (size_t k = 0; k < ins.args.size(); ++k) e += (k ? ", " : "") + comp(ins.args[k], c);
+                return e + ")";
+            });
+            return true;
+        }
         case Op::FCmp: {
             static const char *const form[] = {
                 "(%a == %b)", "(%a < %b || %a > %b)", "(%a < %b)", "(%a <= %b)", "(%a > %b)", "(%a >= %b)",

@@ -107,6 +107,15 @@ const char *op_text(Op op) {  // (op_name below)
     case Op::All: return "all";
     case Op::IsNan: return "is_nan";
     case Op::IsInf: return "is_inf";
+    case Op::Atan2: return "atan2";
+    case Op::BitCount: return "bit_count";
+    case Op::BitReverse: return "bit_reverse";
+    case Op::FindILsb: return "find_lsb";
+    case Op::FindUMsb: return "find_umsb";
+    case Op::FindSMsb: return "find_smsb";
+    case Op::BitFieldInsert: return "bitfield_insert";
+    case Op::BitFieldSExtract: return "bitfield_sextract";
+    case Op::BitFieldUExtract: return "bitfield_uextract";
     case Op::PtrIndex: return "ptr_index";
     case Op::Block: return "block";
     case Op::Loop: return "loop";

@@ -125,6 +125,15 @@ enum class Op : uint8_t {
     All,
     IsNan,
     IsInf,
+    Atan2,             // [y, x]
+    BitCount,
+    BitReverse,
+    FindILsb,          // -1 for 0
+    FindUMsb,          // -1 for 0
+    FindSMsb,          // -1 for 0 and -1
+    BitFieldInsert,    // [base, insert, offset, count]
+    BitFieldSExtract,  // [base, offset, count]
+    BitFieldUExtract,  // [base, offset, count]
     PtrIndex,  // [ptr, index] + field: stride in bytes -> ptr + index * stride
     Block,     // field: block; runs the block's body
     Loop,      // field: the loop's body block; runs it again on each continue

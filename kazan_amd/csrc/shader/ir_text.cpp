@@ -166,7 +166,9 @@ const std::map<std::string, Op> &op_names() {
                             Op::FMod,  Op::FRem,     Op::FSign,     Op::Trunc,    Op::RoundEven, Op::Exp2,
                             Op::Log2,  Op::Exp,      Op::Log,       Op::Pow,      Op::Sin,      Op::Cos,
                             Op::FCmp,  Op::ICmp,     Op::ConvUToF,  Op::ConvFToU, Op::Bitcast,  Op::Any,
-                            Op::All,   Op::IsNan,    Op::IsInf,     Op::PtrIndex, Op::Barrier,  Op::Atomic};
+                            Op::All,   Op::IsNan,    Op::IsInf,     Op::PtrIndex, Op::Barrier,  Op::Atomic,
+                            Op::Atan2, Op::BitCount, Op::BitReverse, Op::FindILsb, Op::FindUMsb, Op::FindSMsb,
+                            Op::BitFieldInsert, Op::BitFieldSExtract, Op::BitFieldUExtract};
         for (Op op : plain) m[op_name(op)] = op;
         return m;
     }();

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <set>
 
@@ -48,7 +49,9 @@ enum SOp : uint16_t {
     OpTypeRuntimeArray = 29, OpControlBarrier = 224, OpMemoryBarrier = 225, OpAtomicLoad = 227, OpAtomicStore = 228,
     OpAtomicExchange = 229, OpAtomicIIncrement = 232, OpAtomicIDecrement = 233, OpAtomicIAdd = 234,
     OpAtomicISub = 235, OpAtomicSMin = 236, OpAtomicUMin = 237, OpAtomicSMax = 238, OpAtomicUMax = 239,
-    OpAtomicAnd = 240, OpAtomicOr = 241, OpAtomicXor = 242,
+    OpAtomicAnd = 240, OpAtomicOr = 241, OpAtomicXor = 242, OpTranspose = 84, OpMatrixTimesScalar = 143,
+    OpVectorTimesMatrix = 144, OpMatrixTimesMatrix = 146, OpOuterProduct = 147, OpBitFieldInsert = 201,
+    OpBitFieldSExtract = 202, OpBitFieldUExtract = 203, OpBitReverse = 204, OpBitCount = 205,
 };
 
 enum Storage : uint32_t {
@@ -1272,11 +1275,15 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
     case OpAny:
     case OpAll:
     case OpIsNan:
-    case OpIsInf: {
+    case OpIsInf:
+    case OpBitReverse:
+    case OpBitCount: {
         const Val *a = V(o[2]);
         if (!a) return false;
         Op op;
         switch (ins.opcode) {
+        case OpBitReverse: op = Op::BitReverse; break;
+        case OpBitCount: op = Op::BitCount; break;
         case OpSNegate: op = Op::INeg; break;
         case OpNot:
         case OpLogicalNot: op = Op::Not; break;
@@ -1576,6 +1583,82 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
             vals[o[1]] = r;
             break;
         }
+        case OpBitFieldInsert:
+        case OpBitFieldSExtract:
+        case OpBitFieldUExtract: {
+            // offset and count are scalars that apply to every component
+            const Type rt = ir_type(o[0]);
+            if (rt.scalar == ir::Scalar::Void) return fail("unsupported result type");
+            std::vector<uint32_t> args;
+            for (size_t i = 2; i < o.size(); ++i) {
+                const Val *p = V(o[i]);
+                if (!p) return false;
+                args.push_back(p->ir[0]);
+            }
+            const size_t want = ins.opcode == OpBitFieldInsert ? 4 : 3;
+            if (args.size() != want) return fail("bit field instruction with " + std::to_string(args.size()) + " operands");
+            const Op op = ins.opcode == OpBitFieldInsert    ? Op::BitFieldInsert
+                          : ins.opcode == OpBitFieldSExtract ? Op::BitFieldSExtract
+                                                             : Op::BitFieldUExtract;
+            return define(o[0], o[1], emit(op, args, rt, {}, value_name(o[1])));
+        }
+        case OpMatrixTimesScalar:
+        case OpMatrixTimesMatrix:
+        case OpOuterProduct:
+        case OpTranspose: {
+            // a matrix is its columns; the result's column type is its element type
+            const SType &rs = types.at(o[0]);
+            if (rs.k != SType::Matrix) return fail("matrix instruction with a non-matrix result");
+            const Type ct = ir_type(rs.elem);
+            const Val *a = V(o[2]);
+            if (!a) return false;
+            Val r;
+            r.type = o[0];
+            if (ins.opcode == OpTranspose) {
+                // column c of the transpose holds row c: lane r of a's column r
+                for (uint32_t c = 0; c < rs.count; ++c) {
+                    std::vector<uint32_t> lanes;
+                    for (size_t k = 0; k < a->ir.size(); ++k)
+                        lanes.push_back(emit(Op::Extract, {a->ir[k]}, Type::f32(), {c}));
+                    r.ir.push_back(emit(Op::MakeVector, lanes, ct));
+                }
+            } else {
+                const Val *b = V(o[3]);
+                if (!b) return false;
+                for (uint32_t c = 0; c < rs.count; ++c) {
+                    if (ins.opcode == OpMatrixTimesScalar) {
+                        r.ir.push_back(emit(Op::FMul, {a->ir.at(c), emit(Op::Splat, {b->ir[0]}, ct)}, ct));
+                    } else if (ins.opcode == OpOuterProduct) {
+                        const uint32_t s = emit(Op::Extract, {b->ir[0]}, Type::f32(), {c});
+                        r.ir.push_back(emit(Op::FMul, {a->ir[0], emit(Op::Splat, {s}, ct)}, ct));
+                    } else {
+                        // column c = a * (column c of b)
+                        uint32_t acc = 0;
+                        for (size_t k = 0; k < a->ir.size(); ++k) {
+                            const uint32_t s = emit(Op::Extract, {b->ir.at(c)}, Type::f32(), {(uint32_t)k});
+                            const uint32_t prod = emit(Op::FMul, {a->ir[k], emit(Op::Splat, {s}, ct)}, ct);
+                            acc = k == 0 ? prod : emit(Op::FAdd, {acc, prod}, ct);
+                        }
+                        r.ir.push_back(acc);
+                    }
+                }
+            }
+            vals[o[1]] = r;
+            break;
+        }
+        case OpVectorTimesMatrix: {
+            // lane c = dot(v, column c)
+            const Val *v = V(o[2]);
+            const Val *mm = V(o[3]);
+            if (!v || !mm) return false;
+            std::vector<uint32_t> lanes;
+            for (uint32_t col : mm->ir) lanes.push_back(emit(Op::Dot, {v->ir[0], col}, Type::f32()));
+            Val r;
+            r.type = o[0];
+            r.ir.push_back(emit(Op::MakeVector, lanes, ir_type(o[0]), {}, value_name(o[1])));
+            vals[o[1]] = r;
+            break;
+        }
         case OpExtInst: {
             if (o[2] != glsl_set) return fail("unknown extended instruction set");
             const uint32_t inst = o[3];
@@ -1593,12 +1676,71 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 {1, 1}, {2, 1}, {3, 1}, {4, 1}, {5, 1}, {6, 1}, {8, 1}, {9, 1}, {10, 1}, {13, 1}, {14, 1},
                 {26, 2}, {27, 1}, {28, 1}, {29, 1}, {30, 1}, {31, 1}, {32, 1}, {37, 2}, {38, 2}, {39, 2},
                 {40, 2}, {41, 2}, {42, 2}, {43, 3}, {44, 3}, {45, 3}, {46, 3}, {48, 2}, {49, 3}, {66, 1},
-                {67, 2}, {68, 2}, {69, 1}, {71, 2}};
+                {67, 2}, {68, 2}, {69, 1}, {71, 2}, {7, 1}, {11, 1}, {12, 1}, {15, 1}, {16, 1}, {17, 1}, {18, 1},
+                {19, 1}, {20, 1}, {21, 1}, {22, 1}, {23, 1}, {24, 1}, {25, 2}, {33, 1}, {34, 1}, {50, 3}, {70, 3},
+                {72, 3}, {73, 1}, {74, 1}, {75, 1}, {79, 2}, {80, 2}, {81, 3}};
             auto ar = arity.find(inst);
             if (ar == arity.end()) return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
             if (a.size() != ar->second)
                 return fail("GLSL.std.450 instruction " + std::to_string(inst) + " takes " +
                             std::to_string(ar->second) + " operands");
+            if (inst == 33 || inst == 34) {
+                // Determinant, MatrixInverse: cofactor expansion over the columns' lanes
+                // (element (row r, column c) is lane r of column c)
+                const size_t n = a[0]->ir.size();
+                if (n < 2 || n > 4 || types.at(a[0]->type).k != SType::Matrix)
+                    return fail("Determinant / MatrixInverse of a non-square or unsupported matrix");
+                uint32_t e[4][4];
+                for (size_t c = 0; c < n; ++c)
+                    for (size_t r = 0; r < n; ++r) e[r][c] = emit(Op::Extract, {a[0]->ir[c]}, Type::f32(), {(uint32_t)r});
+                const Type f = Type::f32();
+                std::function<uint32_t(const std::vector<int> &, const std::vector<int> &)> det =
+                    [&](const std::vector<int> &rows, const std::vector<int> &cols) -> uint32_t {
+                    if (rows.size() == 1) return e[rows[0]][cols[0]];
+                    if (rows.size() == 2)
+                        return emit(Op::FSub, {emit(Op::FMul, {e[rows[0]][cols[0]], e[rows[1]][cols[1]]}, f),
+                                               emit(Op::FMul, {e[rows[1]][cols[0]], e[rows[0]][cols[1]]}, f)}, f);
+                    const std::vector<int> rest(cols.begin() + 1, cols.end());
+                    uint32_t acc = 0;
+                    for (size_t i = 0; i < rows.size(); ++i) {
+                        std::vector<int> sub;
+                        for (size_t j = 0; j < rows.size(); ++j)
+                            if (j != i) sub.push_back(rows[j]);
+                        const uint32_t t = emit(Op::FMul, {e[rows[i]][cols[0]], det(sub, rest)}, f);
+                        acc = i == 0 ? t : emit(i % 2 ? Op::FSub : Op::FAdd, {acc, t}, f);
+                    }
+                    return acc;
+                };
+                std::vector<int> all;
+                for (size_t i = 0; i < n; ++i) all.push_back((int)i);
+                const uint32_t d = det(all, all);
+                Val r;
+                r.type = o[0];
+                if (inst == 33) {
+                    m.value(d).name = value_name(o[1]);
+                    r.ir.push_back(d);
+                } else {
+                    // inverse (row r, column c) = (-1)^(r+c) minor(row c, column r) / det
+                    const uint32_t rd = emit(Op::FDiv, {one(), d}, f);
+                    const Type ct = ir_type(types.at(o[0]).elem);
+                    for (size_t c = 0; c < n; ++c) {
+                        std::vector<uint32_t> lanes;
+                        for (size_t rr = 0; rr < n; ++rr) {
+                            std::vector<int> rows, cols;
+                            for (size_t k = 0; k < n; ++k) {
+                                if (k != c) rows.push_back((int)k);
+                                if (k != rr) cols.push_back((int)k);
+                            }
+                            uint32_t mnr = emit(Op::FMul, {det(rows, cols), rd}, f);
+                            if ((rr + c) % 2) mnr = emit(Op::FNeg, {mnr}, f);
+                            lanes.push_back(mnr);
+                        }
+                        r.ir.push_back(emit(Op::MakeVector, lanes, ct));
+                    }
+                }
+                vals[o[1]] = r;
+                break;
+            }
             if (rt.scalar == ir::Scalar::Void) return fail("GLSL.std.450 result of an unsupported type");
             auto splat = [&](uint32_t scalar) { return rt.lanes > 1 ? emit(Op::Splat, {scalar}, rt) : scalar; };
             auto unary = [&](Op op) { return emit(op, {a[0]->ir[0]}, rt); };
@@ -1612,6 +1754,91 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 res = emit(Op::Trunc, {emit(Op::FAdd, {a[0]->ir[0], h}, rt)}, rt);
                 break;
             }
+            // The forms below are those whose precision the Vulkan specification states
+            // for these functions ("inherited from" them), except where noted.
+            case 7: {                                                                                   // SSign: clamp(x, -1, 1)
+                const uint32_t lo = splat(m.new_const_i32(-1)), hi = splat(m.new_const_i32(1));
+                res = emit(Op::SClamp, {a[0]->ir[0], lo, hi}, rt);
+                break;
+            }
+            case 11: res = emit(Op::FMul, {a[0]->ir[0], splat(m.new_const_f32(0.017453292519943295f))}, rt); break;  // Radians
+            case 12: res = emit(Op::FMul, {a[0]->ir[0], splat(m.new_const_f32(57.29577951308232f))}, rt); break;     // Degrees
+            case 15: res = emit(Op::FDiv, {unary(Op::Sin), unary(Op::Cos)}, rt); break;              // Tan: sin / cos
+            case 16:                                                                                    // Asin: atan2(x, sqrt(1 - x x))
+            case 17: {                                                                                  // Acos: atan2(sqrt(1 - x x), x)
+                const uint32_t x = a[0]->ir[0];
+                const uint32_t c = emit(Op::Sqrt, {emit(Op::FSub, {splat(one()), emit(Op::FMul, {x, x}, rt)}, rt)}, rt);
+                res = inst == 16 ? emit(Op::Atan2, {x, c}, rt) : emit(Op::Atan2, {c, x}, rt);
+                break;
+            }
+            case 18: res = emit(Op::Atan2, {a[0]->ir[0], splat(one())}, rt); break;                  // Atan: atan2(x, 1)
+            case 25: res = binary(Op::Atan2); break;                                                   // Atan2
+            case 19:                                                                                    // Sinh: (e^x - e^-x) / 2
+            case 20: {                                                                                  // Cosh: (e^x + e^-x) / 2
+                const uint32_t ep = unary(Op::Exp), en = emit(Op::Exp, {emit(Op::FNeg, {a[0]->ir[0]}, rt)}, rt);
+                res = emit(Op::FMul, {emit(inst == 19 ? Op::FSub : Op::FAdd, {ep, en}, rt), splat(m.new_const_f32(0.5f))},
+                           rt);
+                break;
+            }
+            case 21: {                                                                                  // Tanh: 1 - 2 / (e^2x + 1)
+                // (sinh / cosh would give inf / inf for |x| > 89; this form saturates to +-1)
+                const uint32_t e2 = emit(Op::Exp, {emit(Op::FAdd, {a[0]->ir[0], a[0]->ir[0]}, rt)}, rt);
+                const uint32_t q = emit(Op::FDiv, {splat(m.new_const_f32(2.0f)), emit(Op::FAdd, {e2, splat(one())}, rt)}, rt);
+                res = emit(Op::FSub, {splat(one()), q}, rt);
+                break;
+            }
+            case 22: {                                                                                  // Asinh: sign(x) log(|x| + sqrt(x x + 1))
+                // (the odd symmetry keeps large negative x from cancelling)
+                const uint32_t x = a[0]->ir[0];
+                const uint32_t ax = emit(Op::FMax, {x, emit(Op::FNeg, {x}, rt)}, rt);
+                const uint32_t r = emit(Op::Sqrt, {emit(Op::FAdd, {emit(Op::FMul, {x, x}, rt), splat(one())}, rt)}, rt);
+                res = emit(Op::FMul, {unary(Op::FSign), emit(Op::Log, {emit(Op::FAdd, {ax, r}, rt)}, rt)}, rt);
+                break;
+            }
+            case 23: {                                                                                  // Acosh: log(x + sqrt(x x - 1))
+                const uint32_t x = a[0]->ir[0];
+                const uint32_t r = emit(Op::Sqrt, {emit(Op::FSub, {emit(Op::FMul, {x, x}, rt), splat(one())}, rt)}, rt);
+                res = emit(Op::Log, {emit(Op::FAdd, {x, r}, rt)}, rt);
+                break;
+            }
+            case 24: {                                                                                  // Atanh: log((1 + x) / (1 - x)) / 2
+                const uint32_t x = a[0]->ir[0];
+                const uint32_t q = emit(Op::FDiv, {emit(Op::FAdd, {splat(one()), x}, rt), emit(Op::FSub, {splat(one()), x}, rt)},
+                                        rt);
+                res = emit(Op::FMul, {emit(Op::Log, {q}, rt), splat(m.new_const_f32(0.5f))}, rt);
+                break;
+            }
+            case 50:                                                                                    // Fma: a b + c (OpFMul, OpFAdd)
+                res = emit(Op::FAdd, {emit(Op::FMul, {a[0]->ir[0], a[1]->ir[0]}, rt), a[2]->ir[0]}, rt);
+                break;
+            case 70: {                                                                                  // FaceForward: dot(Nref, I) < 0 ? N : -N
+                const uint32_t d = rt.lanes > 1 ? emit(Op::Dot, {a[2]->ir[0], a[1]->ir[0]}, Type::f32())
+                                                : emit(Op::FMul, {a[2]->ir[0], a[1]->ir[0]}, rt);
+                const uint32_t lt = emit(Op::FCmp, {d, m.new_const_f32(0.0f)}, Type::boolean(1), {ir::FOLt});
+                res = emit(Op::Select, {lt, a[0]->ir[0], emit(Op::FNeg, {a[0]->ir[0]}, rt)}, rt);
+                break;
+            }
+            case 72: {                                                                                  // Refract(I, N, eta)
+                // k = 1 - eta^2 (1 - dot(N, I)^2); k < 0 ? 0 : eta I - (eta dot(N, I) + sqrt(k)) N
+                const uint32_t i = a[0]->ir[0], nn = a[1]->ir[0], eta = a[2]->ir[0];
+                const Type f = Type::f32();
+                const uint32_t d = rt.lanes > 1 ? emit(Op::Dot, {nn, i}, f) : emit(Op::FMul, {nn, i}, f);
+                const uint32_t k = emit(Op::FSub, {one(), emit(Op::FMul, {emit(Op::FMul, {eta, eta}, f),
+                                                                          emit(Op::FSub, {one(), emit(Op::FMul, {d, d}, f)}, f)}, f)},
+                                        f);
+                const uint32_t t = emit(Op::FAdd, {emit(Op::FMul, {eta, d}, f), emit(Op::Sqrt, {k}, f)}, f);
+                const uint32_t v = emit(Op::FSub, {emit(Op::FMul, {splat(eta), i}, rt), emit(Op::FMul, {splat(t), nn}, rt)}, rt);
+                const uint32_t neg = emit(Op::FCmp, {k, m.new_const_f32(0.0f)}, Type::boolean(1), {ir::FOLt});
+                res = emit(Op::Select, {neg, splat(m.new_const_f32(0.0f)), v}, rt);
+                break;
+            }
+            case 73: res = unary(Op::FindILsb); break;
+            case 74: res = unary(Op::FindSMsb); break;
+            case 75: res = unary(Op::FindUMsb); break;
+            // NMin, NMax, NClamp: fminf / fmaxf already return the operand that is not a NaN
+            case 79: res = binary(Op::FMin); break;
+            case 80: res = binary(Op::FMax); break;
+            case 81: res = emit(Op::FClamp, {a[0]->ir[0], a[1]->ir[0], a[2]->ir[0]}, rt); break;
             case 2: res = unary(Op::RoundEven); break;
             case 3: res = unary(Op::Trunc); break;
             case 5: res = unary(Op::SAbs); break;

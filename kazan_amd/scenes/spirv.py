@@ -98,6 +98,18 @@ OpAtomicIAdd = 234
 OpULessThan = 176
 OpUGreaterThan = 172
 OpShiftLeftLogical = 196
+OpBitwiseXor = 198
+OpBitcast = 124
+OpTranspose = 84
+OpMatrixTimesScalar = 143
+OpVectorTimesMatrix = 144
+OpMatrixTimesMatrix = 146
+OpOuterProduct = 147
+OpBitFieldInsert = 201
+OpBitFieldSExtract = 202
+OpBitFieldUExtract = 203
+OpBitReverse = 204
+OpBitCount = 205
 
 # enumerants
 CapabilityShader = 1
@@ -158,6 +170,31 @@ GLSL_Sqrt = 31
 GLSL_InverseSqrt = 32
 GLSL_Length = 66
 GLSL_Normalize = 69
+GLSL_SSign = 7
+GLSL_Radians = 11
+GLSL_Degrees = 12
+GLSL_Tan = 15
+GLSL_Asin = 16
+GLSL_Acos = 17
+GLSL_Atan = 18
+GLSL_Sinh = 19
+GLSL_Cosh = 20
+GLSL_Tanh = 21
+GLSL_Asinh = 22
+GLSL_Acosh = 23
+GLSL_Atanh = 24
+GLSL_Atan2 = 25
+GLSL_Determinant = 33
+GLSL_MatrixInverse = 34
+GLSL_Fma = 50
+GLSL_FaceForward = 70
+GLSL_Refract = 72
+GLSL_FindILsb = 73
+GLSL_FindSMsb = 74
+GLSL_FindUMsb = 75
+GLSL_NMin = 79
+GLSL_NMax = 80
+GLSL_NClamp = 81
 
 
 def _string_words(s):
@@ -266,6 +303,10 @@ class Module:
 
     def t_mat4(self):
         return self._type(("mat4",), OpTypeMatrix, self.t_vec(4), 4)
+
+    def t_mat(self, n):
+        """An n x n float matrix (n columns of vec n)."""
+        return self.t_mat4() if n == 4 else self._type(("mat", n), OpTypeMatrix, self.t_vec(n), n)
 
     def t_ptr(self, storage, pointee):
         return self._type(("ptr", storage, pointee), OpTypePointer, storage, pointee)
@@ -433,6 +474,10 @@ class Function:
 
     def binop(self, op, type_id, a, b):
         return self._result(op, type_id, a, b)
+
+    def op(self, op, type_id, *operands):
+        """Any instruction with a result: op result_type result operands..."""
+        return self._result(op, type_id, *operands)
 
     def fadd(self, t, a, b):
         return self.binop(OpFAdd, t, a, b)
