@@ -2015,13 +2015,29 @@ __global__ void kz_resolve4(const uint32_t *src, uint32_t *dst, uint32_t width, 
     }
 }
 
-// Sums the KZ_STAT_STRIPES partial counters into the totals that follow them.
-__global__ void kz_stats_reduce(unsigned long long *stats) {
-    const uint32_t c = threadIdx.x;
-    if (c >= KZ_STAT_COUNT) return;
-    unsigned long long s = 0;
-    for (uint32_t k = 0; k < KZ_STAT_STRIPES; ++k) s += stats[(size_t)k * KZ_STAT_COUNT + c];
-    stats[(size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT + c] = s;
+// Sums the KZ_STAT_STRIPES partial counters into the totals that follow them:
+// thread t reads stripe t's KZ_STAT_COUNT counters (one load each, all in
+// flight at once), the waves sum them by butterfly and the first
+// KZ_STAT_COUNT threads add up the waves.
+constexpr uint32_t kStatsThreads = KZ_STAT_STRIPES;
+static_assert(kStatsThreads % 64 == 0 && kStatsThreads <= 1024, "one stripe per thread");
+__global__ __launch_bounds__(kStatsThreads) void kz_stats_reduce(unsigned long long *stats) {
+    __shared__ unsigned long long s_part[kStatsThreads / 64][KZ_STAT_COUNT];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    unsigned long long v[KZ_STAT_COUNT];
+#pragma unroll
+    for (uint32_t c = 0; c < KZ_STAT_COUNT; ++c) v[c] = stats[(size_t)tid * KZ_STAT_COUNT + c];
+#pragma unroll
+    for (uint32_t c = 0; c < KZ_STAT_COUNT; ++c) {
+        for (uint32_t d = 32; d > 0; d >>= 1) v[c] += __shfl_xor(v[c], (int)d, 64);
+        if (lane == 0) s_part[wave][c] = v[c];
+    }
+    __syncthreads();
+    if (tid < KZ_STAT_COUNT) {
+        unsigned long long s = 0;
+        for (uint32_t w = 0; w < kStatsThreads / 64; ++w) s += s_part[w][tid];
+        stats[(size_t)KZ_STAT_STRIPES * KZ_STAT_COUNT + tid] = s;
+    }
 }
 
 uint32_t grid_for(uint64_t n, uint32_t block) {
@@ -2209,7 +2225,7 @@ hipError_t kz_launch_split(hipStream_t st, const KzFrame &f, uint32_t grid, uint
 }
 
 hipError_t kz_launch_stats_reduce(hipStream_t st, unsigned long long *stats) {
-    kz_stats_reduce<<<1, 64, 0, st>>>(stats);
+    kz_stats_reduce<<<1, kStatsThreads, 0, st>>>(stats);
     return hipGetLastError();
 }
 
