@@ -54,6 +54,7 @@ struct KzComputeFrame {
     KzTexture tex[KZ_MAX_TEXTURES];
     const uint8_t *ubo[KZ_MAX_UBOS];
     uint8_t *ssbo[KZ_MAX_SSBOS];
+    uint32_t ssbo_bytes[KZ_MAX_SSBOS];  // the bound range of each storage buffer (OpArrayLength)
     uint32_t push[KZ_PUSH_WORDS];
     uint32_t base[3];
     uint32_t pad;

@@ -1204,14 +1204,28 @@ static void record_dispatch(CommandBuffer *cb, uint32_t bx, uint32_t by, uint32_
     f.base[1] = by;
     f.base[2] = bz;
     std::memcpy(f.push, st.push, sizeof f.push);
-    auto buffer_at = [&](uint32_t set, uint32_t binding) -> uint8_t * {
+    auto entry_at = [&](uint32_t set, uint32_t binding) -> const DescriptorEntry * {
         if (set >= 8 || !st.compute_sets[set]) return nullptr;
         auto it = st.compute_sets[set]->entries.find(binding);
         if (it == st.compute_sets[set]->entries.end() || !it->second.buffer) return nullptr;
-        return it->second.buffer->address() + it->second.offset;
+        return &it->second;
+    };
+    auto buffer_at = [&](uint32_t set, uint32_t binding) -> uint8_t * {
+        const DescriptorEntry *e = entry_at(set, binding);
+        return e ? e->buffer->address() + e->offset : nullptr;
     };
     for (const auto &kv : p->ubo_slot) f.ubo[kv.second] = buffer_at(kv.first.first, kv.first.second);
-    for (const auto &kv : p->ssbo_slot) f.ssbo[kv.second] = buffer_at(kv.first.first, kv.first.second);
+    for (const auto &kv : p->ssbo_slot) {
+        f.ssbo[kv.second] = buffer_at(kv.first.first, kv.first.second);
+        // the bound range (VK_WHOLE_SIZE: to the end of the buffer), for OpArrayLength
+        const DescriptorEntry *e = entry_at(kv.first.first, kv.first.second);
+        uint64_t bytes = 0;
+        if (e) {
+            const uint64_t size = e->buffer->size, off = e->offset;
+            bytes = off >= size ? 0 : (e->range == VK_WHOLE_SIZE ? size - off : std::min<uint64_t>(e->range, size - off));
+        }
+        f.ssbo_bytes[kv.second] = (uint32_t)std::min<uint64_t>(bytes, 0x7FFFFFFFu);
+    }
     cb->ops.push_back(op);
     st.batch_open = false;
 }

@@ -232,6 +232,12 @@ struct StageGen {
             os << "    return false;\n";
             return true;
         case Op::Unreachable: os << "    return true;\n"; return true;
+        case Op::BufferSize: {
+            auto it = ptrs.find(ins.args[0]);
+            if (it == ptrs.end() || it->second.root != Root::Storage) return fail("buffer size of a non-storage-buffer pointer");
+            os << "    " << lhs(res(), 0) << " = (int32_t)f.ssbo_bytes[" << it->second.index << "];\n";
+            return true;
+        }
         case Op::Load: {
             auto it = ptrs.find(ins.args[0]);
             if (it == ptrs.end()) return fail("load from an unknown pointer");

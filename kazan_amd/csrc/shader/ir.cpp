@@ -116,6 +116,7 @@ const char *op_text(Op op) {  // (op_name below)
     case Op::BitFieldInsert: return "bitfield_insert";
     case Op::BitFieldSExtract: return "bitfield_sextract";
     case Op::BitFieldUExtract: return "bitfield_uextract";
+    case Op::BufferSize: return "buffer_size";
     case Op::PtrIndex: return "ptr_index";
     case Op::Block: return "block";
     case Op::Loop: return "loop";

@@ -134,6 +134,7 @@ enum class Op : uint8_t {
     BitFieldInsert,    // [base, insert, offset, count]
     BitFieldSExtract,  // [base, offset, count]
     BitFieldUExtract,  // [base, offset, count]
+    BufferSize,        // [storage buffer pointer] -> bytes bound to it (i32)
     PtrIndex,  // [ptr, index] + field: stride in bytes -> ptr + index * stride
     Block,     // field: block; runs the block's body
     Loop,      // field: the loop's body block; runs it again on each continue

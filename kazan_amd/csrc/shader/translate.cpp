@@ -51,7 +51,7 @@ enum SOp : uint16_t {
     OpAtomicISub = 235, OpAtomicSMin = 236, OpAtomicUMin = 237, OpAtomicSMax = 238, OpAtomicUMax = 239,
     OpAtomicAnd = 240, OpAtomicOr = 241, OpAtomicXor = 242, OpTranspose = 84, OpMatrixTimesScalar = 143,
     OpVectorTimesMatrix = 144, OpMatrixTimesMatrix = 146, OpOuterProduct = 147, OpBitFieldInsert = 201,
-    OpBitFieldSExtract = 202, OpBitFieldUExtract = 203, OpBitReverse = 204, OpBitCount = 205,
+    OpBitFieldSExtract = 202, OpBitFieldUExtract = 203, OpBitReverse = 204, OpBitCount = 205, OpArrayLength = 68,
 };
 
 enum Storage : uint32_t {
@@ -1645,6 +1645,27 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
             }
             vals[o[1]] = r;
             break;
+        }
+        case OpArrayLength: {
+            // elements of the run-time array that ends the storage buffer's
+            // block: (bound bytes - the member's offset) / its stride, 0 when
+            // the range ends before the member
+            const Val *b = V(o[2]);
+            if (!b) return false;
+            if (!b->is_ptr || b->layout != 1) return fail("OpArrayLength of a non-buffer pointer");
+            const SType &st = types.at(b->type);
+            if (st.k != SType::Struct || o[3] >= st.members.size() || types.at(st.members[o[3]]).k != SType::Array ||
+                types.at(st.members[o[3]]).count != 0)
+                return fail("OpArrayLength of a member that is not a run-time array");
+            const uint32_t off = member_offset(b->type, o[3], 1);
+            const uint32_t stride = array_stride(st.members[o[3]], 1);
+            if (stride == 0) return fail("run-time array without a stride");
+            const Type i = Type::i32();
+            const uint32_t bytes = emit(Op::BufferSize, {b->ir[0]}, i);
+            const uint32_t over = emit(Op::UMax, {bytes, m.new_const_i32((int32_t)off)}, i);
+            const uint32_t n = emit(Op::UDiv, {emit(Op::ISub, {over, m.new_const_i32((int32_t)off)}, i),
+                                               m.new_const_i32((int32_t)stride)}, i, {}, value_name(o[1]));
+            return define(o[0], o[1], n);
         }
         case OpVectorTimesMatrix: {
             // lane c = dot(v, column c)

@@ -253,3 +253,32 @@ def cs_grid_vertices():
         return f
 
     return _compute_module((64, 1, 1), body).to_bytes()
+
+
+def cs_array_length():
+    """layout(local_size_x = 1) in;
+    layout(binding = 0) buffer A { uint count; vec4 v[]; };   // v at offset 16, stride 16
+    layout(binding = 1) buffer B { uint o[]; };
+    void main() { o[0] = v.length(); o[1] = o.length(); }"""
+
+    def body(m, builtin):
+        u, v4 = m.t_int(False), m.t_vec(4)
+        arr = m.t_runtime_array(v4)
+        m.decorate(arr, sp.DecorationArrayStride, 16)
+        st = m.t_struct(u, arr)
+        m.decorate(st, sp.DecorationBufferBlock)
+        m.member_decorate(st, 0, sp.DecorationOffset, 0)
+        m.member_decorate(st, 1, sp.DecorationOffset, 16)
+        a = m.variable(sp.StorageUniform, st, "a")
+        m.decorate(a, sp.DecorationDescriptorSet, 0)
+        m.decorate(a, sp.DecorationBinding, 0)
+        ob = _ssbo(m, u, 4, 1, "o")
+        f = sp.Function(m)
+        na = f.op(sp.OpArrayLength, u, a, 1)
+        no = f.op(sp.OpArrayLength, u, ob, 0)
+        f.store(_elem(f, m, ob, u, m.c_int(0)), na)
+        f.store(_elem(f, m, ob, u, m.c_int(1)), no)
+        f.ret()
+        return f
+
+    return _compute_module((1, 1, 1), body).to_bytes()

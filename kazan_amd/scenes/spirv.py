@@ -110,6 +110,7 @@ OpBitFieldSExtract = 202
 OpBitFieldUExtract = 203
 OpBitReverse = 204
 OpBitCount = 205
+OpArrayLength = 68
 
 # enumerants
 CapabilityShader = 1

@@ -19,7 +19,7 @@ from kazan_amd import vk
 from kazan_amd.scenes import compute_shaders as cs
 from kazan_amd.scenes import spirv as sp
 
-SHADERS = ["cs_saxpy", "cs_reduce", "cs_histogram", "cs_grid_vertices"]
+SHADERS = ["cs_saxpy", "cs_reduce", "cs_histogram", "cs_grid_vertices", "cs_array_length"]
 
 
 def _words(code):
@@ -70,7 +70,7 @@ def test_compute_compiles_for_gfx950(driver, name, tmp_path, monkeypatch):
     rc, msg = _compile(driver, code, 2)
     assert rc == 0, msg
     assert "cache_hit=0" in msg and int(msg.split("code_object_bytes=")[1].split()[0]) > 1000
-    want = {"cs_reduce": "256x1x1"}.get(name, "64x1x1")
+    want = {"cs_reduce": "256x1x1", "cs_array_length": "1x1x1"}.get(name, "64x1x1")
     assert "local_size=" + want in msg
     rc, msg = _compile(driver, code, 2)  # the second compile is served by the cache
     assert rc == 0 and "cache_hit=1" in msg, msg

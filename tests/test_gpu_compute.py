@@ -11,6 +11,8 @@ compared bit for bit:
 - a workgroup reduction through shared memory with barriers inside a loop
   (the same pairwise tree in numpy);
 - a histogram of atomic adds (np.bincount);
+- OpArrayLength of a run-time array after a 16-byte header and of a plain
+  one, under bound ranges shorter than the buffer and VK_WHOLE_SIZE;
 - a dispatch that writes a vertex buffer, drawn by the same command buffer:
   the frame equals the oracle's render of the vertices numpy computes.
 """
@@ -155,3 +157,25 @@ def test_compute_writes_vertices_then_draw(ctx, oracle):
         assert np.abs(c.astype(np.int16) - oc.astype(np.int16)).max() <= 1
     finally:
         k.destroy()
+
+
+@pytest.mark.parametrize("range_a,range_o,want", [
+    (1000, 64, ((1000 - 16) // 16, 16)),   # ranges shorter than the buffers
+    (8, 4096, (0, 1024)),                  # a range that ends before the array: 0 elements
+    (None, None, ((4096 - 16) // 16, 1024)),  # VK_WHOLE_SIZE: to the end of each buffer
+])
+def test_array_length(ctx, range_a, range_o, want):
+    whole = 0xFFFFFFFFFFFFFFFF
+    ab = ctx.device_buffer_with(np.zeros(1024, np.uint32))
+    ob = ctx.device_buffer_with(np.full(1024, 0xDEADBEEF, np.uint32))
+    k = ComputeKernel(ctx, cs.cs_array_length(), 2)
+    try:
+        k.bind([(ab[0], whole if range_a is None else range_a), (ob[0], whole if range_o is None else range_o)])
+        k.run((1,))
+        got = read_buffer(ctx, ob[0], 4096, np.uint32)
+        assert (int(got[0]), int(got[1])) == want
+        assert (got[2:] == 0xDEADBEEF).all()
+    finally:
+        k.destroy()
+        ctx.destroy_buffer(ab)
+        ctx.destroy_buffer(ob)
