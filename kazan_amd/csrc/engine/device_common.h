@@ -324,6 +324,12 @@ __device__ __forceinline__ int32_t kz_uclamp(int32_t x, int32_t lo, int32_t hi) 
 __device__ __forceinline__ int32_t kz_sabs(int32_t a) { return a < 0 ? (int32_t)(0u - (uint32_t)a) : a; }
 __device__ __forceinline__ float kz_fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x); }
 __device__ __forceinline__ float kz_fmod_glsl(float x, float y) { return x - y * __builtin_floorf(x / y); }
+__device__ __forceinline__ int32_t kz_f32_to_f16_bits(float x) {
+    return (int32_t)__builtin_bit_cast(unsigned short, (_Float16)x);
+}
+__device__ __forceinline__ float kz_f16_bits_to_f32(int32_t v) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(uint32_t)v);
+}
 __device__ __forceinline__ int32_t kz_bit_count(int32_t x) { return __builtin_popcount((uint32_t)x); }
 __device__ __forceinline__ int32_t kz_bit_reverse(int32_t x) { return (int32_t)__builtin_bitreverse32((uint32_t)x); }
 __device__ __forceinline__ int32_t kz_find_lsb(int32_t x) { return x == 0 ? -1 : __builtin_ctz((uint32_t)x); }

@@ -373,8 +373,11 @@ struct StageGen {
         case Op::FindSMsb:
         case Op::BitFieldInsert:
         case Op::BitFieldSExtract:
-        case Op::BitFieldUExtract: {
+        case Op::BitFieldUExtract:
+        case Op::F32ToF16:
+        case Op::F16ToF32: {
             static const std::map<Op, const char *> fn = {
+                {Op::F32ToF16, "kz_f32_to_f16_bits"}, {Op::F16ToF32, "kz_f16_bits_to_f32"},
                 {Op::Atan2, "atan2f"}, {Op::BitCount, "kz_bit_count"}, {Op::BitReverse, "kz_bit_reverse"},
                 {Op::FindILsb, "kz_find_lsb"}, {Op::FindUMsb, "kz_find_umsb"}, {Op::FindSMsb, "kz_find_smsb"},
                 {Op::BitFieldInsert, "kz_bitfield_insert"}, {Op::BitFieldSExtract, "kz_bitfield_sextract"},

@@ -117,6 +117,8 @@ const char *op_text(Op op) {  // (op_name below)
     case Op::BitFieldSExtract: return "bitfield_sextract";
     case Op::BitFieldUExtract: return "bitfield_uextract";
     case Op::BufferSize: return "buffer_size";
+    case Op::F32ToF16: return "f32_to_f16_bits";
+    case Op::F16ToF32: return "f16_bits_to_f32";
     case Op::PtrIndex: return "ptr_index";
     case Op::Block: return "block";
     case Op::Loop: return "loop";

@@ -135,6 +135,8 @@ enum class Op : uint8_t {
     BitFieldSExtract,  // [base, offset, count]
     BitFieldUExtract,  // [base, offset, count]
     BufferSize,        // [storage buffer pointer] -> bytes bound to it (i32)
+    F32ToF16,          // f32 -> i32 holding the binary16 bits (round to nearest even)
+    F16ToF32,          // i32 whose low 16 bits are binary16 -> f32
     PtrIndex,  // [ptr, index] + field: stride in bytes -> ptr + index * stride
     Block,     // field: block; runs the block's body
     Loop,      // field: the loop's body block; runs it again on each continue

@@ -168,7 +168,8 @@ const std::map<std::string, Op> &op_names() {
                             Op::FCmp,  Op::ICmp,     Op::ConvUToF,  Op::ConvFToU, Op::Bitcast,  Op::Any,
                             Op::All,   Op::IsNan,    Op::IsInf,     Op::PtrIndex, Op::Barrier,  Op::Atomic,
                             Op::Atan2, Op::BitCount, Op::BitReverse, Op::FindILsb, Op::FindUMsb, Op::FindSMsb,
-                            Op::BitFieldInsert, Op::BitFieldSExtract, Op::BitFieldUExtract, Op::BufferSize};
+                            Op::BitFieldInsert, Op::BitFieldSExtract, Op::BitFieldUExtract, Op::BufferSize,
+                            Op::F32ToF16, Op::F16ToF32};
         for (Op op : plain) m[op_name(op)] = op;
         return m;
     }();

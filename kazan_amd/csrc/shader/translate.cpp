@@ -1699,7 +1699,8 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 {40, 2}, {41, 2}, {42, 2}, {43, 3}, {44, 3}, {45, 3}, {46, 3}, {48, 2}, {49, 3}, {66, 1},
                 {67, 2}, {68, 2}, {69, 1}, {71, 2}, {7, 1}, {11, 1}, {12, 1}, {15, 1}, {16, 1}, {17, 1}, {18, 1},
                 {19, 1}, {20, 1}, {21, 1}, {22, 1}, {23, 1}, {24, 1}, {25, 2}, {33, 1}, {34, 1}, {50, 3}, {70, 3},
-                {72, 3}, {73, 1}, {74, 1}, {75, 1}, {79, 2}, {80, 2}, {81, 3}};
+                {72, 3}, {73, 1}, {74, 1}, {75, 1}, {79, 2}, {80, 2}, {81, 3}, {54, 1}, {55, 1}, {56, 1}, {57, 1},
+                {58, 1}, {59, 1}, {60, 1}, {61, 1}, {62, 1}, {63, 1}};
             auto ar = arity.find(inst);
             if (ar == arity.end()) return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
             if (a.size() != ar->second)
@@ -1851,6 +1852,68 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 const uint32_t v = emit(Op::FSub, {emit(Op::FMul, {splat(eta), i}, rt), emit(Op::FMul, {splat(t), nn}, rt)}, rt);
                 const uint32_t neg = emit(Op::FCmp, {k, m.new_const_f32(0.0f)}, Type::boolean(1), {ir::FOLt});
                 res = emit(Op::Select, {neg, splat(m.new_const_f32(0.0f)), v}, rt);
+                break;
+            }
+            case 54:                                                                                    // PackSnorm4x8
+            case 55:                                                                                    // PackUnorm4x8
+            case 56:                                                                                    // PackSnorm2x16
+            case 57:                                                                                    // PackUnorm2x16
+            case 58: {                                                                                  // PackHalf2x16
+                // field c = round(clamp(v[c], lo, 1) * scale) (snorm: two's complement),
+                // at bit c * width; half: the binary16 bits
+                const bool half = inst == 58, snorm = inst == 54 || inst == 56, four = inst <= 55;
+                const uint32_t width = four ? 8 : 16, n = four ? 4 : 2;
+                const float scale = four ? (snorm ? 127.0f : 255.0f) : (snorm ? 32767.0f : 65535.0f);
+                const Type i = Type::i32(), f = Type::f32();
+                uint32_t acc = 0;
+                for (uint32_t c = 0; c < n; ++c) {
+                    const uint32_t x = emit(Op::Extract, {a[0]->ir[0]}, f, {c});
+                    uint32_t q;
+                    if (half) {
+                        q = emit(Op::F32ToF16, {x}, i);
+                    } else {
+                        const uint32_t cl = emit(Op::FClamp, {x, m.new_const_f32(snorm ? -1.0f : 0.0f), one()}, f);
+                        const uint32_t r = emit(Op::RoundEven, {emit(Op::FMul, {cl, m.new_const_f32(scale)}, f)}, f);
+                        q = emit(snorm ? Op::ConvFToS : Op::ConvFToU, {r}, i);
+                        q = emit(Op::And, {q, m.new_const_i32((int32_t)((1u << width) - 1u))}, i);
+                    }
+                    if (c) q = emit(Op::Shl, {q, m.new_const_i32((int32_t)(c * width))}, i);
+                    acc = c ? emit(Op::Or, {acc, q}, i) : q;
+                }
+                res = acc;
+                break;
+            }
+            case 59:                                                                                    // UnpackHalf2x16
+            case 60:                                                                                    // UnpackSnorm2x16
+            case 61:                                                                                    // UnpackUnorm2x16
+            case 62:                                                                                    // UnpackSnorm4x8
+            case 63: {                                                                                  // UnpackUnorm4x8
+                // component c from the field at bit c * width: unorm f / scale,
+                // snorm clamp(f / scale, -1, 1) of the sign-extended field
+                const bool half = inst == 59, snorm = inst == 60 || inst == 62, four = inst >= 62;
+                const uint32_t width = four ? 8 : 16, n = four ? 4 : 2;
+                const float scale = four ? (snorm ? 127.0f : 255.0f) : (snorm ? 32767.0f : 65535.0f);
+                const Type i = Type::i32(), f = Type::f32();
+                const uint32_t u = a[0]->ir[0];
+                std::vector<uint32_t> lanes;
+                for (uint32_t c = 0; c < n; ++c) {
+                    uint32_t x;
+                    if (half) {
+                        const uint32_t bits = c ? emit(Op::ShrL, {u, m.new_const_i32(16)}, i) : u;
+                        x = emit(Op::F16ToF32, {emit(Op::And, {bits, m.new_const_i32(0xFFFF)}, i)}, f);
+                    } else if (snorm) {
+                        const uint32_t up = emit(Op::Shl, {u, m.new_const_i32((int32_t)(32 - (c + 1) * width))}, i);
+                        const uint32_t v = emit(Op::ShrA, {up, m.new_const_i32((int32_t)(32 - width))}, i);
+                        x = emit(Op::FDiv, {emit(Op::ConvSToF, {v}, f), m.new_const_f32(scale)}, f);
+                        x = emit(Op::FClamp, {x, m.new_const_f32(-1.0f), one()}, f);
+                    } else {
+                        const uint32_t sh = c ? emit(Op::ShrL, {u, m.new_const_i32((int32_t)(c * width))}, i) : u;
+                        const uint32_t v = emit(Op::And, {sh, m.new_const_i32((int32_t)((1u << width) - 1u))}, i);
+                        x = emit(Op::FDiv, {emit(Op::ConvUToF, {v}, f), m.new_const_f32(scale)}, f);
+                    }
+                    lanes.push_back(x);
+                }
+                res = emit(Op::MakeVector, lanes, rt);
                 break;
             }
             case 73: res = unary(Op::FindILsb); break;

@@ -196,6 +196,17 @@ GLSL_FindUMsb = 75
 GLSL_NMin = 79
 GLSL_NMax = 80
 GLSL_NClamp = 81
+GLSL_RoundEven = 2
+GLSL_PackSnorm4x8 = 54
+GLSL_PackUnorm4x8 = 55
+GLSL_PackSnorm2x16 = 56
+GLSL_PackUnorm2x16 = 57
+GLSL_PackHalf2x16 = 58
+GLSL_UnpackHalf2x16 = 59
+GLSL_UnpackSnorm2x16 = 60
+GLSL_UnpackUnorm2x16 = 61
+GLSL_UnpackSnorm4x8 = 62
+GLSL_UnpackUnorm4x8 = 63
 
 
 def _string_words(s):

@@ -2,7 +2,8 @@
 translator takes (it lists them as not yet translated,
 vulkan-shader-compiler/src/parse/unimplemented_instructions.rs): the
 trigonometric and hyperbolic family, Fma, FaceForward, Refract, NMin/NMax/NClamp,
-SSign, FindILsb/FindSMsb/FindUMsb, Determinant, MatrixInverse, and
+SSign, FindILsb/FindSMsb/FindUMsb, Determinant, MatrixInverse, the
+Pack/Unpack {Unorm4x8, Snorm4x8, Unorm2x16, Snorm2x16, Half2x16} family, and
 OpBitCount, OpBitReverse, OpBitField{Insert,SExtract,UExtract}, OpTranspose,
 OpMatrixTimesScalar, OpVectorTimesMatrix, OpMatrixTimesMatrix, OpOuterProduct.
 
@@ -13,7 +14,8 @@ the oracle's interpolated colour, on covered pixels, within 2 LSB (the colour
 is quantised to 8 bits before the formula: <= 1/2 LSB through slopes of at
 most ~1.2, interpolation <= 1 LSB, rounding). The integer shader works on the
 exact pixel coordinate (gl_FragCoord) and is checked bit for bit against the
-same bit operations restated in Python."""
+same bit operations restated in Python; so is the packing shader, whose
+inputs are chosen so that every packed field is exact."""
 import math
 
 import numpy as np
@@ -246,7 +248,76 @@ def _matrix_expected(x, y, z, w):
     return np.stack([r, gch, b, wch], -1)
 
 
-SHADERS = {"trig": fs_trig, "misc": fs_misc, "bits": fs_bits, "matrix": fs_matrix}
+def fs_pack():
+    # (ix, iy) = floor(gl_FragCoord.xy); a = ix & 255, b = iy & 255, c = (7 ix + iy) & 255
+    #   p  = packUnorm4x8(vec4(a / 255, b / 255, c / 255, 0.5)) -> bytes a, b, c, 128 (127.5 to even)
+    #   q  = packSnorm4x8(vec4(-1, -0.5, 0.25, 1.5))        -> 0x81, 0xC0, 0x20, 0x7F
+    #   ps = packSnorm2x16(vec2(-0.5, 0.5))                 -> 0x4000C000
+    #   h  = unpackHalf2x16(packHalf2x16(vec2(a / 8, -b)))  -> (a / 8, -b), exact
+    #   u2 = unpackUnorm2x16(packUnorm2x16(vec2(b / 255, 0.25)))
+    #   R = (p >> 8 (ix & 3)) & 255
+    #   G = ((q >> 8 (iy & 3)) & 255) ^ (ps >> 24) ^ ((ps >> 8) & 255)
+    #   B = (uint(h.x * 8) ^ uint(-h.y)) & 255
+    #   A = (uint(roundEven(unpackUnorm4x8(p).z * 255)) + uint(roundEven(u2.x * 255))
+    #        + uint(roundEven(unpackSnorm4x8(q).y * 127) + 64) + uint(roundEven(unpackSnorm2x16(ps).y * 2))) & 255
+    #   o = vec4(R, G, B, A) / 255
+    def body(m, var):
+        f32, v2, v4, u32, i32 = m.t_float(), m.t_vec(2), m.t_vec(4), m.t_int(False), m.t_int(True)
+        fc_in = var(sp.StorageInput, v4, "fc", builtin=sp.BuiltInFragCoord)
+        out = var(sp.StorageOutput, v4, "o", location=0)
+        f = sp.Function(m)
+        fc = f.load(v4, fc_in)
+        ix = f.op(sp.OpConvertFToU, u32, f.extract(f32, fc, 0))
+        iy = f.op(sp.OpConvertFToU, u32, f.extract(f32, fc, 1))
+        cu, k = (lambda v: m.c_int(v, signed=False)), m.c_float
+        band = lambda x, y: f.binop(sp.OpBitwiseAnd, u32, x, y)
+        iadd = lambda x, y: f.binop(sp.OpIAdd, u32, x, y)
+        shr = lambda x, y: f.binop(sp.OpShiftRightLogical, u32, x, y)
+        xor = lambda x, y: f.binop(sp.OpBitwiseXor, u32, x, y)
+        tof = lambda x: f.op(sp.OpConvertUToF, f32, x)
+        tou = lambda x: f.op(sp.OpConvertFToU, u32, x)
+        rnd = lambda x: f.ext(f32, sp.GLSL_RoundEven, x)
+        a, b = band(ix, cu(255)), band(iy, cu(255))
+        c = band(iadd(f.binop(sp.OpIMul, u32, ix, cu(7)), iy), cu(255))
+        inv = k(1.0 / 255.0)
+        p = f.ext(u32, sp.GLSL_PackUnorm4x8, f.construct(v4, f.fdiv(f32, tof(a), k(255.0)), f.fdiv(f32, tof(b), k(255.0)),
+                                                         f.fdiv(f32, tof(c), k(255.0)), k(0.5)))
+        q = f.ext(u32, sp.GLSL_PackSnorm4x8, m.c_vec(-1.0, -0.5, 0.25, 1.5))
+        ps = f.ext(u32, sp.GLSL_PackSnorm2x16, m.c_vec(-0.5, 0.5))
+        h = f.ext(v2, sp.GLSL_UnpackHalf2x16, f.ext(u32, sp.GLSL_PackHalf2x16, f.construct(
+            v2, f.fmul(f32, tof(a), k(0.125)), f.op(sp.OpFNegate, f32, tof(b)))))
+        u2 = f.ext(v2, sp.GLSL_UnpackUnorm2x16, f.ext(u32, sp.GLSL_PackUnorm2x16, f.construct(
+            v2, f.fdiv(f32, tof(b), k(255.0)), k(0.25))))
+        eight = lambda x: f.binop(sp.OpShiftLeftLogical, u32, x, cu(3))
+        r = band(shr(p, eight(band(ix, cu(3)))), cu(255))
+        g_ = xor(xor(band(shr(q, eight(band(iy, cu(3)))), cu(255)), shr(ps, cu(24))), band(shr(ps, cu(8)), cu(255)))
+        bl = band(xor(tou(f.fmul(f32, f.extract(f32, h, 0), k(8.0))), tou(f.op(sp.OpFNegate, f32, f.extract(f32, h, 1)))),
+                  cu(255))
+        t1 = tou(rnd(f.fmul(f32, f.extract(f32, f.ext(v4, sp.GLSL_UnpackUnorm4x8, p), 2), k(255.0))))
+        t2 = tou(rnd(f.fmul(f32, f.extract(f32, u2, 0), k(255.0))))
+        t3 = tou(f.fadd(f32, rnd(f.fmul(f32, f.extract(f32, f.ext(v4, sp.GLSL_UnpackSnorm4x8, q), 1), k(127.0))), k(64.0)))
+        t4 = tou(rnd(f.fmul(f32, f.extract(f32, f.ext(v2, sp.GLSL_UnpackSnorm2x16, ps), 1), k(2.0))))
+        al = band(iadd(iadd(t1, t2), iadd(t3, t4)), cu(255))
+        chans = [f.fmul(f32, tof(v), inv) for v in (r, g_, bl, al)]
+        f.store(out, f.construct(v4, *chans))
+        f.ret()
+        return f
+
+    return shaders._fragment_module(body).to_bytes()
+
+
+def _pack_expected(ix, iy):
+    a, b, c = ix & 255, iy & 255, (7 * ix + iy) & 255
+    r = (a, b, c, 128)[ix & 3]
+    g_ = (0x81, 0xC0, 0x20, 0x7F)[iy & 3] ^ 0x40 ^ 0xC0
+    bl = (a ^ b) & 255
+    # unpackUnorm4x8(p).z * 255 -> c; u2.x * 255 -> b (16-bit field round(b / 255 * 65535) / 65535 * 255);
+    # unpackSnorm4x8(q).y * 127 = -64 -> +64 = 0; unpackSnorm2x16(ps).y * 2 = 16384 / 32767 * 2 -> 1
+    al = (c + b + 0 + 1) & 255
+    return r, g_, bl, al
+
+
+SHADERS = {"trig": fs_trig, "misc": fs_misc, "bits": fs_bits, "matrix": fs_matrix, "pack": fs_pack}
 
 
 def test_expected_forms_pin_the_numpy_restatements():
@@ -322,3 +393,11 @@ def test_bits_render(ctx, oracle, monkeypatch):
 def test_matrix_render(ctx, oracle, monkeypatch):
     c, oc, covered = _render(ctx, oracle, monkeypatch, fs_matrix, 44)
     _check_float(c, oc, covered, _matrix_expected)
+
+
+@pytest.mark.gpu
+def test_pack_render(ctx, oracle, monkeypatch):
+    c, oc, covered = _render(ctx, oracle, monkeypatch, fs_pack, 45)
+    ys, xs, _ = np.nonzero(covered)
+    want = np.array([_pack_expected(int(x), int(y)) for y, x in zip(ys, xs)], dtype=np.int32)
+    np.testing.assert_array_equal(c[covered].astype(np.int32), want)
