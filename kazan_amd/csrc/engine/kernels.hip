@@ -40,7 +40,7 @@
 
 namespace {
 
-constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanThreads = 256;  // single-workgroup scans: 256 threads x 8 per step launch and sync faster than 1024
 constexpr uint32_t kSortThreads = 1024;
 constexpr uint32_t kSortCap = 16384;  // ids one workgroup sorts in LDS (64 KB)
 constexpr uint32_t kNone = 0xFFFFFFFFu;
