@@ -330,6 +330,16 @@ __device__ __forceinline__ int32_t kz_f32_to_f16_bits(float x) {
 __device__ __forceinline__ float kz_f16_bits_to_f32(int32_t v) {
     return (float)__builtin_bit_cast(_Float16, (unsigned short)(uint32_t)v);
 }
+__device__ __forceinline__ float kz_ldexp(float x, int32_t e) { return __builtin_ldexpf(x, e); }
+__device__ __forceinline__ float kz_frexp_mant(float x) {
+    int e;
+    return __builtin_frexpf(x, &e);
+}
+__device__ __forceinline__ int32_t kz_frexp_exp(float x) {
+    int e;
+    __builtin_frexpf(x, &e);
+    return e;
+}
 __device__ __forceinline__ int32_t kz_bit_count(int32_t x) { return __builtin_popcount((uint32_t)x); }
 __device__ __forceinline__ int32_t kz_bit_reverse(int32_t x) { return (int32_t)__builtin_bitreverse32((uint32_t)x); }
 __device__ __forceinline__ int32_t kz_find_lsb(int32_t x) { return x == 0 ? -1 : __builtin_ctz((uint32_t)x); }

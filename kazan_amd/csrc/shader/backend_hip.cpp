@@ -375,8 +375,12 @@ struct StageGen {
         case Op::BitFieldSExtract:
         case Op::BitFieldUExtract:
         case Op::F32ToF16:
-        case Op::F16ToF32: {
+        case Op::F16ToF32:
+        case Op::Ldexp:
+        case Op::FrexpMant:
+        case Op::FrexpExp: {
             static const std::map<Op, const char *> fn = {
+                {Op::Ldexp, "kz_ldexp"}, {Op::FrexpMant, "kz_frexp_mant"}, {Op::FrexpExp, "kz_frexp_exp"},
                 {Op::F32ToF16, "kz_f32_to_f16_bits"}, {Op::F16ToF32, "kz_f16_bits_to_f32"},
                 {Op::Atan2, "atan2f"}, {Op::BitCount, "kz_bit_count"}, {Op::BitReverse, "kz_bit_reverse"},
                 {Op::FindILsb, "kz_find_lsb"}, {Op::FindUMsb, "kz_find_umsb"}, {Op::FindSMsb, "kz_find_smsb"},

@@ -119,6 +119,9 @@ const char *op_text(Op op) {  // (op_name below)
     case Op::BufferSize: return "buffer_size";
     case Op::F32ToF16: return "f32_to_f16_bits";
     case Op::F16ToF32: return "f16_bits_to_f32";
+    case Op::Ldexp: return "ldexp";
+    case Op::FrexpMant: return "frexp_mantissa";
+    case Op::FrexpExp: return "frexp_exponent";
     case Op::PtrIndex: return "ptr_index";
     case Op::Block: return "block";
     case Op::Loop: return "loop";

@@ -137,6 +137,9 @@ enum class Op : uint8_t {
     BufferSize,        // [storage buffer pointer] -> bytes bound to it (i32)
     F32ToF16,          // f32 -> i32 holding the binary16 bits (round to nearest even)
     F16ToF32,          // i32 whose low 16 bits are binary16 -> f32
+    Ldexp,             // [x, exponent: i32] -> x * 2^exponent
+    FrexpMant,         // x = mantissa * 2^exponent, |mantissa| in [0.5, 1) (0 for 0)
+    FrexpExp,          // -> exponent (i32)
     PtrIndex,  // [ptr, index] + field: stride in bytes -> ptr + index * stride
     Block,     // field: block; runs the block's body
     Loop,      // field: the loop's body block; runs it again on each continue

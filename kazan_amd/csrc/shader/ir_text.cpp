@@ -169,7 +169,7 @@ const std::map<std::string, Op> &op_names() {
                             Op::All,   Op::IsNan,    Op::IsInf,     Op::PtrIndex, Op::Barrier,  Op::Atomic,
                             Op::Atan2, Op::BitCount, Op::BitReverse, Op::FindILsb, Op::FindUMsb, Op::FindSMsb,
                             Op::BitFieldInsert, Op::BitFieldSExtract, Op::BitFieldUExtract, Op::BufferSize,
-                            Op::F32ToF16, Op::F16ToF32};
+                            Op::F32ToF16, Op::F16ToF32, Op::Ldexp, Op::FrexpMant, Op::FrexpExp};
         for (Op op : plain) m[op_name(op)] = op;
         return m;
     }();

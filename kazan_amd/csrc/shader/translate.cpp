@@ -1700,7 +1700,7 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 {67, 2}, {68, 2}, {69, 1}, {71, 2}, {7, 1}, {11, 1}, {12, 1}, {15, 1}, {16, 1}, {17, 1}, {18, 1},
                 {19, 1}, {20, 1}, {21, 1}, {22, 1}, {23, 1}, {24, 1}, {25, 2}, {33, 1}, {34, 1}, {50, 3}, {70, 3},
                 {72, 3}, {73, 1}, {74, 1}, {75, 1}, {79, 2}, {80, 2}, {81, 3}, {54, 1}, {55, 1}, {56, 1}, {57, 1},
-                {58, 1}, {59, 1}, {60, 1}, {61, 1}, {62, 1}, {63, 1}};
+                {58, 1}, {59, 1}, {60, 1}, {61, 1}, {62, 1}, {63, 1}, {35, 2}, {51, 2}, {53, 2}};
             auto ar = arity.find(inst);
             if (ar == arity.end()) return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
             if (a.size() != ar->second)
@@ -1916,6 +1916,17 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 res = emit(Op::MakeVector, lanes, rt);
                 break;
             }
+            case 35:                                                                                    // Modf(x, *whole): x - trunc(x)
+            case 51: {                                                                                  // Frexp(x, *exponent): mantissa
+                if (!a[1]->is_ptr) return fail("Modf / Frexp without a pointer operand");
+                Val second;
+                second.type = a[1]->type;
+                second.ir.push_back(inst == 35 ? unary(Op::Trunc) : emit(Op::FrexpExp, {a[0]->ir[0]}, Type::i32(rt.lanes)));
+                if (!store(*a[1], second)) return false;
+                res = inst == 35 ? emit(Op::FSub, {a[0]->ir[0], second.ir[0]}, rt) : unary(Op::FrexpMant);
+                break;
+            }
+            case 53: res = binary(Op::Ldexp); break;                                                   // Ldexp
             case 73: res = unary(Op::FindILsb); break;
             case 74: res = unary(Op::FindSMsb); break;
             case 75: res = unary(Op::FindUMsb); break;

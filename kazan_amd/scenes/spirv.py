@@ -197,6 +197,9 @@ GLSL_NMin = 79
 GLSL_NMax = 80
 GLSL_NClamp = 81
 GLSL_RoundEven = 2
+GLSL_Modf = 35
+GLSL_Frexp = 51
+GLSL_Ldexp = 53
 GLSL_PackSnorm4x8 = 54
 GLSL_PackUnorm4x8 = 55
 GLSL_PackSnorm2x16 = 56

@@ -3,7 +3,8 @@ translator takes (it lists them as not yet translated,
 vulkan-shader-compiler/src/parse/unimplemented_instructions.rs): the
 trigonometric and hyperbolic family, Fma, FaceForward, Refract, NMin/NMax/NClamp,
 SSign, FindILsb/FindSMsb/FindUMsb, Determinant, MatrixInverse, the
-Pack/Unpack {Unorm4x8, Snorm4x8, Unorm2x16, Snorm2x16, Half2x16} family, and
+Pack/Unpack {Unorm4x8, Snorm4x8, Unorm2x16, Snorm2x16, Half2x16} family,
+Modf, Frexp and Ldexp (their second results through Function variables), and
 OpBitCount, OpBitReverse, OpBitField{Insert,SExtract,UExtract}, OpTranspose,
 OpMatrixTimesScalar, OpVectorTimesMatrix, OpMatrixTimesMatrix, OpOuterProduct.
 
@@ -14,8 +15,8 @@ the oracle's interpolated colour, on covered pixels, within 2 LSB (the colour
 is quantised to 8 bits before the formula: <= 1/2 LSB through slopes of at
 most ~1.2, interpolation <= 1 LSB, rounding). The integer shader works on the
 exact pixel coordinate (gl_FragCoord) and is checked bit for bit against the
-same bit operations restated in Python; so is the packing shader, whose
-inputs are chosen so that every packed field is exact."""
+same bit operations restated in Python; so are the packing shader and the
+Modf/Frexp/Ldexp shader, whose inputs are chosen so that every result is exact."""
 import math
 
 import numpy as np
@@ -317,7 +318,57 @@ def _pack_expected(ix, iy):
     return r, g_, bl, al
 
 
-SHADERS = {"trig": fs_trig, "misc": fs_misc, "bits": fs_bits, "matrix": fs_matrix, "pack": fs_pack}
+def fs_frexp():
+    # (ix, iy) = floor(gl_FragCoord.xy); a = ix & 255, b = iy & 7; v = a / 16 + 1/8 (exact)
+    #   fr = modf(v, wh); m = frexp(v, e)
+    #   R = uint(fr * 16) + 16 * (uint(wh) & 15)
+    #   G = uint(e + 2) + 8 * (uint(m * 16) - 8)
+    #   B = uint(ldexp(m, e) * 16) & 255          (= a + 2)
+    #   A = uint(ldexp(1.0, b) + ldexp(-0.5, -1)) (= 2^b - 0.25, truncated)
+    #   o = vec4(R, G, B, A) / 255
+    def body(m, var):
+        f32, v4, u32, i32 = m.t_float(), m.t_vec(4), m.t_int(False), m.t_int(True)
+        fc_in = var(sp.StorageInput, v4, "fc", builtin=sp.BuiltInFragCoord)
+        out = var(sp.StorageOutput, v4, "o", location=0)
+        f = sp.Function(m)
+        wh_var, e_var = f.var(f32), f.var(i32)
+        fc = f.load(v4, fc_in)
+        ix = f.op(sp.OpConvertFToU, u32, f.extract(f32, fc, 0))
+        iy = f.op(sp.OpConvertFToU, u32, f.extract(f32, fc, 1))
+        cu, k = (lambda v: m.c_int(v, signed=False)), m.c_float
+        band = lambda x, y: f.binop(sp.OpBitwiseAnd, u32, x, y)
+        iadd = lambda x, y: f.binop(sp.OpIAdd, u32, x, y)
+        imul = lambda x, y: f.binop(sp.OpIMul, u32, x, y)
+        tou = lambda x: f.op(sp.OpConvertFToU, u32, x)
+        a, b = band(ix, cu(255)), band(iy, cu(7))
+        v = f.fadd(f32, f.fmul(f32, f.op(sp.OpConvertUToF, f32, a), k(0.0625)), k(0.125))
+        fr = f.ext(f32, sp.GLSL_Modf, v, wh_var)
+        mt = f.ext(f32, sp.GLSL_Frexp, v, e_var)
+        wh, e = f.load(f32, wh_var), f.load(i32, e_var)
+        r = iadd(tou(f.fmul(f32, fr, k(16.0))), imul(cu(16), band(tou(wh), cu(15))))
+        g_ = iadd(f.binop(sp.OpIAdd, u32, e, m.c_int(2)), imul(cu(8), f.binop(sp.OpISub, u32, tou(f.fmul(f32, mt, k(16.0))), cu(8))))
+        bl = band(tou(f.fmul(f32, f.ext(f32, sp.GLSL_Ldexp, mt, e), k(16.0))), cu(255))
+        al = tou(f.fadd(f32, f.ext(f32, sp.GLSL_Ldexp, k(1.0), b), f.ext(f32, sp.GLSL_Ldexp, k(-0.5), m.c_int(-1))))
+        inv = k(1.0 / 255.0)
+        chans = [f.fmul(f32, f.op(sp.OpConvertUToF, f32, x), inv) for x in (r, g_, bl, al)]
+        f.store(out, f.construct(v4, *chans))
+        f.ret()
+        return f
+
+    return shaders._fragment_module(body).to_bytes()
+
+
+def _frexp_expected(ix, iy):
+    a, b = ix & 255, iy & 7
+    v = a / 16 + 0.125
+    wh = math.floor(v)
+    mt, e = math.frexp(v)
+    return (int((v - wh) * 16) + 16 * (wh & 15), (e + 2) + 8 * (int(mt * 16) - 8), (a + 2) & 255,
+            int(2.0 ** b - 0.25))
+
+
+SHADERS = {"trig": fs_trig, "misc": fs_misc, "bits": fs_bits, "matrix": fs_matrix, "pack": fs_pack,
+           "frexp": fs_frexp}
 
 
 def test_expected_forms_pin_the_numpy_restatements():
@@ -400,4 +451,12 @@ def test_pack_render(ctx, oracle, monkeypatch):
     c, oc, covered = _render(ctx, oracle, monkeypatch, fs_pack, 45)
     ys, xs, _ = np.nonzero(covered)
     want = np.array([_pack_expected(int(x), int(y)) for y, x in zip(ys, xs)], dtype=np.int32)
+    np.testing.assert_array_equal(c[covered].astype(np.int32), want)
+
+
+@pytest.mark.gpu
+def test_frexp_render(ctx, oracle, monkeypatch):
+    c, oc, covered = _render(ctx, oracle, monkeypatch, fs_frexp, 46)
+    ys, xs, _ = np.nonzero(covered)
+    want = np.array([_frexp_expected(int(x), int(y)) for y, x in zip(ys, xs)], dtype=np.int32)
     np.testing.assert_array_equal(c[covered].astype(np.int32), want)
