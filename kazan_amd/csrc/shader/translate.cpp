@@ -1452,6 +1452,15 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 } else {
                     r.ir.push_back(emit(Op::Extract, {col}, ir_type(o[0]), {o[4]}, value_name(o[1])));
                 }
+            } else if (ct.k == SType::Struct && c->ir.size() == ct.members.size()) {
+                // a struct value (ModfStruct, FrexpStruct) is its members
+                const uint32_t mem = c->ir.at(o[3]);
+                if (o.size() == 4) {
+                    r.ir.push_back(mem);
+                } else {
+                    if (o.size() != 5) return fail("nested extract from a struct value");
+                    r.ir.push_back(emit(Op::Extract, {mem}, ir_type(o[0]), {o[4]}, value_name(o[1])));
+                }
             } else if (ct.k == SType::Vector) {
                 if (o.size() != 4) return fail("nested extract from a vector");
                 r.ir.push_back(emit(Op::Extract, {c->ir[0]}, ir_type(o[0]), {o[3]}, value_name(o[1])));
@@ -1700,12 +1709,29 @@ bool Translator::body_ins(const SpirvInstruction &ins) {
                 {67, 2}, {68, 2}, {69, 1}, {71, 2}, {7, 1}, {11, 1}, {12, 1}, {15, 1}, {16, 1}, {17, 1}, {18, 1},
                 {19, 1}, {20, 1}, {21, 1}, {22, 1}, {23, 1}, {24, 1}, {25, 2}, {33, 1}, {34, 1}, {50, 3}, {70, 3},
                 {72, 3}, {73, 1}, {74, 1}, {75, 1}, {79, 2}, {80, 2}, {81, 3}, {54, 1}, {55, 1}, {56, 1}, {57, 1},
-                {58, 1}, {59, 1}, {60, 1}, {61, 1}, {62, 1}, {63, 1}, {35, 2}, {51, 2}, {53, 2}};
+                {58, 1}, {59, 1}, {60, 1}, {61, 1}, {62, 1}, {63, 1}, {35, 2}, {51, 2}, {53, 2}, {36, 1}, {52, 1}};
             auto ar = arity.find(inst);
             if (ar == arity.end()) return fail("unsupported GLSL.std.450 instruction " + std::to_string(inst));
             if (a.size() != ar->second)
                 return fail("GLSL.std.450 instruction " + std::to_string(inst) + " takes " +
                             std::to_string(ar->second) + " operands");
+            if (inst == 36 || inst == 52) {
+                // ModfStruct {x - trunc(x), trunc(x)}, FrexpStruct {mantissa, exponent}
+                const SType &st = types.at(o[0]);
+                if (st.k != SType::Struct || st.members.size() != 2) return fail("ModfStruct / FrexpStruct result type");
+                const Type ft = ir_type(st.members[0]), it = ir_type(st.members[1]);
+                if (ft.scalar != ir::Scalar::F32) return fail("ModfStruct / FrexpStruct result type");
+                Val r;
+                r.type = o[0];
+                if (inst == 36) {
+                    const uint32_t w = emit(Op::Trunc, {a[0]->ir[0]}, ft);
+                    r.ir = {emit(Op::FSub, {a[0]->ir[0], w}, ft), w};
+                } else {
+                    r.ir = {emit(Op::FrexpMant, {a[0]->ir[0]}, ft), emit(Op::FrexpExp, {a[0]->ir[0]}, it)};
+                }
+                vals[o[1]] = r;
+                break;
+            }
             if (inst == 33 || inst == 34) {
                 // Determinant, MatrixInverse: cofactor expansion over the columns' lanes
                 // (element (row r, column c) is lane r of column c)

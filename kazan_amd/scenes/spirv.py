@@ -198,6 +198,8 @@ GLSL_NMax = 80
 GLSL_NClamp = 81
 GLSL_RoundEven = 2
 GLSL_Modf = 35
+GLSL_ModfStruct = 36
+GLSL_FrexpStruct = 52
 GLSL_Frexp = 51
 GLSL_Ldexp = 53
 GLSL_PackSnorm4x8 = 54

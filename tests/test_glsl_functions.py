@@ -4,7 +4,8 @@ vulkan-shader-compiler/src/parse/unimplemented_instructions.rs): the
 trigonometric and hyperbolic family, Fma, FaceForward, Refract, NMin/NMax/NClamp,
 SSign, FindILsb/FindSMsb/FindUMsb, Determinant, MatrixInverse, the
 Pack/Unpack {Unorm4x8, Snorm4x8, Unorm2x16, Snorm2x16, Half2x16} family,
-Modf, Frexp and Ldexp (their second results through Function variables), and
+Modf, Frexp and Ldexp (their second results through Function variables),
+ModfStruct and FrexpStruct (struct-valued results), and
 OpBitCount, OpBitReverse, OpBitField{Insert,SExtract,UExtract}, OpTranspose,
 OpMatrixTimesScalar, OpVectorTimesMatrix, OpMatrixTimesMatrix, OpOuterProduct.
 
@@ -325,6 +326,7 @@ def fs_frexp():
     #   G = uint(e + 2) + 8 * (uint(m * 16) - 8)
     #   B = uint(ldexp(m, e) * 16) & 255          (= a + 2)
     #   A = uint(ldexp(1.0, b) + ldexp(-0.5, -1)) (= 2^b - 0.25, truncated)
+    #   R, G and B also add the differences (0) between modfStruct(v) / frexpStruct(v) and the above
     #   o = vec4(R, G, B, A) / 255
     def body(m, var):
         f32, v4, u32, i32 = m.t_float(), m.t_vec(4), m.t_int(False), m.t_int(True)
@@ -345,9 +347,16 @@ def fs_frexp():
         fr = f.ext(f32, sp.GLSL_Modf, v, wh_var)
         mt = f.ext(f32, sp.GLSL_Frexp, v, e_var)
         wh, e = f.load(f32, wh_var), f.load(i32, e_var)
-        r = iadd(tou(f.fmul(f32, fr, k(16.0))), imul(cu(16), band(tou(wh), cu(15))))
-        g_ = iadd(f.binop(sp.OpIAdd, u32, e, m.c_int(2)), imul(cu(8), f.binop(sp.OpISub, u32, tou(f.fmul(f32, mt, k(16.0))), cu(8))))
-        bl = band(tou(f.fmul(f32, f.ext(f32, sp.GLSL_Ldexp, mt, e), k(16.0))), cu(255))
+        ms = f.ext(m.t_struct(f32, f32), sp.GLSL_ModfStruct, v)
+        fs = f.ext(m.t_struct(f32, i32), sp.GLSL_FrexpStruct, v)
+        isub = lambda x, y: f.binop(sp.OpISub, u32, x, y)
+        x16 = lambda x: tou(f.fmul(f32, x, k(16.0)))
+        d_modf = iadd(isub(x16(f.extract(f32, ms, 0)), x16(fr)), isub(tou(f.extract(f32, ms, 1)), tou(wh)))
+        d_exp = isub(f.extract(i32, fs, 1), e)
+        d_mant = isub(x16(f.extract(f32, fs, 0)), x16(mt))
+        r = iadd(iadd(x16(fr), imul(cu(16), band(tou(wh), cu(15)))), d_modf)
+        g_ = iadd(iadd(f.binop(sp.OpIAdd, u32, e, m.c_int(2)), imul(cu(8), isub(x16(mt), cu(8)))), d_exp)
+        bl = band(iadd(tou(f.fmul(f32, f.ext(f32, sp.GLSL_Ldexp, mt, e), k(16.0))), d_mant), cu(255))
         al = tou(f.fadd(f32, f.ext(f32, sp.GLSL_Ldexp, k(1.0), b), f.ext(f32, sp.GLSL_Ldexp, k(-0.5), m.c_int(-1))))
         inv = k(1.0 / 255.0)
         chans = [f.fmul(f32, f.op(sp.OpConvertUToF, f32, x), inv) for x in (r, g_, bl, al)]
